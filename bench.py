@@ -1,0 +1,175 @@
+"""Benchmark of the SH WFS hot path on one MI355X.
+
+A step is one frame of the 80x80 Shack-Hartmann WFS (D = 16 m, 4964 valid
+sub-apertures, 16x16 phase points zero-padded to a 32x32 transform, 4x4
+binning to 8x8-pixel spots, noise off) followed by its slopes (thresholded CoG
+with float64 moments, threshold 0.1 * max): phase screen -> spots -> detector
+image -> slopes, inputs resident in HBM.  Frames are issued back to back by a
+native loop; wall clock around K frames, synchronised on both sides.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "compass-saxoplus_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "SH WFS frames/s (spots+slopes) at 80x80 subaps vs plain FP32; slope err vs FP64"
+HBM_PEAK_GBS = 8000.0
+METHOD_NAMES = {0: "fft_fp32", 1: "dft_bf16x3", 2: "dft_fp16"}
+
+
+def algorithmic_bytes(w, nvalid):
+    """Bytes one frame must move at minimum: phase + pupil windows, tip-tilt
+    offsets, spots, detector image, per-subap flux/intensity/slopes/indices."""
+    pd, npix, nx = w._pdiam, w.npix, w.nxsub
+    return (nvalid * pd * pd * 8 + pd * pd * 4 + nvalid * npix * npix * 4
+            + (npix * nx) ** 2 * 4 + nvalid * 32)
+
+
+def sync():
+    import carma
+    carma.deviceSync()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def measure(method, cfg_name, steps, warmup, phase, thr):
+    import sutra
+    from shwfs import configs, system
+    c = configs.get(cfg_name)
+    s = system.ShSystem(c, method=method, fp64_moments=True, threshold=thr)
+    s.set_phase(phase)
+    slopes = s.rtc.d_control[0].d_centroids
+    sutra.run_frames(s.wfs, s.centro, slopes, warmup, False, True)
+    sync()
+    t0 = time.perf_counter()
+    gpu_ms = sutra.run_frames(s.wfs, s.centro, slopes, steps, False, True)
+    sync()
+    wall = time.perf_counter() - t0
+    out = dict(frames_per_s=steps / wall, ms_per_frame=1e3 * wall / steps,
+               kernel_ms=gpu_ms / steps, slopes=np.array(slopes), bincube=s.bincube())
+    return out, s
+
+
+def oracle_frame(s, phase, thr):
+    from oracle import centroid as oc
+    from oracle import wfs as ow
+    t = s.tables()
+    b, _, img = ow.comp_image(phase, t["pupil"], t["scale"], t["offsets"], t["istart"], t["jstart"],
+                              t["validsubsx"], t["validsubsy"], t["binmap"], t["fluxPerSub"],
+                              t["nphot"], t["pdiam"], t["Nfft"], t["npix"], t["nxsub"])
+    w = s.p_wfs
+    sl, _, _ = oc.tcog(img, w._validsubsx, w._validsubsy, w.npix, s.centro.offset,
+                       s.centro.scale, thr)
+    return b, sl
+
+
+def cpu_baseline(s, phase, thr, budget_s=12.0):
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle_frame(s, phase, thr)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n / el, unit="frames/s", cores=1, kind="port",
+                sample="%d frames of the 80x80 workload by the float64 NumPy oracle "
+                       "(fields, pocketfft, binning, normalisation, TCoG), %.1f s" % (n, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--methods", default="0,1,2")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path has no multi-GPU stage); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: single-GPU build, WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from shwfs import configs, system
+    import sutra  # noqa: F401
+
+    cfg = system.geometry(configs.get("large"))
+    w = cfg["p_wfs"]
+    phase = system.make_screens(cfg, 1)[0]
+
+    # threshold = 0.1 * max of the noise-free image, fixed for the run
+    probe = system.ShSystem(configs.get("large"))
+    probe.set_phase(phase)
+    probe.comp_image(noise=False)
+    thr = 0.1 * float(probe.binimg().max())
+    ref_b, ref_s = oracle_frame(probe, phase, thr)
+    del probe
+
+    results = {}
+    for m in [int(x) for x in a.methods.split(",") if x != ""]:
+        try:
+            r, s = measure(m, "large", a.steps, a.warmup, phase, thr)
+        except RuntimeError as e:
+            print("method %s unavailable: %s" % (METHOD_NAMES[m], e), file=sys.stderr)
+            continue
+        err_px = float(np.sqrt(np.mean((r["slopes"] - ref_s) ** 2)) / w.pixsize)
+        spot_err = float(np.abs(r["bincube"] - ref_b).max() / np.abs(ref_b).max())
+        results[m] = dict(frames_per_s=r["frames_per_s"], ms_per_frame=r["ms_per_frame"],
+                          kernel_us=1e3 * r["kernel_ms"], slope_rms_err_px=err_px,
+                          spot_rel_err=spot_err, system=s)
+        print("[bench] %s: %.0f frames/s, %.2f us/frame (gpu events), slope rms err %.2e px, "
+              "spot err %.2e" % (METHOD_NAMES[m], r["frames_per_s"], 1e3 * r["kernel_ms"], err_px,
+                                  spot_err), file=sys.stderr, flush=True)
+    if not results:
+        raise RuntimeError("no method ran")
+    best = max(results, key=lambda k: results[k]["frames_per_s"])
+    rb = results[best]
+    nbytes = algorithmic_bytes(w, w._nvalid)
+    achieved = nbytes / (rb["kernel_us"] * 1e-6) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="spot+TCoG fused kernel (%s)" % METHOD_NAMES[best],
+                bytes_per_launch=nbytes, avg_launch_us=round(rb["kernel_us"], 3))
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(rb["system"], phase, thr)
+    plain = results.get(0)
+    line = dict(
+        metric=METRIC, value=round(rb["frames_per_s"], 1), unit="frames/s", n_gpus=1,
+        steps=a.steps, warmup=a.warmup, ms_per_step=round(rb["ms_per_frame"], 6),
+        higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+        data="synthetic",
+        config=dict(workload="80x80 SH, D=16 m, 4964 valid subaps, 16x16 phase px, Nfft 32, "
+                             "4x4 binning to 8x8 px, noise off, TCoG thr 0.1*max fp64 moments",
+                    method=METHOD_NAMES[best], nvalid=w._nvalid, pdiam=w._pdiam, nfft=w._Nfft,
+                    npix=w.npix, screen="von Karman r0=0.16 m L0=25 m seed 1"),
+        slope_rms_err_px=rb["slope_rms_err_px"], spot_rel_err=rb["spot_rel_err"],
+        methods={METHOD_NAMES[k]: dict(frames_per_s=round(v["frames_per_s"], 1),
+                                       kernel_us=round(v["kernel_us"], 3),
+                                       slope_rms_err_px=v["slope_rms_err_px"],
+                                       spot_rel_err=v["spot_rel_err"])
+                 for k, v in results.items()},
+        speedup_vs_plain_fp32=(round(rb["frames_per_s"] / plain["frames_per_s"], 3)
+                               if plain else None),
+        roofline=roof, cpu_baseline=cpu,
+    )
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,451 @@
+// pybind11 module "sutra": the SH WFS path of the reference's sutra module.
+// Class and method names follow python_module/sutra/src/{telescope,source,
+// wfs,wfs_sh,sensors,centroider,centroider_tcog,centroider_wcog,rtc}.cpp.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <complex>
+
+#include "../sutra/sutra.hpp"
+
+namespace py = pybind11;
+
+template <typename T>
+using ArrayFStyle = py::array_t<T, py::array::f_style | py::array::forcecast>;
+
+using Centro = SutraCentroider<float, float>;
+using Rtc = SutraRtc<float, float, float>;
+using Ctrl = SutraController<float, float>;
+
+static void declare_telescope(py::module& m) {
+  py::class_<SutraTelescope>(m, "Telescope")
+      .def(py::init([](CarmaContext& c, int64_t n_pup, int64_t npos, ArrayFStyle<float>& pupil,
+                       int64_t n_pup_m, ArrayFStyle<float>& pupil_m) {
+             if (pupil.size() != n_pup * n_pup || pupil_m.size() != n_pup_m * n_pup_m)
+               throw std::runtime_error("Telescope: pupil sizes do not match n_pup / n_pup_m");
+             return std::unique_ptr<SutraTelescope>(new SutraTelescope(
+                 &c, n_pup, npos, pupil.mutable_data(), n_pup_m, pupil_m.mutable_data()));
+           }),
+           py::arg("context"), py::arg("n_pup"), py::arg("npos"), py::arg("pupil"),
+           py::arg("n_pup_m"), py::arg("pupil_m"))
+      .def_property_readonly("device", [](SutraTelescope& t) { return t.device; })
+      .def_property_readonly("pup_size", [](SutraTelescope& t) { return t.pup_size; })
+      .def_property_readonly("pup_size_m", [](SutraTelescope& t) { return t.pup_size_m; })
+      .def_property_readonly("num_eleme_pup", [](SutraTelescope& t) { return t.num_eleme_pup; })
+      .def_property_readonly("d_pupil", [](SutraTelescope& t) { return t.d_pupil; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_pupil_m", [](SutraTelescope& t) { return t.d_pupil_m; },
+                             py::return_value_policy::reference_internal);
+}
+
+static void declare_source(py::module& m) {
+  py::class_<SutraSource>(m, "Source")
+      .def_property_readonly("device", [](SutraSource& s) { return s.device; })
+      .def_property_readonly("type", [](SutraSource& s) { return s.type; })
+      .def_property_readonly("posx", [](SutraSource& s) { return s.posx; })
+      .def_property_readonly("posy", [](SutraSource& s) { return s.posy; })
+      .def_property_readonly("npts", [](SutraSource& s) { return s.npts; })
+      .def_property_readonly("mag", [](SutraSource& s) { return s.mag; })
+      .def_property_readonly("Lambda", [](SutraSource& s) { return s.lambda; })
+      .def_property_readonly("zp", [](SutraSource& s) { return s.zp; })
+      .def_property_readonly("scale", [](SutraSource& s) { return s.scale; })
+      .def_property_readonly("lgs", [](SutraSource& s) { return s.lgs; })
+      .def_property_readonly("d_phase", [](SutraSource& s) { return s.d_phase->d_screen; },
+                             py::return_value_policy::reference_internal)
+      .def("reset_phase", &SutraSource::reset_phase)
+      .def("set_phase",
+           [](SutraSource& s, ArrayFStyle<float>& data) {
+             if ((int64_t)data.size() != s.d_phase->d_screen->get_nb_elements())
+               throw std::runtime_error("set_phase: wrong phase size");
+             return s.set_phase(data.mutable_data());
+           },
+           py::arg("data"));
+}
+
+static void declare_wfs(py::module& m) {
+  py::class_<SutraWfs>(m, "Wfs")
+      .def_property_readonly("device", [](SutraWfs& w) { return w.device; })
+      .def_property_readonly("type", [](SutraWfs& w) { return w.type; })
+      .def_property_readonly("is_low_order", [](SutraWfs& w) { return w.is_low_order; })
+      .def_property_readonly("nxsub", [](SutraWfs& w) { return w.nxsub; })
+      .def_property_readonly("nvalid", [](SutraWfs& w) { return w.nvalid; })
+      .def_property_readonly("npix", [](SutraWfs& w) { return w.npix; })
+      .def_property_readonly("nrebin", [](SutraWfs& w) { return w.nrebin; })
+      .def_property_readonly("nfft", [](SutraWfs& w) { return w.nfft; })
+      .def_property_readonly("ntot", [](SutraWfs& w) { return w.ntot; })
+      .def_property_readonly("npup", [](SutraWfs& w) { return w.npup; })
+      .def_property_readonly("nphase", [](SutraWfs& w) { return w.nphase; })
+      .def_property_readonly("nmaxhr", [](SutraWfs& w) { return w.nmaxhr; })
+      .def_property_readonly("nffthr", [](SutraWfs& w) { return w.nffthr; })
+      .def_property_readonly("subapd", [](SutraWfs& w) { return w.subapd; })
+      .def_property_readonly("nphot", [](SutraWfs& w) { return w.nphot; })
+      .def_property_readonly("nphot4imat", [](SutraWfs& w) { return w.nphot4imat; })
+      .def_property_readonly("noise", [](SutraWfs& w) { return w.noise; })
+      .def_property_readonly("lgs", [](SutraWfs& w) { return w.lgs; })
+      .def_property_readonly("kernconv", [](SutraWfs& w) { return w.kernconv; })
+      .def_property_readonly("fakecam", [](SutraWfs& w) { return w.fakecam; })
+      .def_property_readonly("max_flux_per_pix", [](SutraWfs& w) { return w.max_flux_per_pix; })
+      .def_property_readonly("max_pix_value", [](SutraWfs& w) { return w.max_pix_value; })
+      .def_property_readonly("roket", [](SutraWfs& w) { return w.roket; })
+#define RO_OBJ(name) \
+  .def_property_readonly(#name, [](SutraWfs& w) { return w.name; }, py::return_value_policy::reference_internal)
+          RO_OBJ(d_camplipup) RO_OBJ(d_camplifoc) RO_OBJ(d_fttotim) RO_OBJ(d_pupil)
+              RO_OBJ(d_bincube) RO_OBJ(d_binimg) RO_OBJ(d_binimg_notnoisy) RO_OBJ(d_intensities)
+                  RO_OBJ(d_offsets) RO_OBJ(d_fluxPerSub) RO_OBJ(d_sincar) RO_OBJ(d_hrmap)
+                      RO_OBJ(d_camimg) RO_OBJ(d_dark) RO_OBJ(d_flat) RO_OBJ(d_slopes) RO_OBJ(d_gs)
+                          RO_OBJ(d_phasemap) RO_OBJ(d_ttprojmat) RO_OBJ(d_ttprojvec)
+                              RO_OBJ(d_validsubsx) RO_OBJ(d_validsubsy) RO_OBJ(d_submask)
+#undef RO_OBJ
+      .def("comp_image", &SutraWfs::comp_image, py::arg("noise") = true)
+      .def("slopes_geom", [](SutraWfs& w, int32_t type) { return w.slopes_geom(type); },
+           py::arg("type") = 0)
+      .def("fill_binimage", &SutraWfs::fill_binimage, py::arg("async") = 0)
+      .def("set_pupil",
+           [](SutraWfs& w, ArrayFStyle<float>& data) {
+             if ((int64_t)data.size() != w.d_pupil->get_nb_elements())
+               throw std::runtime_error("set_pupil: wrong dimensions");
+             return w.set_pupil(data.mutable_data());
+           },
+           py::arg("pupil"))
+      .def("set_noise", &SutraWfs::set_noise, py::arg("noise"), py::arg("seed"))
+      .def("set_fakecam", &SutraWfs::set_fakecam, py::arg("fakecam"))
+      .def("set_max_flux_per_pix", &SutraWfs::set_max_flux_per_pix, py::arg("max_flux_per_pix"))
+      .def("set_max_pix_value", &SutraWfs::set_max_pix_value, py::arg("max_pix_value"))
+      .def("set_binimg",
+           [](SutraWfs& w, ArrayFStyle<float>& d, int32_t n) { return w.set_binimg(d.mutable_data(), n); },
+           py::arg("binimg"), py::arg("nElem"))
+      .def("set_dark",
+           [](SutraWfs& w, ArrayFStyle<float>& d, int32_t n) { return w.set_dark(d.mutable_data(), n); },
+           py::arg("dark"), py::arg("nElem"))
+      .def("set_flat",
+           [](SutraWfs& w, ArrayFStyle<float>& d, int32_t n) { return w.set_flat(d.mutable_data(), n); },
+           py::arg("flat"), py::arg("nElem"));
+}
+
+static void declare_wfs_sh(py::module& m) {
+  py::class_<SutraWfsSH, SutraWfs>(m, "SHWFS")
+      .def_property_readonly("d_binmap", [](SutraWfsSH& w) { return w.d_binmap; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_validpuppixx", [](SutraWfsSH& w) { return w.d_validpuppixx; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_validpuppixy", [](SutraWfsSH& w) { return w.d_validpuppixy; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_fsamplipup", [](SutraWfsSH& w) { return w.d_fsamplipup; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_fsamplifoc", [](SutraWfsSH& w) { return w.d_fsamplifoc; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("method", &SutraWfsSH::get_method)
+      .def_property_readonly("noise_call", [](SutraWfsSH& w) { return w.noise_call; })
+      .def("set_method", &SutraWfsSH::set_method, py::arg("method"),
+           "0: FP32 FFT, 1: MFMA DFT split-BF16x3, 2: MFMA DFT FP16 with per-subap scale")
+      .def("load_arrays",
+           [](SutraWfsSH& w, ArrayFStyle<int32_t>& phasemap, ArrayFStyle<int32_t>& hrmap,
+              ArrayFStyle<int32_t>& binmap, ArrayFStyle<float>& offsets,
+              ArrayFStyle<float>& fluxPerSub, ArrayFStyle<int32_t>& validsubsx,
+              ArrayFStyle<int32_t>& validsubsy, ArrayFStyle<int32_t>& istart,
+              ArrayFStyle<int32_t>& jstart, ArrayFStyle<float>& ttprojmat,
+              ArrayFStyle<std::complex<float>>& kernel) {
+             const int64_t nv = w.nvalid_tot, np2 = w.nphase * w.nphase;
+             if (phasemap.size() < np2 * w.nvalid || binmap.size() != w.nrebin * w.nrebin * w.npix * w.npix ||
+                 offsets.size() != np2 || fluxPerSub.size() < nv || validsubsx.size() < nv ||
+                 validsubsy.size() < nv || istart.size() < w.nvalid || jstart.size() < w.nvalid ||
+                 ttprojmat.size() != np2 * 2 * w.nvalid || kernel.size() != w.ntot * w.ntot)
+               throw std::runtime_error("load_arrays: array sizes do not match the WFS");
+             return w.load_arrays(phasemap.mutable_data(), hrmap.mutable_data(),
+                                  binmap.mutable_data(), offsets.mutable_data(),
+                                  fluxPerSub.mutable_data(), validsubsx.mutable_data(),
+                                  validsubsy.mutable_data(), istart.mutable_data(),
+                                  jstart.mutable_data(), ttprojmat.mutable_data(),
+                                  reinterpret_cast<hipFloatComplex*>(kernel.mutable_data()));
+           },
+           py::arg("phasemap"), py::arg("hrmap"), py::arg("binmap"), py::arg("offsets"),
+           py::arg("fluxPerSub"), py::arg("validsubsx"), py::arg("validsubsy"), py::arg("istart"),
+           py::arg("jstart"), py::arg("ttprojmat"), py::arg("kernel"))
+      .def("comp_nphot", &SutraWfsSH::comp_nphot, py::arg("ittime"), py::arg("optthroughput"),
+           py::arg("diam"), py::arg("nxsub"), py::arg("zerop") = 0, py::arg("gsmag") = 0,
+           py::arg("lgsreturnperwatt") = 0, py::arg("laserpower") = 0)
+      .def("set_bincube",
+           [](SutraWfsSH& w, ArrayFStyle<float>& b, int32_t n) { return w.set_bincube(b.mutable_data(), n); },
+           py::arg("bincube"), py::arg("nElem"));
+}
+
+static void declare_sensors(py::module& m) {
+  py::class_<SutraSensors>(m, "Sensors")
+      .def(py::init([](CarmaContext& context, SutraTelescope* d_tel, vector<string> type,
+                       int32_t nwfs, ArrayFStyle<int64_t>& nxsub, ArrayFStyle<int64_t>& nvalid,
+                       ArrayFStyle<int64_t>& npupils, ArrayFStyle<int64_t>& npix,
+                       ArrayFStyle<int64_t>& nphase, ArrayFStyle<int64_t>& nrebin,
+                       ArrayFStyle<int64_t>& nfft, ArrayFStyle<int64_t>& ntot,
+                       ArrayFStyle<int64_t>& npup, ArrayFStyle<float>& pdiam,
+                       ArrayFStyle<float>& nphot, ArrayFStyle<float>& nphot4imat,
+                       ArrayFStyle<int32_t>& lgs, ArrayFStyle<bool>& fakecam,
+                       ArrayFStyle<int32_t>& max_flux_per_pix, ArrayFStyle<int32_t>& max_pix_value,
+                       int32_t device, bool roket) {
+             for (auto* a : {&nxsub, &nvalid, &npupils, &npix, &nphase, &nrebin, &nfft, &ntot, &npup})
+               if (a->size() < nwfs) throw std::runtime_error("Sensors: per-WFS arrays shorter than nwfs");
+             if (pdiam.size() < nwfs || nphot.size() < nwfs || nphot4imat.size() < nwfs ||
+                 lgs.size() < nwfs || fakecam.size() < nwfs || max_flux_per_pix.size() < nwfs ||
+                 max_pix_value.size() < nwfs)
+               throw std::runtime_error("Sensors: per-WFS arrays shorter than nwfs");
+             return std::unique_ptr<SutraSensors>(new SutraSensors(
+                 &context, d_tel, type, nwfs, nxsub.mutable_data(), nvalid.mutable_data(),
+                 npupils.mutable_data(), npix.mutable_data(), nphase.mutable_data(),
+                 nrebin.mutable_data(), nfft.mutable_data(), ntot.mutable_data(),
+                 npup.mutable_data(), pdiam.mutable_data(), nphot.mutable_data(),
+                 nphot4imat.mutable_data(), lgs.mutable_data(), fakecam.mutable_data(),
+                 max_flux_per_pix.mutable_data(), max_pix_value.mutable_data(), device, roket));
+           }),
+           py::arg("context"), py::arg("d_tel"), py::arg("type"), py::arg("nwfs"),
+           py::arg("nxsub"), py::arg("nvalid"), py::arg("npupils"), py::arg("npix"),
+           py::arg("nphase"), py::arg("nrebin"), py::arg("nfft"), py::arg("ntot"), py::arg("npup"),
+           py::arg("pdiam"), py::arg("nphot"), py::arg("nphot4imat"), py::arg("lgs"),
+           py::arg("fakecam"), py::arg("max_flux_per_pix"), py::arg("max_pix_value"),
+           py::arg("device"), py::arg("roket"), py::keep_alive<1, 3>())
+      .def_property_readonly("device", [](SutraSensors& s) { return s.device; })
+      .def_property_readonly("roket", [](SutraSensors& s) { return s.roket; })
+      .def_property_readonly("nsensors", [](SutraSensors& s) { return s.nsensors(); })
+      .def_property_readonly(
+          "d_wfs",
+          [](SutraSensors& s) {
+            py::list l;
+            for (auto* w : s.d_wfs) {
+              SutraWfsSH* sh = dynamic_cast<SutraWfsSH*>(w);
+              l.append(py::cast(sh, py::return_value_policy::reference));
+            }
+            return l;
+          })
+      .def_property_readonly("d_camplipup", [](SutraSensors& s) { return s.d_camplipup; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_camplifoc", [](SutraSensors& s) { return s.d_camplifoc; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_fttotim", [](SutraSensors& s) { return s.d_fttotim; },
+                             py::return_value_policy::reference_internal)
+      .def("initgs",
+           [](SutraSensors& s, ArrayFStyle<float>& xpos, ArrayFStyle<float>& ypos,
+              ArrayFStyle<float>& lambda, ArrayFStyle<float>& mag, float zerop,
+              ArrayFStyle<int64_t>& size, ArrayFStyle<float>& noise, ArrayFStyle<int64_t>& seed,
+              ArrayFStyle<float>& G, ArrayFStyle<float>& thetaML, ArrayFStyle<float>& dx,
+              ArrayFStyle<float>& dy) {
+             const py::ssize_t n = (py::ssize_t)s.nsensors();
+             for (py::ssize_t v : {xpos.size(), ypos.size(), lambda.size(), mag.size(), size.size(),
+                                   noise.size(), seed.size(), G.size(), thetaML.size(), dx.size(),
+                                   dy.size()})
+               if (v < n) throw std::runtime_error("initgs: per-WFS arrays shorter than nsensors");
+             return s.initgs(xpos.mutable_data(), ypos.mutable_data(), lambda.mutable_data(),
+                             mag.mutable_data(), zerop, size.mutable_data(), noise.mutable_data(),
+                             seed.mutable_data(), G.mutable_data(), thetaML.mutable_data(),
+                             dx.mutable_data(), dy.mutable_data());
+           },
+           py::arg("xpos"), py::arg("ypos"), py::arg("lambda_um"), py::arg("mag"), py::arg("zerop"),
+           py::arg("sizes"), py::arg("noise"), py::arg("seeds"), py::arg("G"), py::arg("thetaML"),
+           py::arg("dx"), py::arg("dy"))
+      .def("set_noise", &SutraSensors::set_noise, py::arg("nwfs"), py::arg("noise"), py::arg("seed"))
+      .def("set_field_stop",
+           [](SutraSensors& s, int32_t nwfs, ArrayFStyle<float>& fs, int32_t N) {
+             return s.set_field_stop(nwfs, fs.mutable_data(), N);
+           },
+           py::arg("nwfs"), py::arg("field_stop"), py::arg("N"));
+}
+
+static void declare_centroiders(py::module& m) {
+  py::class_<Centro>(m, "Centroider_FF")
+      .def_property_readonly("device", [](Centro& c) { return c.device; })
+      .def_property_readonly("type", [](Centro& c) { return c.get_type(); })
+      .def_property_readonly("nslopes", [](Centro& c) { return c.nslopes; })
+      .def_property_readonly("wfs", [](Centro& c) { return dynamic_cast<SutraWfsSH*>(c.wfs); },
+                             py::return_value_policy::reference)
+      .def_property_readonly("nvalid", [](Centro& c) { return c.nvalid; })
+      .def_property_readonly("nxsub", [](Centro& c) { return c.nxsub; })
+      .def_property_readonly("npix", [](Centro& c) { return c.npix; })
+      .def_property_readonly("offset", [](Centro& c) { return c.offset; })
+      .def_property_readonly("scale", [](Centro& c) { return c.scale; })
+      .def_property_readonly("filter_TT", [](Centro& c) { return c.filter_TT; })
+      .def_property_readonly("fp64_moments", [](Centro& c) { return c.fp64_moments; })
+#define RO_C(name) \
+  .def_property_readonly(#name, [](Centro& c) { return c.name; }, py::return_value_policy::reference_internal)
+          RO_C(d_bincube) RO_C(d_intensities) RO_C(d_centroids_ref) RO_C(d_img) RO_C(d_img_raw)
+              RO_C(d_validx) RO_C(d_validy) RO_C(d_dark) RO_C(d_flat) RO_C(d_lutPix)
+                  RO_C(d_validMask) RO_C(d_ref_Tip) RO_C(d_ref_Tilt) RO_C(d_TT_slopes)
+#undef RO_C
+      .def("get_cog", [](Centro& c) { return c.get_cog(); })
+      .def("load_validpos",
+           [](Centro& c, ArrayFStyle<int32_t>& vx, ArrayFStyle<int32_t>& vy, int32_t N) {
+             if (vx.size() < N || vy.size() < N) throw std::runtime_error("load_validpos: arrays shorter than N");
+             return c.load_validpos(vx.mutable_data(), vy.mutable_data(), N);
+           },
+           py::arg("validx"), py::arg("validy"), py::arg("N"))
+      .def("set_npix", &Centro::set_npix, py::arg("npix"))
+      .def("set_nxsub", &Centro::set_nxsub, py::arg("nxsub"))
+      .def("set_fp64_moments", &Centro::set_fp64_moments, py::arg("value"),
+           "sum flux and first moments in float64")
+      .def("load_img",
+           [](Centro& c, ArrayFStyle<float>& img, int32_t m_, int32_t n, int32_t loc) {
+             if (img.size() != (py::ssize_t)m_ * n) throw std::runtime_error("load_img: wrong size");
+             return c.load_img(img.mutable_data(), m_, n, loc);
+           },
+           py::arg("img"), py::arg("m"), py::arg("n"), py::arg("location"))
+      .def("load_img",
+           [](Centro& c, ArrayFStyle<float>& img, int32_t n, int32_t loc) {
+             if (img.size() != (py::ssize_t)n * n) throw std::runtime_error("load_img: wrong size");
+             return c.load_img(img.mutable_data(), n, loc);
+           },
+           py::arg("img"), py::arg("n"), py::arg("location") = -1)
+      .def("load_img",
+           [](Centro& c, CarmaObj<float>* img) { return c.load_img(img); }, py::arg("img"))
+      .def("calibrate_img", [](Centro& c) { return c.calibrate_img(); })
+      .def("set_centroids_ref",
+           [](Centro& c, ArrayFStyle<float>& r) {
+             if (r.size() != c.nslopes) throw std::runtime_error("set_centroids_ref: wrong size");
+             return c.set_centroids_ref(r.mutable_data());
+           },
+           py::arg("refslopes"))
+      .def("set_scale", &Centro::set_scale, py::arg("scale"))
+      .def("set_offset", &Centro::set_offset, py::arg("offset"))
+      .def("set_dark",
+           [](Centro& c, ArrayFStyle<float>& d, int32_t n) { return c.set_dark(d.mutable_data(), n); },
+           py::arg("dark"), py::arg("n"))
+      .def("set_flat",
+           [](Centro& c, ArrayFStyle<float>& d, int32_t n) { return c.set_flat(d.mutable_data(), n); },
+           py::arg("flat"), py::arg("n"))
+      .def("init_calib", &Centro::init_calib, py::arg("n"), py::arg("m"))
+      .def("init_img_raw", &Centro::init_img_raw, py::arg("n"), py::arg("m"))
+      .def("set_lutPix",
+           [](Centro& c, ArrayFStyle<int32_t>& l, int32_t n) { return c.set_lutPix(l.mutable_data(), n); },
+           py::arg("lutPix"), py::arg("n"));
+
+  using Cog = SutraCentroiderCog<float, float>;
+  py::class_<Cog, Centro>(m, "CentroiderCOG_FF");
+
+  using Tcog = SutraCentroiderTcog<float, float>;
+  py::class_<Tcog, Centro>(m, "CentroiderTCOG_FF")
+      .def_property_readonly("threshold", [](Tcog& c) { return c.threshold; })
+      .def("set_threshold", &Tcog::set_threshold, py::arg("thresh"));
+
+  using Wcog = SutraCentroiderWcog<float, float>;
+  py::class_<Wcog, Centro>(m, "CentroiderWCOG_FF")
+      .def_property_readonly("threshold", [](Wcog& c) { return c.threshold; })
+      .def_property_readonly("d_weights", [](Wcog& c) { return c.d_weights; },
+                             py::return_value_policy::reference_internal)
+      .def("init_weights", &Wcog::init_weights)
+      .def("load_weights",
+           [](Wcog& c, ArrayFStyle<float>& w, int32_t ndim) {
+             const py::ssize_t need = (py::ssize_t)c.npix * c.npix * (ndim == 3 ? c.nvalid : 1);
+             if (w.size() != need) throw std::runtime_error("load_weights: wrong size");
+             return c.load_weights(w.mutable_data(), ndim);
+           },
+           py::arg("weights"), py::arg("ndim"))
+      .def("set_threshold", &Wcog::set_threshold, py::arg("thresh"));
+}
+
+static void declare_rtc(py::module& m) {
+  py::class_<Ctrl>(m, "Controller_FF")
+      .def_property_readonly("type", [](Ctrl& c) { return c.type; })
+      .def_property_readonly("nslope", [](Ctrl& c) { return c.nslope; })
+      .def_property_readonly("nactu", [](Ctrl& c) { return c.nactu; })
+      .def_property_readonly("delay", [](Ctrl& c) { return c.delay; })
+      .def_property_readonly("gain", [](Ctrl& c) { return c.gain; })
+      .def_property_readonly("centro_idx", [](Ctrl& c) { return c.centro_idx; })
+      .def_property_readonly("d_centroids", [](Ctrl& c) { return c.d_centroids; },
+                             py::return_value_policy::reference_internal);
+
+  py::class_<Rtc>(m, "Rtc_FFF")
+      .def(py::init([]() { return std::unique_ptr<Rtc>(new Rtc()); }))
+      .def_property_readonly(
+          "d_centro",
+          [](Rtc& r) {
+            py::list l;
+            for (auto* c : r.d_centro) {
+              if (auto* t = dynamic_cast<SutraCentroiderTcog<float, float>*>(c))
+                l.append(py::cast(t, py::return_value_policy::reference));
+              else if (auto* w = dynamic_cast<SutraCentroiderWcog<float, float>*>(c))
+                l.append(py::cast(w, py::return_value_policy::reference));
+              else
+                l.append(py::cast(dynamic_cast<SutraCentroiderCog<float, float>*>(c),
+                                  py::return_value_policy::reference));
+            }
+            return l;
+          })
+      .def_property_readonly(
+          "d_control",
+          [](Rtc& r) {
+            py::list l;
+            for (auto* c : r.d_control) l.append(py::cast(c, py::return_value_policy::reference));
+            return l;
+          })
+      .def("add_centroider",
+           [](Rtc& r, CarmaContext& ctx, int64_t nvalid, float offset, float scale, bool filter_TT,
+              int64_t device, std::string typec, SutraWfs* wfs) {
+             return r.add_centroider(&ctx, nvalid, offset, scale, filter_TT, device, typec, wfs);
+           },
+           py::arg("context"), py::arg("nvalid"), py::arg("offset"), py::arg("scale"),
+           py::arg("filter_TT"), py::arg("device"), py::arg("typec"), py::arg("wfs"),
+           py::keep_alive<1, 9>())
+      .def("add_centroider",
+           [](Rtc& r, CarmaContext& ctx, int64_t nvalid, float offset, float scale, bool filter_TT,
+              int64_t device, std::string typec) {
+             return r.add_centroider(&ctx, nvalid, offset, scale, filter_TT, device, typec);
+           },
+           py::arg("context"), py::arg("nvalid"), py::arg("offset"), py::arg("scale"),
+           py::arg("filter_TT"), py::arg("device"), py::arg("typec"))
+      .def("add_controller",
+           [](Rtc& r, CarmaContext& ctx, std::string typec, int64_t device, float delay,
+              int32_t nslope, int32_t nactu, py::kwargs kw) {
+             std::vector<int32_t> idx;
+             int32_t ncentro = 0;
+             if (kw.contains("idx_centro")) {
+               auto a = py::array_t<int32_t, py::array::forcecast>::ensure(kw["idx_centro"]);
+               if (!a) throw std::runtime_error("add_controller: idx_centro must be an integer array");
+               idx.assign(a.data(), a.data() + a.size());
+             }
+             if (kw.contains("ncentro")) ncentro = kw["ncentro"].cast<int32_t>();
+             if ((int32_t)idx.size() < ncentro) {
+               for (int32_t i = (int32_t)idx.size(); i < ncentro; ++i) idx.push_back(i);
+             }
+             for (int32_t i : idx)
+               if (i < 0) throw std::runtime_error("add_controller: negative idx_centro");
+             return r.add_controller(&ctx, typec, device, delay, nslope, nactu, idx.data(), ncentro);
+           },
+           py::arg("context"), py::arg("typec"), py::arg("device"), py::arg("delay"),
+           py::arg("nslope"), py::arg("nactu"))
+      .def("remove_centroider", &Rtc::remove_centroider, py::arg("ncentro"))
+      .def("do_calibrate_img", &Rtc::do_calibrate_img, py::arg("ncontrol"))
+      .def("do_centroids", [](Rtc& r) { return r.do_centroids(); })
+      .def("do_centroids", [](Rtc& r, int32_t n) { return r.do_centroids(n); }, py::arg("ncontrol"))
+      .def("do_centroids", [](Rtc& r, int32_t n, bool noise) { return r.do_centroids(n, noise); },
+           py::arg("ncontrol"), py::arg("noise"));
+}
+
+PYBIND11_MODULE(sutra, m) {
+  m.doc() = "sutra: Shack-Hartmann WFS hot path on MI355X (gfx950)";
+  py::module::import("carma");  // registers the CarmaObj / CarmaContext types
+  declare_telescope(m);
+  declare_source(m);
+  declare_wfs(m);
+  declare_wfs_sh(m);
+  declare_sensors(m);
+  declare_centroiders(m);
+  declare_rtc(m);
+  m.def("comp_frame",
+        [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, bool noise) {
+          if (slopes.get_nb_elements() < 2 * w.nvalid) throw std::runtime_error("comp_frame: slope vector too short");
+          return comp_frame(&w, &c, c.d_intensities->get_data(), slopes.get_data(), noise);
+        },
+        py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("noise") = true,
+        "spots + slopes of one frame in one launch");
+  m.def("run_frames",
+        [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, int32_t n, bool noise, bool fused) {
+          if (slopes.get_nb_elements() < 2 * w.nvalid) throw std::runtime_error("run_frames: slope vector too short");
+          if (n < 0) throw std::runtime_error("run_frames: negative frame count");
+          py::gil_scoped_release nogil;
+          return run_frames(&w, &c, slopes.get_data(), n, noise, fused);
+        },
+        py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
+        py::arg("noise") = true, py::arg("fused") = true,
+        "issues nframes frames back to back; returns hipEvent-timed milliseconds");
+  m.attr("SHWFS_METHOD_FFT_FP32") = (int)SHWFS_METHOD_FFT_FP32;
+  m.attr("SHWFS_METHOD_DFT_BF16X3") = (int)SHWFS_METHOD_DFT_BF16X3;
+  m.attr("SHWFS_METHOD_DFT_FP16") = (int)SHWFS_METHOD_DFT_FP16;
+  m.attr("__version__") = shwfs_version();
+}

@@ -1,0 +1,343 @@
+// C ABI entry points of libshwfs.so, the standalone centroid kernels, the
+// counter-based noise kernel and the weight broadcast.
+#include <cstring>
+#include <mutex>
+
+#include "shwfs_device.hpp"
+#include "shwfs_internal.hpp"
+
+namespace shwfs {
+
+static thread_local char g_err[512] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------- centroiders
+// One group of G lanes per sub-aperture, 256/G sub-apertures per workgroup.
+// Pixel (x, y) of sub-aperture s is img[(x + validx[s]) + (y + validy[s]) * size],
+// guarded by idim < size*size as in the reference kernels.
+template <int G, int KIND, typename Acc>
+__global__ __launch_bounds__(256) void centroid_kernel(const float* img_in, float* img_out,
+                                                       shwfs_centroid_args a) {
+  constexpr int S = 256 / G;
+  const int slot = threadIdx.x / G, lane = threadIdx.x % G;
+  const int s = blockIdx.x * S + slot;
+  const bool valid = s < a.nvalid;
+  const int npix2 = a.npix * a.npix;
+  const long lim = (long)a.size * a.size;
+  Acc I = 0, Sx = 0, Sy = 0;
+  if (valid) {
+    const int xv = a.d_validx[s], yv = a.d_validy[s];
+    for (int p = lane; p < npix2; p += G) {
+      const int x = p % a.npix, y = p / a.npix;
+      const long idim = (long)(x + xv) + (long)(y + yv) * a.size;
+      if (idim < lim) {
+        const float v = img_in[idim];
+        float d = v;
+        if constexpr (KIND != SHWFS_CENTROID_COG) d = (v > a.threshold) ? v - a.threshold : 1.e-6f;
+        if constexpr (KIND == SHWFS_CENTROID_TCOG) {
+          if (img_out) img_out[idim] = d;
+        }
+        const float fx = (float)x, fy = (float)y;
+        if constexpr (KIND == SHWFS_CENTROID_WCOG) {
+          const float w = a.d_weights[p + s * npix2];
+          I += (Acc)d * w;
+          Sx += (Acc)d * fx * w;
+          Sy += (Acc)d * fy * w;
+        } else {
+          I += (Acc)d;
+          Sx += (Acc)d * fx;
+          Sy += (Acc)d * fy;
+        }
+      }
+    }
+  }
+  I = group_sum<G>(I);
+  Sx = group_sum<G>(Sx);
+  Sy = group_sum<G>(Sy);
+  if (valid && lane == 0) {
+    const int ix = a.slope_order == 0 ? s : 2 * s;
+    const int iy = a.slope_order == 0 ? s + a.nvalid : 2 * s + 1;
+    const float rx = a.d_ref ? a.d_ref[ix] : 0.0f;
+    const float ry = a.d_ref ? a.d_ref[iy] : 0.0f;
+    a.d_slopes[ix] = ((float)((double)Sx / ((double)I + 1.e-6)) - a.offset) * a.scale - rx;
+    a.d_slopes[iy] = ((float)((double)Sy / ((double)I + 1.e-6)) - a.offset) * a.scale - ry;
+    a.d_intensities[s] = (float)I;
+  }
+}
+
+template <int G, int KIND>
+static int32_t launch_centroid_g(const float* in, float* out, const shwfs_centroid_args& a,
+                                 hipStream_t st) {
+  constexpr int S = 256 / G;
+  const int grid = (a.nvalid + S - 1) / S;
+  if (a.fp64_moments)
+    hipLaunchKernelGGL((centroid_kernel<G, KIND, double>), dim3(grid), dim3(256), 0, st, in, out, a);
+  else
+    hipLaunchKernelGGL((centroid_kernel<G, KIND, float>), dim3(grid), dim3(256), 0, st, in, out, a);
+  SHWFS_LAUNCHED("centroid_kernel");
+  return SHWFS_OK;
+}
+
+template <int KIND>
+static int32_t launch_centroid_k(const float* in, float* out, const shwfs_centroid_args& a,
+                                 hipStream_t st) {
+  const int n = a.npix * a.npix;
+  if (n <= 4) return launch_centroid_g<4, KIND>(in, out, a, st);
+  if (n <= 8) return launch_centroid_g<8, KIND>(in, out, a, st);
+  if (n <= 16) return launch_centroid_g<16, KIND>(in, out, a, st);
+  if (n <= 32) return launch_centroid_g<32, KIND>(in, out, a, st);
+  return launch_centroid_g<64, KIND>(in, out, a, st);
+}
+
+// ---------------------------------------------------------------- noise on an array
+__global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype, float alpha,
+                                                   float beta, unsigned long long seed,
+                                                   unsigned call, unsigned domain) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+    const unsigned e = (unsigned)i;
+    if (gtype == 'P') {
+      d[i] = (float)rng_poisson((double)d[i], e, call, domain, seed);
+    } else if (gtype == 'N') {
+      d[i] = beta * d[i] + alpha * (float)rng_normal(e, call, domain, seed);
+    } else {
+      u32x4 w = rng_block(e, 0, call, domain, seed);
+      d[i] = beta * d[i] + (float)u01(w.x);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_weights_kernel(float* out, const float* in, int npix2,
+                                                           long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L)
+    out[i] = in[i % npix2];
+}
+
+__global__ __launch_bounds__(256) void calibrate_kernel(const float* raw, float* cal,
+                                                        const float* dark, const float* flat,
+                                                        const int* lut, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+    const int j = lut[i];
+    cal[i] = (j >= 0 && j < n) ? (raw[j] - dark[i]) * flat[i] : 0.0f;
+  }
+}
+
+// Geometric slopes, one group of 64 lanes per sub-aperture.
+// type 0: slope_x = alpha/n * sum_y (phi(n-1, y) - phi(0, y)), same along y.
+// type 2: slope_x = sum_k ttprojmat[k, s] * phi(k), slope_y with column s + nvalid.
+__global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float* phase, float* slopes,
+                                                          const int* pmap, const float* proj,
+                                                          int n, int nvalid, float alpha) {
+  const int slot = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int s = blockIdx.x * 4 + slot;
+  const bool valid = s < nvalid;
+  const int n2 = n * n;
+  float ax = 0.f, ay = 0.f;
+  if (valid) {
+    const int* m = pmap + (size_t)s * n2;
+    if (type == 0) {
+      for (int t = lane; t < n; t += 64) {
+        ax += phase[m[t * n + n - 1]] - phase[m[t * n]];
+        ay += phase[m[t + (n - 1) * n]] - phase[m[t]];
+      }
+    } else {
+      const float* px = proj + (size_t)s * n2;
+      const float* py = proj + (size_t)(s + nvalid) * n2;
+      for (int k = lane; k < n2; k += 64) {
+        const float v = phase[m[k]];
+        ax += px[k] * v;
+        ay += py[k] * v;
+      }
+    }
+  }
+  ax = group_sum<64>(ax);
+  ay = group_sum<64>(ay);
+  if (valid && lane == 0) {
+    if (type == 0) {
+      slopes[s] = ax / n * alpha;
+      slopes[s + nvalid] = ay / n * alpha;
+    } else {
+      slopes[s] = ax;
+      slopes[s + nvalid] = ay;
+    }
+  }
+}
+
+static int32_t check_geom(const shwfs_sh_geom* g, int method) {
+  if (!g) return fail(SHWFS_ERR_ARG, "geometry is NULL");
+  if (g->nvalid <= 0 || g->nphase <= 0 || g->npup <= 0 || g->npix <= 0 || g->nrebin <= 0 ||
+      g->nxsub <= 0)
+    return fail(SHWFS_ERR_ARG, "geometry sizes must be positive");
+  if (!(g->nfft == 8 || g->nfft == 16 || g->nfft == 32 || g->nfft == 64))
+    return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
+  if (g->ntot != g->nfft)
+    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft (enlarged HR field of view) is not implemented");
+  if (g->nphase > g->nfft) return fail(SHWFS_ERR_ARG, "nphase must be <= nfft");
+  if (g->npix * g->nrebin > g->ntot) return fail(SHWFS_ERR_ARG, "npix*nrebin must be <= ntot");
+  if (!g->d_istart || !g->d_jstart || !g->d_validsubsx || !g->d_validsubsy || !g->d_offsets ||
+      !g->d_fluxPerSub || !g->d_binmap)
+    return fail(SHWFS_ERR_STATE, "geometry tables not loaded (load_arrays not called?)");
+  if (method == SHWFS_METHOD_DFT_BF16X3 || method == SHWFS_METHOD_DFT_FP16) {
+    if (g->nfft < 16) return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path needs nfft >= 16");
+  } else if (method != SHWFS_METHOD_FFT_FP32) {
+    return fail(SHWFS_ERR_ARG, "unknown method");
+  }
+  return SHWFS_OK;
+}
+
+}  // namespace shwfs
+
+using namespace shwfs;
+
+extern "C" {
+
+int32_t shwfs_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return SHWFS_ERR_ARG;
+  strncpy(buf, g_err, len - 1);
+  buf[len - 1] = 0;
+  return SHWFS_OK;
+}
+
+const char* shwfs_version(void) { return "shwfs-mi355x 0.1 (gfx950)"; }
+
+int32_t shwfs_sh_check_geom(const shwfs_sh_geom* g, int32_t method) { return check_geom(g, method); }
+
+int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, void* stream) {
+  int32_t rc = check_geom(g, f ? f->method : 0);
+  if (rc) return rc;
+  if (!f || !f->d_phase || !f->d_pupil || !f->d_bincube || !f->d_intensities)
+    return fail(SHWFS_ERR_ARG, "frame: phase, pupil, bincube and intensities are required");
+  // every sub-aperture window must lie inside the screen: checked by the caller
+  // when the tables are loaded (the tables live on the device).
+  hipStream_t st = (hipStream_t)stream;
+  ImageParams p{};
+  p.phase = f->d_phase;
+  p.pupil = f->d_pupil;
+  p.offsets = g->d_offsets;
+  p.istart = g->d_istart;
+  p.jstart = g->d_jstart;
+  p.validsubsx = g->d_validsubsx;
+  p.validsubsy = g->d_validsubsy;
+  p.flux = g->d_fluxPerSub;
+  p.binmap = g->d_binmap;
+  p.npup = g->npup;
+  p.nvalid = g->nvalid;
+  p.pdiam = g->nphase;
+  p.npix = g->npix;
+  p.nrebin = g->nrebin;
+  p.nsub = g->npix * g->nxsub;
+  p.scale = f->scale;
+  p.nphot = f->nphot;
+  p.noise = f->noise;
+  p.seed = f->seed;
+  p.call = f->call;
+  p.bincube = f->d_bincube;
+  p.binimg = f->d_binimg;
+  p.intensities = f->d_intensities;
+  p.ckind = SHWFS_CENTROID_NONE;
+  if (f->centroid && f->centroid->kind != SHWFS_CENTROID_NONE) {
+    const shwfs_centroid_args* c = f->centroid;
+    if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_WCOG)
+      return fail(SHWFS_ERR_ARG, "unknown centroider kind");
+    if (!c->d_slopes || !c->d_intensities) return fail(SHWFS_ERR_ARG, "centroid outputs are NULL");
+    if (c->kind == SHWFS_CENTROID_WCOG && !c->d_weights)
+      return fail(SHWFS_ERR_ARG, "WCOG needs weights");
+    if (c->nvalid != g->nvalid || c->npix != g->npix)
+      return fail(SHWFS_ERR_ARG, "fused centroider sizes differ from the WFS");
+    p.ckind = c->kind;
+    p.fp64 = c->fp64_moments;
+    p.slope_order = c->slope_order;
+    p.threshold = c->threshold;
+    p.coffset = c->offset;
+    p.cscale = c->scale;
+    p.weights = c->d_weights;
+    p.ref = c->d_ref;
+    p.slopes = c->d_slopes;
+    p.cintens = c->d_intensities;
+  }
+  if (f->d_binimg && f->noise_background > 0.0f) {
+    const long n = (long)p.nsub * p.nsub;
+    rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
+    if (rc) return rc;
+  }
+  if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
+  return launch_image_dft(p, g->nfft, f->method, st);
+}
+
+int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_centroid_args* a,
+                        void* stream) {
+  if (!a || !d_img_in) return fail(SHWFS_ERR_ARG, "centroids: NULL argument");
+  if (a->nvalid <= 0 || a->npix <= 0 || a->size <= 0)
+    return fail(SHWFS_ERR_ARG, "centroids: sizes must be positive");
+  if (!a->d_validx || !a->d_validy || !a->d_slopes || !a->d_intensities)
+    return fail(SHWFS_ERR_STATE, "centroids: valid positions or outputs not set");
+  hipStream_t st = (hipStream_t)stream;
+  switch (a->kind) {
+    case SHWFS_CENTROID_COG: return launch_centroid_k<SHWFS_CENTROID_COG>(d_img_in, nullptr, *a, st);
+    case SHWFS_CENTROID_TCOG:
+      return launch_centroid_k<SHWFS_CENTROID_TCOG>(d_img_in, d_img_out, *a, st);
+    case SHWFS_CENTROID_WCOG:
+      if (!a->d_weights) return fail(SHWFS_ERR_STATE, "WCOG: weights not initialised");
+      return launch_centroid_k<SHWFS_CENTROID_WCOG>(d_img_in, nullptr, *a, st);
+    default: return fail(SHWFS_ERR_ARG, "centroids: unknown kind");
+  }
+}
+
+int32_t shwfs_fill_weights(float* d_out, const float* d_in, int32_t npix, int32_t n_total,
+                           void* stream) {
+  if (!d_out || !d_in || npix <= 0 || n_total < 0) return fail(SHWFS_ERR_ARG, "fill_weights: bad args");
+  if (n_total == 0) return SHWFS_OK;
+  long blocks = (n_total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_weights_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     d_out, d_in, npix * npix, (long)n_total);
+  SHWFS_LAUNCHED("fill_weights_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_calibrate(const float* d_raw, float* d_cal, const float* d_dark, const float* d_flat,
+                        const int32_t* d_lutPix, int64_t n, void* stream) {
+  if (!d_raw || !d_cal || !d_dark || !d_flat || !d_lutPix || n < 0)
+    return fail(SHWFS_ERR_ARG, "calibrate: bad args");
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(calibrate_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     d_raw, d_cal, d_dark, d_flat, d_lutPix, (long)n);
+  SHWFS_LAUNCHED("calibrate_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
+                          const int32_t* d_phasemap, const float* d_ttprojmat, int32_t nphase,
+                          int32_t nvalid, float alpha, void* stream) {
+  if (!d_phase || !d_slopes || !d_phasemap || nphase < 2 || nvalid <= 0)
+    return fail(SHWFS_ERR_ARG, "slopes_geom: bad args");
+  if (type != 0 && type != 2) return fail(SHWFS_ERR_UNSUPPORTED, "slopes_geom: type must be 0 or 2");
+  if (type == 2 && !d_ttprojmat) return fail(SHWFS_ERR_ARG, "slopes_geom: type 2 needs ttprojmat");
+  hipLaunchKernelGGL(slopes_geom_kernel, dim3((nvalid + 3) / 4), dim3(256), 0, (hipStream_t)stream,
+                     type, d_phase, d_slopes, d_phasemap, d_ttprojmat, nphase, nvalid, alpha);
+  SHWFS_LAUNCHED("slopes_geom_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_prng(float* d_data, int64_t n, char gtype, float alpha, float beta, uint64_t seed,
+                   uint32_t call, uint32_t domain, void* stream) {
+  if (!d_data || n < 0) return fail(SHWFS_ERR_ARG, "prng: bad args");
+  if (gtype != 'P' && gtype != 'N' && gtype != 'U') return fail(SHWFS_ERR_ARG, "prng: gtype must be P, N or U");
+  if (n > 0xFFFFFFFFLL) return fail(SHWFS_ERR_UNSUPPORTED, "prng: more than 2^32 elements");
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(prng_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, d_data,
+                     (long)n, gtype, alpha, beta, (unsigned long long)seed, call, domain);
+  SHWFS_LAUNCHED("prng_kernel");
+  return SHWFS_OK;
+}
+
+}  // extern "C"

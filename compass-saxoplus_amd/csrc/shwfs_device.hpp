@@ -1,0 +1,283 @@
+// Device-side building blocks shared by the SH WFS kernels (gfx950).
+//
+//  * Philox4x32-10 counter RNG (Salmon et al., SC'11) and the float64
+//    Poisson / Box-Muller samplers the noise stage draws with.
+//  * Phase -> (cos, sin) with a compensated Cody-Waite reduction: the float
+//    product phase*scale keeps its exact rounding error (FMA two-product), the
+//    multiple of 2*pi is removed with a two-constant split, then the HIP
+//    sincosf runs on a small argument.
+//  * In-register radix-2 FFT of compile-time size with compile-time twiddles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace shwfs {
+
+// ---------------------------------------------------------------- constexpr trig
+// Twiddle tables are generated at compile time from these (Taylor series after
+// reduction to [-pi/4, pi/4]; accurate to double rounding for the k/N angles
+// used here).
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double cx_sin_small(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_small(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+// cos/sin of 2*pi*num/den, exact octant handling (no accumulated range error).
+constexpr double cx_cos2pi(long num, long den) {
+  long n = ((num % den) + den) % den;  // angle = 2*pi*n/den in [0, 2pi)
+  // reduce to octant: 8n/den
+  long oct = (8 * n) / den;
+  double r = 2.0 * kPi * (double)(8 * n - oct * den) / (8.0 * (double)den);  // in [0, pi/4)
+  switch (oct) {
+    case 0: return cx_cos_small(r);
+    case 1: return cx_sin_small(kPi / 4 - r);
+    case 2: return -cx_sin_small(r);
+    case 3: return -cx_cos_small(kPi / 4 - r);
+    case 4: return -cx_cos_small(r);
+    case 5: return -cx_sin_small(kPi / 4 - r);
+    case 6: return cx_sin_small(r);
+    default: return cx_cos_small(kPi / 4 - r);
+  }
+}
+constexpr double cx_sin2pi(long num, long den) {
+  return cx_cos2pi(4 * num - den, 4 * den);  // sin(a) = cos(a - pi/2)
+}
+
+// Forward twiddles W_N^k = exp(-2*pi*i*k/N), k = 0..N-1.
+template <int N>
+struct Twiddle {
+  float c[N];
+  float s[N];
+  constexpr Twiddle() : c(), s() {
+    for (int k = 0; k < N; ++k) {
+      c[k] = (float)cx_cos2pi(k, N);
+      s[k] = (float)(-cx_sin2pi(k, N));
+    }
+  }
+};
+template <int N>
+__device__ constexpr Twiddle<N> kTw{};
+
+// ---------------------------------------------------------------- complex helpers
+// out = a * W_N^k (compile-time k), with the trivial angles special-cased.
+template <int N, int K>
+__device__ __forceinline__ void twiddle_mul(float& re, float& im) {
+  constexpr int k = ((K % N) + N) % N;
+  if constexpr (k == 0) {
+    return;
+  } else if constexpr (4 * k == N) {  // -i
+    float t = re;
+    re = im;
+    im = -t;
+  } else if constexpr (2 * k == N) {  // -1
+    re = -re;
+    im = -im;
+  } else if constexpr (4 * k == 3 * N) {  // +i
+    float t = re;
+    re = -im;
+    im = t;
+  } else {
+    constexpr float wr = kTw<N>.c[k];
+    constexpr float wi = kTw<N>.s[k];
+    float r = re * wr - im * wi;
+    float i = re * wi + im * wr;
+    re = r;
+    im = i;
+  }
+}
+
+template <int N, int I>
+struct TwAt {
+  __device__ __forceinline__ static void apply(float& re, float& im) { twiddle_mul<N, I>(re, im); }
+};
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr int bitrev(int x, int bits) {
+  int r = 0;
+  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+  return r;
+}
+
+// ---- compile-time unrolled radix-2 DIT butterflies via templates
+template <int M, int LEN, int I, int J>
+struct Bfly {
+  __device__ __forceinline__ static void run(float* re, float* im) {
+    constexpr int a = I + J, b = I + J + LEN / 2;
+    float vr = re[b], vi = im[b];
+    twiddle_mul<LEN, J>(vr, vi);
+    float ur = re[a], ui = im[a];
+    re[a] = ur + vr;
+    im[a] = ui + vi;
+    re[b] = ur - vr;
+    im[b] = ui - vi;
+    if constexpr (J + 1 < LEN / 2) Bfly<M, LEN, I, J + 1>::run(re, im);
+  }
+};
+template <int M, int LEN, int I>
+struct Group {
+  __device__ __forceinline__ static void run(float* re, float* im) {
+    Bfly<M, LEN, I, 0>::run(re, im);
+    if constexpr (I + LEN < M) Group<M, LEN, I + LEN>::run(re, im);
+  }
+};
+template <int M, int LEN>
+struct Stage {
+  __device__ __forceinline__ static void run(float* re, float* im) {
+    Group<M, LEN, 0>::run(re, im);
+    if constexpr (LEN * 2 <= M) Stage<M, LEN * 2>::run(re, im);
+  }
+};
+template <int M, int K>
+struct BitRev {
+  __device__ __forceinline__ static void run(const float* ir, const float* ii, float* re, float* im) {
+    constexpr int r = bitrev(K, ilog2(M));
+    re[r] = ir[K];
+    im[r] = ii[K];
+    if constexpr (K + 1 < M) BitRev<M, K + 1>::run(ir, ii, re, im);
+  }
+};
+
+// Forward DFT of M points held in registers, natural order in and out.
+template <int M>
+__device__ __forceinline__ void fft_reg(float (&re)[M], float (&im)[M]) {
+  if constexpr (M == 1) return;
+  float tr[M], ti[M];
+  BitRev<M, 0>::run(re, im, tr, ti);
+  Stage<M, 2>::run(tr, ti);
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    re[k] = tr[k];
+    im[k] = ti[k];
+  }
+}
+
+// Multiplies element x of a register array by W_N^(x*r) for a runtime r in {0,1}
+// (r = 1 applies W_N^x).  Unrolled so every twiddle is a constant.
+template <int N, int M, int X>
+struct PreTw {
+  __device__ __forceinline__ static void run(float* re, float* im) {
+    twiddle_mul<N, X>(re[X], im[X]);
+    if constexpr (X + 1 < M) PreTw<N, M, X + 1>::run(re, im);
+  }
+};
+
+// ---------------------------------------------------------------- phase -> field
+// 2*pi split for Cody-Waite: C1 = float(2*pi), C2 = float(2*pi - C1).
+__device__ __forceinline__ void phase_sincos(float phase, float scale, float offset, float& s,
+                                             float& c) {
+  constexpr float kInv2Pi = 0.15915494309189535f;
+  constexpr float kC1 = 6.28318548202514648f;        // float nearest 2*pi
+  constexpr float kC2 = -1.7484556000744883e-07f;    // 2*pi - kC1
+  float p = phase * scale;
+  float e = __builtin_fmaf(phase, scale, -p);  // exact rounding error of the product
+  float k = __builtin_rintf(p * kInv2Pi);
+  float r = __builtin_fmaf(-k, kC1, p);
+  r = __builtin_fmaf(-k, kC2, r);
+  r = (r + e) - offset;
+  sincosf(r, &s, &c);
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+    uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+__device__ __forceinline__ u32x4 rng_block(uint32_t elem, uint32_t j, uint32_t call, uint32_t domain,
+                                           uint64_t seed) {
+  return philox4x32_10(u32x4{elem, j, call, domain}, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__device__ __forceinline__ double u01(uint32_t x) {
+  return ((double)x + 0.5) * (1.0 / 4294967296.0);
+}
+
+// Box-Muller on words 0, 1 of block 0 (float64 arithmetic).
+__device__ __forceinline__ double rng_normal(uint32_t elem, uint32_t call, uint32_t domain,
+                                             uint64_t seed) {
+  u32x4 w = rng_block(elem, 0, call, domain, seed);
+  double u0 = u01(w.x), u1 = u01(w.y);
+  return sqrt(-2.0 * log(u0)) * cos(2.0 * 3.14159265358979323846 * u1);
+}
+
+// Poisson(lam): inversion for lam < 10, Hoermann's PTRS (1993) otherwise.
+__device__ __forceinline__ double rng_poisson(double lam, uint32_t elem, uint32_t call,
+                                              uint32_t domain, uint64_t seed) {
+  if (!(lam > 0.0)) return 0.0;
+  if (lam < 10.0) {
+    u32x4 w = rng_block(elem, 0, call, domain, seed);
+    double u = u01(w.x);
+    double p = exp(-lam);
+    double F = p;
+    int k = 0;
+    while (u > F && k < 200) {
+      k += 1;
+      p = p * lam / (double)k;
+      F = F + p;
+    }
+    return (double)k;
+  }
+  double slam = sqrt(lam);
+  double loglam = log(lam);
+  double b = 0.931 + 2.53 * slam;
+  double a = -0.059 + 0.02483 * b;
+  double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  double vr = 0.9277 - 3.6224 / (b - 2.0);
+  u32x4 w{0, 0, 0, 0};
+  // Bounded loop: the acceptance rate of PTRS is > 0.87 per attempt, 4096
+  // attempts cannot all be rejected in practice; the bound keeps every wave finite.
+  for (int t = 0; t < 4096; ++t) {
+    if ((t & 1) == 0) w = rng_block(elem, (uint32_t)(t >> 1), call, domain, seed);
+    uint32_t wu = (t & 1) ? w.z : w.x;
+    uint32_t wv = (t & 1) ? w.w : w.y;
+    double U = u01(wu) - 0.5;
+    double V = u01(wv);
+    double us = 0.5 - fabs(U);
+    double k = floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return k;
+    if (k < 0.0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0))
+      return k;
+  }
+  return floor(lam);
+}
+
+// ---------------------------------------------------------------- reductions
+// Sum over groups of G consecutive lanes (G a power of two <= 64).
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+}  // namespace shwfs

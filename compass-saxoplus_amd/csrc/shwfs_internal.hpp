@@ -1,0 +1,84 @@
+// Host-side helpers shared by the translation units of libshwfs.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "shwfs.h"
+
+namespace shwfs {
+
+void set_error(const char* fmt, ...);
+
+inline int32_t fail(int32_t code, const char* msg) {
+  set_error("%s", msg);
+  return code;
+}
+
+#define SHWFS_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ::shwfs::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                         __FILE__, __LINE__);                                        \
+      return SHWFS_ERR_HIP;                                                          \
+    }                                                                                \
+  } while (0)
+
+// Launch-error check right after a kernel launch.
+#define SHWFS_LAUNCHED(name)                                                         \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      ::shwfs::set_error("launch of %s failed: %s", name, hipGetErrorString(e_));    \
+      return SHWFS_ERR_HIP;                                                          \
+    }                                                                                \
+  } while (0)
+
+inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Dynamic LDS a workgroup may use on the current device (64 KiB if unknown).
+inline size_t max_lds_per_block() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    return 65536;
+  return (size_t)v;
+}
+
+// Kernel-side parameter block of the fused spot + centroid kernels.
+struct ImageParams {
+  const float* phase;
+  const float* pupil;
+  const float* offsets;
+  const int* istart;
+  const int* jstart;
+  const int* validsubsx;
+  const int* validsubsy;
+  const float* flux;
+  const int* binmap;
+  int npup, nvalid, pdiam, npix, nrebin, nsub;
+  float scale, nphot, noise;
+  unsigned long long seed;
+  unsigned call;
+  float* bincube;
+  float* binimg;
+  float* intensities;
+  // fused centroider
+  int ckind, fp64, slope_order;
+  float threshold, coffset, cscale;
+  const float* weights;
+  const float* ref;
+  float* slopes;
+  float* cintens;
+};
+
+int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
+int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s);
+int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
+                          hipStream_t s);
+
+}  // namespace shwfs

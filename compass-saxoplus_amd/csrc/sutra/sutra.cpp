@@ -1,0 +1,642 @@
+// sutra classes of the SH WFS path over the libshwfs C ABI.
+#include "sutra.hpp"
+
+#include <cmath>
+#include <iostream>
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+int64_t* dims1(int64_t* d, int64_t n) {
+  d[0] = 1;
+  d[1] = n;
+  return d;
+}
+int64_t* dims2(int64_t* d, int64_t a, int64_t b) {
+  d[0] = 2;
+  d[1] = a;
+  d[2] = b;
+  return d;
+}
+int64_t* dims3(int64_t* d, int64_t a, int64_t b, int64_t c) {
+  d[0] = 3;
+  d[1] = a;
+  d[2] = b;
+  d[3] = c;
+  return d;
+}
+template <class T>
+void del(T*& p) {
+  delete p;
+  p = nullptr;
+}
+}  // namespace
+
+// ============================================================ SutraTelescope
+// reference: libsutra/src.cpp/sutra_telescope.cpp:27-60
+SutraTelescope::SutraTelescope(CarmaContext* context, int64_t n_pup, int64_t npos, float* pupil,
+                               int64_t n_pup_m, float* pupil_m)
+    : current_context(context),
+      device(context->get_active_device()),
+      pup_size(n_pup),
+      pup_size_m(n_pup_m),
+      num_eleme_pup(npos) {
+  int64_t d[4];
+  d_pupil = new CarmaObj<float>(context, dims2(d, n_pup, n_pup), pupil);
+  d_pupil_m = new CarmaObj<float>(context, dims2(d, n_pup_m, n_pup_m), pupil_m);
+}
+
+SutraTelescope::~SutraTelescope() {
+  del(d_pupil);
+  del(d_pupil_m);
+}
+
+// ============================================================ SutraPhase / SutraSource
+SutraPhase::SutraPhase(CarmaContext* context, int64_t size) : screen_size(size) {
+  int64_t d[4];
+  d_screen = new CarmaObj<float>(context, dims2(d, size, size));
+}
+SutraPhase::~SutraPhase() { del(d_screen); }
+
+// reference: libsutra/src.cpp/sutra_source.cpp:62-100 (scale = 2*pi/lambda, phase in µm)
+SutraSource::SutraSource(CarmaContext* context, float xpos, float ypos, float lambda_, float mag_,
+                         float zerop, int64_t size, string type_, int32_t device_)
+    : current_context(context),
+      device(device_),
+      posx(xpos),
+      posy(ypos),
+      npts(size),
+      mag(mag_),
+      lambda(lambda_),
+      zp(zerop),
+      scale(float(2 * kPi / lambda_)),
+      lgs(false),
+      G(1.f),
+      thetaML(0.f),
+      dx(0.f),
+      dy(0.f),
+      type(type_) {
+  context->set_active_device(device, 1);
+  d_phase = new SutraPhase(context, size);
+}
+
+SutraSource::~SutraSource() { del(d_phase); }
+
+int32_t SutraSource::set_phase(float* data) { return d_phase->d_screen->host2device(data); }
+int32_t SutraSource::reset_phase() { return d_phase->d_screen->reset(); }
+
+// ============================================================ SutraWfs
+// reference: libsutra/src.cpp/sutra_wfs.cpp:25-96
+SutraWfs::SutraWfs(CarmaContext* context, SutraTelescope* d_tel,
+                   CarmaObj<hipFloatComplex>* camplipup, CarmaObj<hipFloatComplex>* camplifoc,
+                   CarmaObj<hipFloatComplex>* fttotim, string type_, int64_t nxsub_,
+                   int64_t nvalid_, int64_t npix_, int64_t nphase_, int64_t nrebin_, int64_t nfft_,
+                   int64_t ntot_, int64_t npup_, float pdiam, float nphotons, float nphot4imat_,
+                   int32_t lgs_, bool fakecam_, int32_t max_flux_per_pix_, int32_t max_pix_value_,
+                   bool is_low_order_, bool roket_, int32_t device_)
+    : device(device_),
+      type(type_),
+      nxsub(nxsub_),
+      nvalid(nvalid_),
+      npix(npix_),
+      nrebin(nrebin_),
+      nfft(nfft_),
+      ntot(ntot_),
+      npup(npup_),
+      nphase(nphase_),
+      nmaxhr(nvalid_),
+      nffthr(1),
+      subapd(pdiam),
+      nphot(nphotons),
+      nphot4imat(nphot4imat_),
+      noise(0),
+      lgs(lgs_ != 0),
+      kernconv(false),
+      roket(roket_),
+      is_low_order(is_low_order_),
+      fakecam(fakecam_),
+      max_flux_per_pix(max_flux_per_pix_),
+      max_pix_value(max_pix_value_),
+      campli_plan(nullptr),
+      fttotim_plan(nullptr),
+      d_ftkernel(nullptr),
+      d_camplipup(nullptr),
+      d_camplifoc(nullptr),
+      d_fttotim(nullptr),
+      d_pupil(d_tel->d_pupil_m),
+      d_bincube(nullptr),
+      d_binimg(nullptr),
+      d_binimg_notnoisy(nullptr),
+      d_intensities(nullptr),
+      d_offsets(nullptr),
+      d_fluxPerSub(nullptr),
+      d_sincar(nullptr),
+      d_hrmap(nullptr),
+      d_camimg(nullptr),
+      d_dark(nullptr),
+      d_flat(nullptr),
+      d_slopes(nullptr),
+      d_gs(nullptr),
+      d_phasemap(nullptr),
+      d_ttprojmat(nullptr),
+      d_ttprojvec(nullptr),
+      d_validsubsx(nullptr),
+      d_validsubsy(nullptr),
+      d_submask(nullptr),
+      current_context(context),
+      offset(0),
+      nvalid_tot((int32_t)nvalid_),
+      rank(0),
+      stream(nullptr) {
+  if (camplipup != nullptr && !is_low_order) d_camplipup = camplipup;
+  if (camplifoc != nullptr && !is_low_order) d_camplifoc = camplifoc;
+  if (fttotim != nullptr && !is_low_order) d_fttotim = fttotim;
+}
+
+SutraWfs::~SutraWfs() {}
+
+int32_t SutraWfs::wfs_initgs(float xpos, float ypos, float lambda, float mag, float zerop,
+                             int64_t size, float noise_, int64_t seed, float G, float thetaML,
+                             float dx, float dy) {
+  current_context->set_active_device(device, 1);
+  if (lgs) throw std::runtime_error("LGS wavefront sensors are not part of this build");
+  d_gs = new SutraSource(current_context, xpos, ypos, lambda, mag, zerop, size, "wfs", device);
+  d_gs->G = G;
+  d_gs->thetaML = thetaML;
+  d_gs->dx = dx;
+  d_gs->dy = dy;
+  return set_noise(noise_, seed);
+}
+
+// reference: libsutra/src.cpp/sutra_wfs.cpp:123-141 (init_prng(seed) of bincube and,
+// for readout noise, of binimg; here one Philox key per WFS, call counter reset).
+int32_t SutraWfs::set_noise(float noise_, int64_t seed) {
+  current_context->set_active_device(device, 1);
+  noise = noise_;
+  if (noise > -1 && d_bincube) d_bincube->init_prng(seed);
+  if (noise > 0 && d_binimg) d_binimg->init_prng(seed);
+  auto* sh = dynamic_cast<SutraWfsSH*>(this);
+  if (sh) {
+    sh->noise_seed = (uint64_t)seed;
+    sh->noise_call = 0;
+  }
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfs::set_pupil(float* pupil) {
+  current_context->set_active_device(device, 1);
+  return d_pupil->host2device(pupil);
+}
+
+// reference: libsutra/src.cpp/sutra_wfs.cpp:247-261 (wrong size only prints)
+int32_t SutraWfs::set_binimg(float* binimg, int32_t nElem) {
+  current_context->set_active_device(device, 1);
+  if (nElem == d_binimg->get_nb_elements())
+    d_binimg->host2device(binimg);
+  else
+    std::cerr << "SutraWfs::set_binimg: wrong dimension of binimg" << std::endl;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfs::set_dark(float* dark, int32_t nElem) {
+  if (d_dark == nullptr) {
+    std::cout << "WFS d_dark array has not been initialized yet" << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (nElem == d_dark->get_nb_elements()) d_dark->host2device(dark);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfs::set_flat(float* flat, int32_t nElem) {
+  if (d_flat == nullptr) {
+    std::cout << "WFS d_flat array has not been initialized yet" << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (nElem == d_flat->get_nb_elements()) d_flat->host2device(flat);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfs::set_fakecam(bool f) {
+  if (f) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
+  fakecam = f;
+  return EXIT_SUCCESS;
+}
+int32_t SutraWfs::set_max_flux_per_pix(int32_t v) {
+  max_flux_per_pix = v;
+  return EXIT_SUCCESS;
+}
+int32_t SutraWfs::set_max_pix_value(int32_t v) {
+  max_pix_value = v;
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_wfs.cpp:199-245
+int32_t SutraWfs::slopes_geom(float* slopes, int32_t type_) {
+  current_context->set_active_device(device, 1);
+  if (!d_phasemap) throw std::runtime_error("slopes_geom: arrays not loaded");
+  float alpha = float(0.206265 / subapd * nphase / (nphase - 1));
+  shwfs_throw_if(shwfs_slopes_geom(type_, d_gs->d_phase->d_screen->get_data(), slopes,
+                                   d_phasemap->get_data(), d_ttprojmat->get_data(),
+                                   (int32_t)nphase, (int32_t)nvalid, alpha, stream),
+                 "slopes_geom");
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfs::slopes_geom(int32_t type_) { return slopes_geom(d_slopes->get_data(), type_); }
+
+// ============================================================ SutraWfsSH
+SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
+                       CarmaObj<hipFloatComplex>* camplipup, CarmaObj<hipFloatComplex>* camplifoc,
+                       CarmaObj<hipFloatComplex>* fttotim, int64_t nxsub_, int64_t nvalid_,
+                       int64_t npix_, int64_t nphase_, int64_t nrebin_, int64_t nfft_,
+                       int64_t ntot_, int64_t npup_, float pdiam, float nphotons,
+                       float nphot4imat_, int32_t lgs_, bool fakecam_, int32_t max_flux_per_pix_,
+                       int32_t max_pix_value_, bool is_low_order_, bool roket_, int32_t device_)
+    : SutraWfs(context, d_tel, camplipup, camplifoc, fttotim, "sh", nxsub_, nvalid_, npix_,
+               nphase_, nrebin_, nfft_, ntot_, npup_, pdiam, nphotons, nphot4imat_, lgs_,
+               fakecam_, max_flux_per_pix_, max_pix_value_, is_low_order_, roket_, device_),
+      d_binmap(nullptr),
+      d_validpuppixx(nullptr),
+      d_validpuppixy(nullptr),
+      d_fsamplipup(nullptr),
+      d_fsamplifoc(nullptr),
+      fsampli_plan(nullptr),
+      noise_seed(1234),
+      noise_call(0),
+      arrays_loaded(false),
+      own_plan(false),
+      noise_this_call(true) {
+  if (fakecam_) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
+}
+
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:48-75 (vestigial MPI split;
+// kept for API completeness, the kernels always see the local nvalid).
+int32_t SutraWfsSH::define_mpi_rank(int32_t rank_, int32_t size) {
+  if (size <= 0) throw std::runtime_error("define_mpi_rank: size must be positive");
+  rank = rank_;
+  int32_t r = (int32_t)(nvalid % size);
+  if (rank_ < r) {
+    nvalid = nvalid / size + 1;
+    offset = rank_ * (int32_t)nvalid;
+  } else {
+    nvalid = nvalid / size;
+    offset = rank_ * (int32_t)nvalid + r;
+  }
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:77-271
+int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_plans) {
+  (void)fttotim_plans;
+  current_context->set_active_device(device, 1);
+  if (ntot != nfft)
+    throw std::runtime_error("Ntot != Nfft (enlarged HR field of view) is not part of this build");
+  int64_t d[5];
+  if (rank == 0) {
+    d_binimg = new CarmaObj<float>(current_context, dims2(d, npix * nxsub, npix * nxsub));
+    if (roket) d_binimg_notnoisy = new CarmaObj<float>(current_context, dims2(d, npix * nxsub, npix * nxsub));
+  }
+  if (d_camplipup == nullptr)
+    d_camplipup = new CarmaObj<hipFloatComplex>(current_context, dims3(d, nfft, nfft, nvalid));
+  if (d_camplifoc == nullptr)
+    d_camplifoc = new CarmaObj<hipFloatComplex>(current_context, dims3(d, nfft, nfft, nvalid));
+
+  vector<int32_t> vdims{(int32_t)nfft, (int32_t)nfft, (int32_t)nvalid};
+  auto it = campli_plans.find(vdims);
+  if (it == campli_plans.end()) {
+    ShwfsPlan* plan = new ShwfsPlan();
+    plan->nfft = (int32_t)nfft;
+    plan->nvalid = (int32_t)nvalid;
+    plan->method = SHWFS_METHOD_FFT_FP32;
+    campli_plans[vdims] = plan;
+    campli_plan = plan;
+  } else {
+    campli_plan = it->second;
+  }
+
+  d_bincube = new CarmaObj<float>(current_context, dims3(d, npix, npix, nvalid_tot));
+  d_slopes = new CarmaObj<float>(current_context, dims1(d, 2 * nvalid));
+  d_ftkernel = new CarmaObj<hipFloatComplex>(current_context, dims2(d, ntot, ntot));
+  d_offsets = new CarmaObj<float>(current_context, dims2(d, nphase, nphase));
+  d_binmap = new CarmaObj<int32_t>(current_context, dims2(d, nrebin * nrebin, npix * npix));
+  d_intensities = new CarmaObj<float>(current_context, dims1(d, nvalid_tot));
+  d_fluxPerSub = new CarmaObj<float>(current_context, dims1(d, nvalid_tot));
+  d_validsubsx = new CarmaObj<int32_t>(current_context, dims1(d, nvalid_tot));
+  d_validsubsy = new CarmaObj<int32_t>(current_context, dims1(d, nvalid_tot));
+  d_validpuppixx = new CarmaObj<int32_t>(current_context, dims1(d, nvalid_tot));
+  d_validpuppixy = new CarmaObj<int32_t>(current_context, dims1(d, nvalid_tot));
+  d_phasemap = new CarmaObj<int32_t>(current_context, dims2(d, nphase * nphase, nvalid));
+  d_ttprojmat = new CarmaObj<float>(current_context, dims2(d, nphase * nphase, nvalid * 2));
+  d_ttprojvec = new CarmaObj<float>(current_context, dims1(d, nphase * nphase * nvalid));
+  return EXIT_SUCCESS;
+}
+
+SutraWfsSH::~SutraWfsSH() {
+  current_context->set_active_device(device, 1);
+  del(d_ftkernel);
+  del(d_bincube);
+  del(d_binimg);
+  del(d_binimg_notnoisy);
+  del(d_intensities);
+  del(d_offsets);
+  del(d_fluxPerSub);
+  del(d_sincar);
+  del(d_hrmap);
+  del(d_slopes);
+  del(d_phasemap);
+  del(d_ttprojmat);
+  del(d_ttprojvec);
+  del(d_binmap);
+  del(d_validsubsx);
+  del(d_validsubsy);
+  del(d_validpuppixx);
+  del(d_validpuppixy);
+  del(d_submask);
+  del(d_fsamplipup);
+  del(d_fsamplifoc);
+  del(d_gs);
+  // shared scratch (d_camplipup/d_camplifoc/d_fttotim) belongs to SutraSensors
+  // unless this WFS created it
+}
+
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:308-331
+int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binmap,
+                                float* offsets, float* fluxPerSub, int32_t* validsubsx,
+                                int32_t* validsubsy, int32_t* istart, int32_t* jstart,
+                                float* ttprojmat, hipFloatComplex* kernel) {
+  (void)hrmap;
+  if (d_bincube == nullptr)
+    throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
+  current_context->set_active_device(device, 1);
+  // host-side bounds checks of every table the kernels index with: a
+  // sub-aperture window must lie in the screen, a spot in the image, a binmap
+  // entry in the HR grid.
+  const int64_t nsub = npix * nxsub;
+  for (int64_t s = 0; s < nvalid; ++s) {
+    const int64_t i0 = istart[s], j0 = jstart[s];
+    if (i0 < 0 || j0 < 0 || i0 + nphase > npup || j0 + nphase > npup)
+      throw std::runtime_error("load_arrays: sub-aperture window outside the phase screen");
+    const int64_t vx = validsubsx[offset + s], vy = validsubsy[offset + s];
+    if (vx < 0 || vy < 0 || vx + npix > nsub || vy + npix > nsub)
+      throw std::runtime_error("load_arrays: spot window outside the detector image");
+  }
+  for (int64_t i = 0; i < nrebin * nrebin * npix * npix; ++i)
+    if (binmap[i] < 0 || binmap[i] >= nfft * nfft)
+      throw std::runtime_error("load_arrays: binmap entry outside the HR grid");
+  for (int64_t i = 0; i < nphase * nphase * nvalid; ++i)
+    if (phasemap[offset * nphase * nphase + i] < 0 || phasemap[offset * nphase * nphase + i] >= npup * npup)
+      throw std::runtime_error("load_arrays: phasemap entry outside the phase screen");
+  d_phasemap->host2device(&phasemap[offset * nphase * nphase]);
+  d_ttprojmat->host2device(ttprojmat);
+  d_offsets->host2device(offsets);
+  d_binmap->host2device(binmap);
+  d_fluxPerSub->host2device(&fluxPerSub[offset]);
+  d_validsubsx->host2device(&validsubsx[offset]);
+  d_validsubsy->host2device(&validsubsy[offset]);
+  d_validpuppixx->host2device(istart);
+  d_validpuppixy->host2device(jstart);
+  d_ftkernel->host2device(kernel);
+  arrays_loaded = true;
+  return EXIT_SUCCESS;
+}
+
+shwfs_sh_geom SutraWfsSH::geom() const {
+  shwfs_sh_geom g{};
+  g.nvalid = (int32_t)nvalid;
+  g.nphase = (int32_t)nphase;
+  g.npup = (int32_t)d_pupil->get_dims(1);
+  g.nfft = (int32_t)nfft;
+  g.ntot = (int32_t)ntot;
+  g.npix = (int32_t)npix;
+  g.nrebin = (int32_t)nrebin;
+  g.nxsub = (int32_t)nxsub;
+  if (arrays_loaded) {
+    g.d_istart = d_validpuppixx->get_data();
+    g.d_jstart = d_validpuppixy->get_data();
+    g.d_validsubsx = d_validsubsx->get_data();
+    g.d_validsubsy = d_validsubsy->get_data();
+    g.d_offsets = d_offsets->get_data();
+    g.d_fluxPerSub = d_fluxPerSub->get_data();
+    g.d_binmap = d_binmap->get_data();
+  }
+  return g;
+}
+
+shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
+  shwfs_frame_args f{};
+  f.d_phase = d_gs ? d_gs->d_phase->d_screen->get_data() : nullptr;
+  f.d_pupil = d_pupil->get_data();
+  f.scale = d_gs ? d_gs->scale : 0.f;
+  f.nphot = nphot;
+  f.noise = with_noise ? noise : -1.0f;
+  f.noise_background = noise;  // fill_binimage uses the WFS noise even for comp_image(false)
+  f.seed = noise_seed;
+  f.call = noise_call;
+  f.method = get_method();
+  f.d_bincube = d_bincube->get_data();
+  f.d_binimg = d_binimg ? d_binimg->get_data() : nullptr;
+  f.d_intensities = d_intensities->get_data();
+  f.centroid = nullptr;
+  return f;
+}
+
+// One fused launch: fillcamplipup + FFT + abs2c + fillbincube + subap_reduce +
+// subap_norm + prng 'P' / 'N' + (readout background) + fillbinimg
+// (reference sequence: libsutra/src.cpp/sutra_wfs_sh.cpp:337-621).
+int32_t SutraWfsSH::comp_generic() {
+  if (d_bincube == nullptr)
+    throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
+  if (!arrays_loaded) throw std::runtime_error("comp_image: load_arrays has not been called");
+  if (d_gs == nullptr) throw std::runtime_error("comp_image: initgs has not been called");
+  if (d_gs->d_phase->d_screen->get_dims(1) != d_pupil->get_dims(1))
+    throw std::runtime_error("comp_image: phase screen and pupil sizes differ");
+  current_context->set_active_device(device, 1);
+  shwfs_sh_geom g = geom();
+  shwfs_frame_args f = frame_args(noise_this_call);
+  shwfs_throw_if(shwfs_sh_comp_image(&g, &f, stream), "SutraWfsSH::comp_image");
+  noise_call++;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfsSH::fill_binimage(int32_t async) {
+  (void)async;
+  // comp_generic already scattered the spots (and the background readout
+  // noise when noise > 0) into d_binimg in the same launch.
+  if (d_binimg == nullptr)
+    throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:623-643
+int32_t SutraWfsSH::comp_image(bool noise_) {
+  current_context->set_active_device(device, 1);
+  noise_this_call = noise_;
+  int32_t result = comp_generic();
+  noise_this_call = true;
+  result *= fill_binimage(0);
+  return result;
+}
+
+int32_t SutraWfsSH::comp_nphot(float ittime, float optthroughput, float diam, int32_t nxsub_,
+                               float zerop, float gsmag, float lgsreturnperwatt,
+                               float laserpower) {
+  d_gs->mag = gsmag;
+  if (laserpower == 0) {
+    if (zerop == 0) zerop = 1e11;
+    nphot = zerop * pow(10., (-0.4 * gsmag)) * ittime * optthroughput * (diam / nxsub_) *
+            (diam / nxsub_);
+  } else {
+    nphot = lgsreturnperwatt * laserpower * optthroughput * (diam / nxsub_) * (diam / nxsub_) *
+            1e4 * ittime;
+  }
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfsSH::set_bincube(float* bincube, int32_t nElem) {
+  current_context->set_active_device(device, 1);
+  if (nElem == d_bincube->get_nb_elements())
+    d_bincube->host2device(bincube);
+  else
+    std::cerr << "SutraWfsSH::set_bincube: wrong size of cube" << std::endl;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraWfsSH::set_field_stop(PlanMap& campli_plans, float* field_stop, int32_t N) {
+  (void)campli_plans;
+  (void)field_stop;
+  (void)N;
+  throw std::runtime_error("field stops are not part of this build");
+}
+
+int32_t SutraWfsSH::set_method(int32_t method) {
+  if (method != SHWFS_METHOD_FFT_FP32 && method != SHWFS_METHOD_DFT_BF16X3 &&
+      method != SHWFS_METHOD_DFT_FP16)
+    throw std::runtime_error("set_method: unknown method");
+  if (method != SHWFS_METHOD_FFT_FP32 && nfft < 16)
+    throw std::runtime_error("set_method: the MFMA DFT path needs nfft >= 16");
+  if (!campli_plan) throw std::runtime_error("set_method: allocate_buffers has not run");
+  // a plan may be shared by WFS of identical shape: give this WFS its own
+  if (!own_plan) {
+    campli_plan = new ShwfsPlan(*campli_plan);
+    own_plan = true;
+  }
+  campli_plan->method = method;
+  return EXIT_SUCCESS;
+}
+
+// ============================================================ SutraSensors
+// reference: libsutra/src.cpp/sutra_sensors.cpp:25-135
+SutraSensors::SutraSensors(CarmaContext* context, SutraTelescope* d_tel, vector<string> type,
+                           int32_t nwfs, int64_t* nxsub, int64_t* nvalid, int64_t* npupils,
+                           int64_t* npix, int64_t* nphase, int64_t* nrebin, int64_t* nfft,
+                           int64_t* ntot, int64_t* npup, float* pdiam, float* nphot,
+                           float* nphot4imat, int32_t* lgs, bool* fakecam,
+                           int32_t* max_flux_per_pix, int32_t* max_pix_value, int32_t device_,
+                           bool roket_)
+    : device(device_),
+      roket(roket_),
+      current_context(context),
+      d_camplipup(nullptr),
+      d_camplifoc(nullptr),
+      d_fttotim(nullptr),
+      d_ftlgskern(nullptr),
+      d_lgskern(nullptr) {
+  (void)npupils;
+  current_context->set_active_device(device, 1);
+  if ((int32_t)type.size() < nwfs) throw std::runtime_error("Sensors: fewer types than nwfs");
+  if (nwfs > 0 && type[0] == "sh") {
+    int64_t maxnfft = nfft[0], maxnvalid = nvalid[0];
+    for (int32_t i = 1; i < nwfs; i++) {
+      if (type[i] != "shlo") {
+        maxnvalid = std::max(maxnvalid, nvalid[i]);
+        maxnfft = std::max(maxnfft, nfft[i]);
+      }
+    }
+    int64_t d[5];
+    // Shared complex scratch of the reference's API; the fused kernels do not use it.
+    d_camplifoc = new CarmaObj<hipFloatComplex>(context, dims3(d, maxnfft, maxnfft, maxnvalid));
+    d_camplipup = new CarmaObj<hipFloatComplex>(context, dims3(d, maxnfft, maxnfft, maxnvalid));
+  }
+  for (int32_t i = 0; i < nwfs; i++) {
+    SutraWfs* wfs = nullptr;
+    if (type[i] == "sh" || type[i] == "shlo") {
+      wfs = new SutraWfsSH(context, d_tel, d_camplipup, d_camplifoc, d_fttotim, nxsub[i],
+                           nvalid[i], npix[i], nphase[i], nrebin[i], nfft[i], ntot[i], npup[i],
+                           pdiam[i], nphot[i], nphot4imat[i], lgs[i], fakecam[i],
+                           max_flux_per_pix[i], max_pix_value[i], type[i] == "shlo", roket,
+                           device);
+    } else {
+      throw std::runtime_error("Sensors: only Shack-Hartmann WFS (\"sh\") are part of this build");
+    }
+    d_wfs.push_back(wfs);
+  }
+  allocate_buffers();
+}
+
+SutraSensors::~SutraSensors() {
+  current_context->set_active_device(device, 1);
+  std::vector<ShwfsPlan*> owned;
+  for (auto& kv : campli_plans) owned.push_back(kv.second);
+  while (!d_wfs.empty()) {
+    SutraWfsSH* sh = dynamic_cast<SutraWfsSH*>(d_wfs.back());
+    if (sh && sh->own_plan) delete sh->campli_plan;
+    if (sh && sh->d_camplipup != d_camplipup) delete sh->d_camplipup;
+    if (sh && sh->d_camplifoc != d_camplifoc) delete sh->d_camplifoc;
+    delete d_wfs.back();
+    d_wfs.pop_back();
+  }
+  for (auto* p : owned) delete p;
+  del(d_camplifoc);
+  del(d_camplipup);
+  del(d_fttotim);
+  del(d_ftlgskern);
+  del(d_lgskern);
+}
+
+int32_t SutraSensors::allocate_buffers() {
+  current_context->set_active_device(device, 1);
+  for (auto* w : d_wfs) w->allocate_buffers(campli_plans, fttotim_plans);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSensors::define_mpi_rank(int32_t rank, int32_t size) {
+  for (auto* w : d_wfs) w->define_mpi_rank(rank, size);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSensors::initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop,
+                             int64_t* size, float* noise, int64_t* seed, float* G, float* thetaML,
+                             float* dx, float* dy) {
+  current_context->set_active_device(device, 1);
+  for (size_t i = 0; i < d_wfs.size(); i++)
+    d_wfs[i]->wfs_initgs(xpos[i], ypos[i], lambda[i], mag[i], zerop, size[i], noise[i], seed[i],
+                         G[i], thetaML[i], dx[i], dy[i]);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSensors::initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop,
+                             int64_t* size, float* noise, float* G, float* thetaML, float* dx,
+                             float* dy) {
+  for (size_t i = 0; i < d_wfs.size(); i++)
+    d_wfs[i]->wfs_initgs(xpos[i], ypos[i], lambda[i], mag[i], zerop, size[i], noise[i],
+                         1234 + (int64_t)i, G[i], thetaML[i], dx[i], dy[i]);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSensors::initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop,
+                             int64_t* size, float* G, float* thetaML, float* dx, float* dy) {
+  for (size_t i = 0; i < d_wfs.size(); i++)
+    d_wfs[i]->wfs_initgs(xpos[i], ypos[i], lambda[i], mag[i], zerop, size[i], -1,
+                         1234 + (int64_t)i, G[i], thetaML[i], dx[i], dy[i]);
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSensors::set_noise(int32_t nwfs, float noise, int64_t seed) {
+  return d_wfs.at(nwfs)->set_noise(noise, seed);
+}
+
+int32_t SutraSensors::set_field_stop(int32_t nwfs, float* field_stop, int32_t N) {
+  SutraWfsSH* wfs = dynamic_cast<SutraWfsSH*>(d_wfs.at(nwfs));
+  if (wfs) return wfs->set_field_stop(campli_plans, field_stop, N);
+  return EXIT_SUCCESS;
+}

@@ -1,0 +1,423 @@
+// sutra: the reference's C++ class API for the SH WFS path, re-implemented
+// over the libshwfs C ABI.
+//
+// Class and method names and argument lists follow the reference
+// (libsutra/include/sutra_{telescope,source,wfs,wfs_sh,sensors,centroider,
+// centroider_cog,centroider_wcog,centroider_tcog,rtc}.hpp).  What changes:
+//  * the FFT plan maps keyed by {nfft, nfft, nvalid} hold a project plan
+//    (ShwfsPlan: method + tile choice) instead of a vendor FFT handle;
+//  * complex arrays are CarmaObj<hipFloatComplex>;
+//  * comp_image is one fused launch (plus a background-noise launch when the
+//    WFS has readout noise); d_camplipup / d_camplifoc exist for the API but
+//    the fused kernels never materialise the complex cubes;
+//  * errors raise std::runtime_error instead of DEBUG-only throws.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../carma/carma.hpp"
+
+using std::map;
+using std::string;
+using std::vector;
+
+// Execution plan of one SH WFS spot computation (replaces cufftHandle*).
+struct ShwfsPlan {
+  int32_t nfft = 0;
+  int32_t nvalid = 0;
+  int32_t method = SHWFS_METHOD_FFT_FP32;
+};
+using PlanMap = map<vector<int32_t>, ShwfsPlan*>;
+
+class SutraTelescope {
+ public:
+  CarmaContext* current_context;
+  int32_t device;
+  int64_t pup_size;
+  int64_t pup_size_m;
+  int64_t num_eleme_pup;
+  CarmaObj<float>* d_pupil;
+  CarmaObj<float>* d_pupil_m;
+
+  SutraTelescope(CarmaContext* context, int64_t n_pup, int64_t npos, float* pupil, int64_t n_pup_m,
+                 float* pupil_m);
+  ~SutraTelescope();
+};
+
+class SutraPhase {
+ public:
+  CarmaObj<float>* d_screen;
+  int64_t screen_size;
+  SutraPhase(CarmaContext* context, int64_t size);
+  ~SutraPhase();
+};
+
+// Guide-star source: holds the phase seen by the WFS (µm) and scale = 2*pi/lambda
+// (reference: libsutra/src.cpp/sutra_source.cpp:25-100).  Raytracing through
+// atmosphere / DM layers is not part of this build.
+class SutraSource {
+ public:
+  CarmaContext* current_context;
+  int32_t device;
+  float posx, posy;
+  int64_t npts;
+  float mag, lambda, zp, scale;
+  bool lgs;
+  float G, thetaML, dx, dy;
+  string type;
+  SutraPhase* d_phase;
+
+  SutraSource(CarmaContext* context, float xpos, float ypos, float lambda, float mag, float zerop,
+              int64_t size, string type, int32_t device);
+  ~SutraSource();
+  int32_t set_phase(float* data);
+  int32_t reset_phase();
+};
+
+class SutraWfs {
+ public:
+  int32_t device;
+  string type;
+  int64_t nxsub, nvalid, npix, nrebin, nfft, ntot, npup, nphase, nmaxhr, nffthr;
+  float subapd, nphot, nphot4imat, noise;
+  bool lgs, kernconv, roket, is_low_order;
+  bool fakecam;
+  int32_t max_flux_per_pix, max_pix_value;
+
+  ShwfsPlan* campli_plan;
+  ShwfsPlan* fttotim_plan;
+  CarmaObj<hipFloatComplex>* d_ftkernel;
+  CarmaObj<hipFloatComplex>* d_camplipup;
+  CarmaObj<hipFloatComplex>* d_camplifoc;
+  CarmaObj<hipFloatComplex>* d_fttotim;
+
+  CarmaObj<float>* d_pupil;
+  CarmaObj<float>* d_bincube;
+  CarmaObj<float>* d_binimg;
+  CarmaObj<float>* d_binimg_notnoisy;
+  CarmaObj<float>* d_intensities;
+  CarmaObj<float>* d_offsets;
+  CarmaObj<float>* d_fluxPerSub;
+  CarmaObj<float>* d_sincar;
+  CarmaObj<int32_t>* d_hrmap;
+  CarmaObj<uint16_t>* d_camimg;
+  CarmaObj<float>* d_dark;
+  CarmaObj<float>* d_flat;
+  CarmaObj<float>* d_slopes;
+
+  SutraSource* d_gs;
+  CarmaObj<int32_t>* d_phasemap;
+  CarmaObj<float>* d_ttprojmat;
+  CarmaObj<float>* d_ttprojvec;
+  CarmaObj<int32_t>* d_validsubsx;
+  CarmaObj<int32_t>* d_validsubsy;
+  CarmaObj<float>* d_submask;
+
+  CarmaContext* current_context;
+
+  int32_t offset, nvalid_tot, rank;
+
+  // Stream the WFS launches on (nullptr = default stream, as the reference).
+  hipStream_t stream;
+
+ public:
+  virtual ~SutraWfs();
+
+  int32_t wfs_initgs(float xpos, float ypos, float lambda, float mag, float zerop, int64_t size,
+                     float noise, int64_t seed, float G, float thetaML, float dx, float dy);
+  int32_t set_pupil(float* pupil);
+  int32_t set_binimg(float* binimg, int32_t nElem);
+  int32_t set_dark(float* dark, int32_t nElem);
+  int32_t set_flat(float* flat, int32_t nElem);
+  int32_t set_fakecam(bool fakecam);
+  int32_t set_max_flux_per_pix(int32_t max_flux_per_pix);
+  int32_t set_max_pix_value(int32_t max_pix_value);
+  int32_t slopes_geom(float* slopes, int32_t type = 0);
+  int32_t slopes_geom(int32_t type = 0);
+  int32_t set_noise(float noise, int64_t seed);
+
+  virtual int32_t fill_binimage(int32_t async) = 0;
+  virtual int32_t comp_image(bool noise = true) = 0;
+  virtual int32_t define_mpi_rank(int32_t rank, int32_t size) = 0;
+  virtual int32_t allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_plans) = 0;
+
+ protected:
+  virtual int32_t comp_generic() = 0;
+  SutraWfs(CarmaContext* context, SutraTelescope* d_tel, CarmaObj<hipFloatComplex>* d_camplipup,
+           CarmaObj<hipFloatComplex>* d_camplifoc, CarmaObj<hipFloatComplex>* d_fttotim,
+           string type, int64_t nxsub, int64_t nvalid, int64_t npix, int64_t nphase,
+           int64_t nrebin, int64_t nfft, int64_t ntot, int64_t npup, float pdiam, float nphotons,
+           float nphot4imat, int32_t lgs, bool fakecam, int32_t max_flux_per_pix,
+           int32_t max_pix_value, bool is_low_order, bool roket, int32_t device);
+};
+
+class SutraWfsSH : public SutraWfs {
+ public:
+  CarmaObj<int32_t>* d_binmap;
+  CarmaObj<int32_t>* d_validpuppixx;
+  CarmaObj<int32_t>* d_validpuppixy;
+  CarmaObj<hipFloatComplex>* d_fsamplipup;
+  CarmaObj<hipFloatComplex>* d_fsamplifoc;
+  ShwfsPlan* fsampli_plan;
+
+  // Philox state of the spot / image noise: (key, call counter).
+  uint64_t noise_seed;
+  uint32_t noise_call;
+  bool arrays_loaded;
+  bool own_plan;  // campli_plan was cloned by set_method and belongs to this WFS
+
+  SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel, CarmaObj<hipFloatComplex>* d_camplipup,
+             CarmaObj<hipFloatComplex>* d_camplifoc, CarmaObj<hipFloatComplex>* d_fttotim,
+             int64_t nxsub, int64_t nvalid, int64_t npix, int64_t nphase, int64_t nrebin,
+             int64_t nfft, int64_t ntot, int64_t npup, float pdiam, float nphotons,
+             float nphot4imat, int32_t lgs, bool fakecam, int32_t max_flux_per_pix,
+             int32_t max_pix_value, bool is_low_order, bool roket, int32_t device);
+  ~SutraWfsSH();
+
+  int32_t define_mpi_rank(int32_t rank, int32_t size);
+  int32_t allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_plans);
+  int32_t load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binmap, float* offsets,
+                      float* fluxPerSub, int32_t* validsubsx, int32_t* validsubsy, int32_t* istart,
+                      int32_t* jstart, float* ttprojmat, hipFloatComplex* kernel);
+  int32_t fill_binimage(int32_t async);
+  int32_t comp_image(bool noise = true);
+  int32_t comp_nphot(float ittime, float optthroughput, float diam, int32_t nxsub, float zerop = 0,
+                     float gsmag = 0, float lgsreturnperwatt = 0, float laserpower = 0);
+  int32_t set_bincube(float* bincube, int32_t nElem);
+  int32_t set_field_stop(PlanMap& campli_plans, float* field_stop, int32_t N);
+
+  // Spot computation method (SHWFS_METHOD_*); FP32 FFT by default.
+  int32_t set_method(int32_t method);
+  int32_t get_method() const { return campli_plan ? campli_plan->method : SHWFS_METHOD_FFT_FP32; }
+  // Geometry descriptor handed to the C ABI.
+  shwfs_sh_geom geom() const;
+  // Frame arguments of the next comp_image (advances nothing).
+  shwfs_frame_args frame_args(bool with_noise) const;
+
+ private:
+  int32_t comp_generic();
+  bool noise_this_call;
+};
+
+class SutraSensors {
+ public:
+  int32_t device;
+  bool roket;
+  CarmaContext* current_context;
+  size_t nsensors() { return d_wfs.size(); }
+  vector<SutraWfs*> d_wfs;
+  PlanMap campli_plans;
+  PlanMap fttotim_plans;
+  PlanMap ftlgskern_plans;
+  PlanMap field_stop_plans;
+
+  CarmaObj<hipFloatComplex>* d_camplipup;
+  CarmaObj<hipFloatComplex>* d_camplifoc;
+  CarmaObj<hipFloatComplex>* d_fttotim;
+  CarmaObj<hipFloatComplex>* d_ftlgskern;
+  CarmaObj<float>* d_lgskern;
+
+  SutraSensors(CarmaContext* context, SutraTelescope* d_tel, vector<string> type, int32_t nwfs,
+               int64_t* nxsub, int64_t* nvalid, int64_t* npupils, int64_t* npix, int64_t* nphase,
+               int64_t* nrebin, int64_t* nfft, int64_t* ntot, int64_t* npup, float* pdiam,
+               float* nphot, float* nphot4imat, int32_t* lgs, bool* fakecam,
+               int32_t* max_flux_per_pix, int32_t* max_pix_value, int32_t device, bool roket);
+  ~SutraSensors();
+
+  int32_t allocate_buffers();
+  int32_t define_mpi_rank(int32_t rank, int32_t size);
+  int32_t set_noise(int32_t nwfs, float noise, int64_t seed);
+  int32_t set_field_stop(int32_t nwfs, float* field_stop, int32_t N);
+  int32_t initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop, int64_t* size,
+                 float* noise, int64_t* seed, float* G, float* thetaML, float* dx, float* dy);
+  int32_t initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop, int64_t* size,
+                 float* noise, float* G, float* thetaML, float* dx, float* dy);
+  int32_t initgs(float* xpos, float* ypos, float* lambda, float* mag, float zerop, int64_t* size,
+                 float* G, float* thetaML, float* dx, float* dy);
+};
+
+enum class SlopeOrder { untied = 0, interlaced = 1 };
+
+template <class Tin, class Tout>
+class SutraCentroider {
+ public:
+  int32_t device;
+  SutraWfs* wfs;
+  int32_t nvalid;
+  int32_t nslopes;
+  int32_t npix;
+  int32_t nxsub;
+  bool filter_TT;
+  float offset;
+  float scale;
+  SlopeOrder slope_order = SlopeOrder::untied;
+  CarmaContext* current_context;
+
+  CarmaObj<Tout>* d_bincube;
+  CarmaObj<float>* d_intensities;
+  CarmaObj<Tout>* d_centroids_ref;
+  CarmaObj<float>* d_img;
+  CarmaObj<Tin>* d_img_raw;
+  CarmaObj<float>* d_dark;
+  CarmaObj<float>* d_flat;
+  CarmaObj<int32_t>* d_lutPix;
+  CarmaObj<int32_t>* d_validx;
+  CarmaObj<int32_t>* d_validy;
+  CarmaObj<int32_t>* d_validMask;
+  CarmaObj<float>* d_centro_filtered;
+  CarmaObj<float>* d_ref_Tip;
+  CarmaObj<float>* d_ref_Tilt;
+  CarmaObj<float>* d_TT_slopes;
+
+ protected:
+  SutraCentroider(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset, float scale,
+                  bool filter_TT, int32_t device);
+
+ public:
+  virtual ~SutraCentroider();
+  int32_t set_scale(float scale);
+  int32_t set_offset(float offset);
+  int32_t set_dark(float* dark, int32_t n);
+  int32_t set_flat(float* flat, int32_t n);
+  int32_t set_lutPix(int32_t* lutPix, int32_t n);
+  int32_t init_calib(int32_t n, int32_t m);
+  int32_t init_roi(int32_t N);
+  int32_t set_centroids_ref(float* centroids_ref);
+  int32_t calibrate_img() { return calibrate_img(nullptr); }
+  int32_t calibrate_img(hipStream_t stream);
+  int32_t load_validpos(int32_t* ivalid, int32_t* jvalid, int32_t N);
+  int32_t set_npix(int32_t npix);
+  int32_t set_nxsub(int32_t nxsub);
+  int32_t load_img(Tin* img, int32_t n);
+  int32_t load_img(Tin* img, int32_t n, int32_t location);
+  int32_t load_img(Tin* img, int32_t m, int32_t n, int32_t location);
+  int32_t load_img(CarmaObj<Tin>* img);
+  int32_t init_img_raw(int32_t m, int32_t n);
+  bool is_type(string typec) { return typec == get_type(); }
+  int32_t init_TT_filter();
+  int32_t apply_TT_filter(Tout* centroids);
+
+  virtual string get_type() = 0;
+  virtual int32_t get_cog(float* img, float* intensities, Tout* centroids, int32_t nvalid,
+                          int32_t npix, int32_t ntot, hipStream_t stream = nullptr) = 0;
+  virtual int32_t get_cog(float* intensities, Tout* slopes, bool noise) = 0;
+  virtual int32_t get_cog() = 0;
+
+  // Arguments of this centroider for the C ABI (image pointers excluded).
+  virtual shwfs_centroid_args centroid_args(float* intensities, Tout* slopes, int32_t nvalid,
+                                            int32_t npix, int32_t size) = 0;
+  // float64 flux / first-moment sums (tcog / wcog / cog).
+  bool fp64_moments = false;
+  int32_t set_fp64_moments(bool v) {
+    fp64_moments = v;
+    return EXIT_SUCCESS;
+  }
+};
+
+template <class Tin, class T>
+class SutraCentroiderCog : public SutraCentroider<Tin, T> {
+ public:
+  SutraCentroiderCog(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                     float scale, bool filter_TT, int32_t device);
+  ~SutraCentroiderCog();
+  string get_type() override { return "cog"; }
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+};
+
+template <class Tin, class T>
+class SutraCentroiderTcog : public SutraCentroider<Tin, T> {
+ public:
+  float threshold;
+  SutraCentroiderTcog(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                      float scale, bool filter_TT, int32_t device);
+  ~SutraCentroiderTcog();
+  string get_type() override { return "tcog"; }
+  int32_t set_threshold(float threshold);
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+};
+
+template <class Tin, class T>
+class SutraCentroiderWcog : public SutraCentroider<Tin, T> {
+ public:
+  float threshold;
+  CarmaObj<float>* d_weights;
+  SutraCentroiderWcog(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                      float scale, bool filter_TT, int32_t device);
+  ~SutraCentroiderWcog();
+  string get_type() override { return "wcog"; }
+  int32_t init_weights();
+  int32_t load_weights(float* weights, int32_t ndim);
+  int32_t set_threshold(float threshold);
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+};
+
+// Slope sink of the RTC: the reference's controllers are outside this build;
+// the generic controller keeps what do_centroids needs (its slope vector and
+// the indices of the centroiders that feed it).
+template <class T, class Tout>
+class SutraController {
+ public:
+  string type;
+  int32_t device;
+  int32_t nslope, nactu;
+  float delay, gain;
+  vector<int32_t> centro_idx;
+  CarmaObj<T>* d_centroids;
+  CarmaObj<T>* d_com;
+  CarmaContext* current_context;
+  SutraController(CarmaContext* context, string type, int32_t nslope, int32_t nactu, float delay,
+                  int32_t device, const int32_t* idx_centro, int32_t ncentro);
+  ~SutraController();
+  string get_type() { return type; }
+};
+
+// One frame of one SH WFS with its centroider fused into the spot launch:
+// same results as wfs->comp_image(noise) followed by c->get_cog(intensities,
+// slopes, noise) (TCoG's write-back into d_binimg included).
+int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* intensities,
+                   float* slopes, bool noise);
+// Issues `nframes` frames back to back from the host (fused: one launch per
+// frame; otherwise comp_image + get_cog) and returns the elapsed time in ms
+// between hipEvents recorded on the WFS stream around the loop.
+double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
+                  int32_t nframes, bool noise, bool fused);
+
+template <typename Tin, typename T, typename Tout>
+class SutraRtc {
+ public:
+  vector<SutraCentroider<Tin, T>*> d_centro;
+  vector<SutraController<T, Tout>*> d_control;
+
+  SutraRtc();
+  ~SutraRtc();
+  int32_t add_centroider(CarmaContext* context, int64_t nvalid, float offset, float scale,
+                         bool filter_TT, int64_t device, string typec);
+  int32_t add_centroider(CarmaContext* context, int64_t nvalid, float offset, float scale,
+                         bool filter_TT, int64_t device, string typec, SutraWfs* wfs);
+  int32_t add_controller(CarmaContext* context, string typec, int64_t device, float delay,
+                         int32_t nslope, int32_t nactu, const int32_t* idx_centro,
+                         int32_t ncentro);
+  int32_t remove_centroider(int32_t ncentro);
+  int32_t do_calibrate_img(int32_t ncntrl);
+  int32_t do_centroids();
+  int32_t do_centroids(int32_t ncntrl);
+  int32_t do_centroids(int32_t ncntrl, bool noise);
+  int32_t set_centroids_ref(float* centroids_ref);
+};

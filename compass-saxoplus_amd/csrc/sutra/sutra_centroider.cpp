@@ -1,0 +1,677 @@
+// CoG-family centroiders and the RTC dispatch of the SH WFS path.
+// reference: libsutra/src.cpp/sutra_centroider{,_cog,_wcog,_tcog}.cpp, sutra_rtc.cpp
+#include <iostream>
+
+#include "sutra.hpp"
+
+namespace {
+template <class T>
+void del(T*& p) {
+  delete p;
+  p = nullptr;
+}
+}  // namespace
+
+// ============================================================ SutraCentroider
+// reference: libsutra/src.cpp/sutra_centroider.cpp:25-88
+template <class Tin, class Tout>
+SutraCentroider<Tin, Tout>::SutraCentroider(CarmaContext* context, SutraWfs* wfs_,
+                                            int64_t nvalid_, float offset_, float scale_,
+                                            bool filter_TT_, int32_t device_)
+    : device(device_),
+      wfs(wfs_),
+      nvalid((int32_t)nvalid_),
+      nslopes(0),
+      npix(0),
+      nxsub(0),
+      filter_TT(filter_TT_),
+      offset(offset_),
+      scale(scale_),
+      current_context(context),
+      d_bincube(nullptr),
+      d_intensities(nullptr),
+      d_centroids_ref(nullptr),
+      d_img(nullptr),
+      d_img_raw(nullptr),
+      d_dark(nullptr),
+      d_flat(nullptr),
+      d_lutPix(nullptr),
+      d_validx(nullptr),
+      d_validy(nullptr),
+      d_validMask(nullptr),
+      d_centro_filtered(nullptr),
+      d_ref_Tip(nullptr),
+      d_ref_Tilt(nullptr),
+      d_TT_slopes(nullptr) {
+  context->set_active_device(device, 1);
+  int64_t d[2] = {1, nvalid};
+  d_intensities = new CarmaObj<float>(context, d);
+  if (wfs != nullptr) d_img = wfs->d_binimg;
+  nslopes = 2 * nvalid;
+  int64_t d2[2] = {1, nslopes};
+  d_centroids_ref = new CarmaObj<Tout>(context, d2);
+  if (filter_TT) init_TT_filter();
+}
+
+template <class Tin, class Tout>
+SutraCentroider<Tin, Tout>::~SutraCentroider() {
+  del(d_intensities);
+  del(d_centroids_ref);
+  if (wfs == nullptr) del(d_img);
+  del(d_img_raw);
+  del(d_validx);
+  del(d_validy);
+  del(d_dark);
+  del(d_flat);
+  del(d_lutPix);
+  del(d_bincube);
+  del(d_validMask);
+  del(d_TT_slopes);
+  del(d_centro_filtered);
+  del(d_ref_Tip);
+  del(d_ref_Tilt);
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_scale(float s) {
+  scale = s;
+  return EXIT_SUCCESS;
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_offset(float o) {
+  offset = o;
+  return EXIT_SUCCESS;
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_nxsub(int32_t n) {
+  nxsub = n;
+  return EXIT_SUCCESS;
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_npix(int32_t n) {
+  npix = n;
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_centroider.cpp:90-114
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::init_calib(int32_t n, int32_t m) {
+  current_context->set_active_device(device, 1);
+  int64_t d[3] = {2, n, m};
+  if (d_img == nullptr) d_img = new CarmaObj<float>(current_context, d);
+  if (d_dark == nullptr) {
+    d_dark = new CarmaObj<float>(current_context, d);
+    d_dark->reset();
+  }
+  if (d_flat == nullptr) {
+    d_flat = new CarmaObj<float>(current_context, d);
+    d_flat->memset(1.f);
+  }
+  if (d_lutPix == nullptr) {
+    int64_t d1[2] = {1, (int64_t)n * m};
+    d_lutPix = new CarmaObj<int32_t>(current_context, d1);
+    std::vector<int32_t> h((size_t)n * m);
+    for (int32_t i = 0; i < n * m; ++i) h[i] = i;
+    d_lutPix->host2device(h.data());
+  }
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::init_roi(int32_t N) {
+  current_context->set_active_device(device, 1);
+  if (d_validx == nullptr) {
+    int64_t d[2] = {1, N};
+    d_validx = new CarmaObj<int32_t>(current_context, d);
+    d_validy = new CarmaObj<int32_t>(current_context, d);
+  }
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_dark(float* dark, int32_t n) {
+  current_context->set_active_device(device, 1);
+  if (d_dark == nullptr) {
+    int64_t d[3] = {2, n, n};
+    d_dark = new CarmaObj<float>(current_context, d);
+  }
+  return d_dark->host2device(dark);
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_flat(float* flat, int32_t n) {
+  current_context->set_active_device(device, 1);
+  if (d_flat == nullptr) {
+    int64_t d[3] = {2, n, n};
+    d_flat = new CarmaObj<float>(current_context, d);
+  }
+  return d_flat->host2device(flat);
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_lutPix(int32_t* lutPix, int32_t n) {
+  current_context->set_active_device(device, 1);
+  if (d_lutPix == nullptr) {
+    int64_t d[2] = {1, n};
+    d_lutPix = new CarmaObj<int32_t>(current_context, d);
+  }
+  return d_lutPix->host2device(lutPix);
+}
+
+// reference: libsutra/src.cpp/sutra_centroider.cpp:226-244 (img = (raw[lut] - dark) * flat)
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::calibrate_img(hipStream_t stream) {
+  current_context->set_active_device(device, 1);
+  if (d_img_raw == nullptr) {
+    std::cout << "Image not initialized\n" << std::endl;
+    return EXIT_FAILURE;
+  }
+  init_calib((int32_t)d_img_raw->get_dims(1), (int32_t)d_img_raw->get_dims(2));
+  if (d_img->get_nb_elements() != d_img_raw->get_nb_elements())
+    throw std::runtime_error("calibrate_img: raw and calibrated images differ in size");
+  shwfs_throw_if(shwfs_calibrate(d_img_raw->get_data(), d_img->get_data(), d_dark->get_data(),
+                                 d_flat->get_data(), d_lutPix->get_data(),
+                                 d_img->get_nb_elements(), stream),
+                 "calibrate_img");
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::load_img(CarmaObj<Tin>* img) {
+  return load_img(img->get_data(), (int32_t)img->get_dims(1), img->get_device());
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::load_img(Tin* img, int32_t n) {
+  return load_img(img, n, -1);
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::load_img(Tin* img, int32_t n, int32_t location) {
+  return load_img(img, n, n, location);
+}
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::load_img(Tin* img, int32_t m, int32_t n, int32_t location) {
+  init_img_raw(m, n);
+  if (d_img_raw->get_dims(1) != m || d_img_raw->get_dims(2) != n)
+    throw std::runtime_error("load_img: image size differs from the first loaded image");
+  if (location < 0)
+    d_img_raw->host2device(img);
+  else
+    d_img_raw->copy_from(img, d_img_raw->get_nb_elements());
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::init_img_raw(int32_t m, int32_t n) {
+  current_context->set_active_device(device, 1);
+  if (d_img_raw == nullptr) {
+    int64_t d[3] = {2, m, n};
+    d_img_raw = new CarmaObj<Tin>(current_context, d);
+  }
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::load_validpos(int32_t* ivalid, int32_t* jvalid, int32_t N) {
+  current_context->set_active_device(device, 1);
+  if (d_validx == nullptr) init_roi(N);
+  if (d_validx->get_nb_elements() < N)
+    throw std::runtime_error("load_validpos: more positions than the centroider holds");
+  d_validx->host2device(ivalid);
+  d_validy->host2device(jvalid);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::set_centroids_ref(float* centroids_ref) {
+  return d_centroids_ref->host2device(centroids_ref);
+}
+
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::init_TT_filter() {
+  current_context->set_active_device(device, 1);
+  int64_t d[2] = {1, 2};
+  d_TT_slopes = new CarmaObj<float>(current_context, d);
+  d[1] = nslopes;
+  d_centro_filtered = new CarmaObj<float>(current_context, d);
+  d_ref_Tip = new CarmaObj<float>(current_context, d);
+  d_ref_Tilt = new CarmaObj<float>(current_context, d);
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_centroider.cpp:322-338.  The projection on
+// the two reference vectors runs on the host here (2*nslopes flops, off the
+// default path: filter_TT is false unless asked for).
+template <class Tin, class Tout>
+int32_t SutraCentroider<Tin, Tout>::apply_TT_filter(Tout* centroids) {
+  std::vector<float> c(nslopes), tip(nslopes), tilt(nslopes);
+  carma_safe_call(hipMemcpy(c.data(), centroids, sizeof(float) * nslopes, hipMemcpyDeviceToHost));
+  d_ref_Tip->device2host(tip.data());
+  d_ref_Tilt->device2host(tilt.data());
+  float a = 0.f, b = 0.f;
+  for (int32_t i = 0; i < nslopes; ++i) a += c[i] * tip[i];
+  for (int32_t i = 0; i < nslopes; ++i) c[i] -= a * tip[i];
+  for (int32_t i = 0; i < nslopes; ++i) b += c[i] * tilt[i];
+  for (int32_t i = 0; i < nslopes; ++i) c[i] -= b * tilt[i];
+  carma_safe_call(hipMemcpy(centroids, c.data(), sizeof(float) * nslopes, hipMemcpyHostToDevice));
+  float tt[2] = {a, b};
+  d_TT_slopes->host2device(tt);
+  return EXIT_SUCCESS;
+}
+
+// ---- common get_cog plumbing ------------------------------------------------
+template <class Tin, class Tout>
+static int32_t run_centroids(SutraCentroider<Tin, Tout>* c, float* img, float* intensities,
+                             Tout* centroids, int32_t nvalid, int32_t npix, int32_t ntot,
+                             hipStream_t stream) {
+  c->current_context->set_active_device(c->device, 1);
+  if (c->d_validx == nullptr)
+    throw std::runtime_error("get_cog: load_validpos has not been called");
+  if (npix <= 0) throw std::runtime_error("get_cog: set_npix has not been called");
+  if (c->d_validx->get_nb_elements() < nvalid)
+    throw std::runtime_error("get_cog: fewer valid positions loaded than sub-apertures");
+  if (c->get_type() == "wcog") {
+    auto* w = dynamic_cast<SutraCentroiderWcog<Tin, Tout>*>(c);
+    if (w && w->d_weights && w->d_weights->get_nb_elements() < (int64_t)npix * npix * nvalid)
+      throw std::runtime_error("get_cog: weights smaller than npix^2 * nvalid");
+  }
+  shwfs_centroid_args a = c->centroid_args(intensities, centroids, nvalid, npix, ntot);
+  float* out = (a.kind == SHWFS_CENTROID_TCOG) ? img : nullptr;
+  shwfs_throw_if(shwfs_centroids(img, out, &a, stream), "get_cog");
+  if (c->filter_TT) c->apply_TT_filter(centroids);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class Tout>
+static shwfs_centroid_args base_args(SutraCentroider<Tin, Tout>* c, int32_t kind,
+                                     float* intensities, Tout* slopes, int32_t nvalid,
+                                     int32_t npix, int32_t size) {
+  shwfs_centroid_args a{};
+  a.kind = kind;
+  a.nvalid = nvalid;
+  a.npix = npix;
+  a.size = size;
+  a.d_validx = c->d_validx ? c->d_validx->get_data() : nullptr;
+  a.d_validy = c->d_validy ? c->d_validy->get_data() : nullptr;
+  a.offset = c->offset;
+  a.scale = c->scale;
+  a.threshold = 0.f;
+  a.d_weights = nullptr;
+  a.d_ref = c->d_centroids_ref->get_data();
+  a.d_slopes = slopes;
+  a.d_intensities = intensities;
+  a.slope_order = (int32_t)c->slope_order;
+  a.fp64_moments = c->fp64_moments ? 1 : 0;
+  return a;
+}
+
+// ============================================================ Cog
+// reference: libsutra/src.cpp/sutra_centroider_cog.cpp:26-98
+template <class Tin, class T>
+SutraCentroiderCog<Tin, T>::SutraCentroiderCog(CarmaContext* context, SutraWfs* wfs,
+                                               int64_t nvalid, float offset, float scale,
+                                               bool filter_TT, int32_t device)
+    : SutraCentroider<Tin, T>(context, wfs, nvalid, offset, scale, filter_TT, device) {}
+template <class Tin, class T>
+SutraCentroiderCog<Tin, T>::~SutraCentroiderCog() {}
+
+template <class Tin, class T>
+shwfs_centroid_args SutraCentroiderCog<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                              int32_t nvalid, int32_t npix,
+                                                              int32_t size) {
+  return base_args(this, SHWFS_CENTROID_COG, intensities, slopes, nvalid, npix, size);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCog<Tin, T>::get_cog(float* img, float* intensities, T* centroids,
+                                            int32_t nvalid, int32_t npix, int32_t ntot,
+                                            hipStream_t stream) {
+  return run_centroids(this, img, intensities, centroids, nvalid, npix, ntot, stream);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCog<Tin, T>::get_cog(float* intensities, T* slopes, bool noise) {
+  if (this->wfs != nullptr) {
+    CarmaObj<float>* img = (noise || !this->wfs->roket) ? this->wfs->d_binimg
+                                                        : this->wfs->d_binimg_notnoisy;
+    return get_cog(img->get_data(), intensities, slopes, this->wfs->nvalid_tot,
+                   (int32_t)this->wfs->npix, (int32_t)this->wfs->d_binimg->get_dims(1),
+                   this->wfs->stream);
+  }
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_SUCCESS;  // as the reference (sutra_centroider_cog.cpp:86-87)
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCog<Tin, T>::get_cog() {
+  if (this->wfs != nullptr)
+    return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_SUCCESS;
+}
+
+// ============================================================ Tcog
+// reference: libsutra/src.cpp/sutra_centroider_tcog.cpp:26-108
+template <class Tin, class T>
+SutraCentroiderTcog<Tin, T>::SutraCentroiderTcog(CarmaContext* context, SutraWfs* wfs,
+                                                 int64_t nvalid, float offset, float scale,
+                                                 bool filter_TT, int32_t device)
+    : SutraCentroider<Tin, T>(context, wfs, nvalid, offset, scale, filter_TT, device),
+      threshold(0.f) {}
+template <class Tin, class T>
+SutraCentroiderTcog<Tin, T>::~SutraCentroiderTcog() {}
+
+template <class Tin, class T>
+int32_t SutraCentroiderTcog<Tin, T>::set_threshold(float t) {
+  threshold = t;
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+shwfs_centroid_args SutraCentroiderTcog<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                               int32_t nvalid, int32_t npix,
+                                                               int32_t size) {
+  shwfs_centroid_args a = base_args(this, SHWFS_CENTROID_TCOG, intensities, slopes, nvalid, npix, size);
+  a.threshold = threshold;
+  return a;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderTcog<Tin, T>::get_cog(float* img, float* intensities, T* centroids,
+                                             int32_t nvalid, int32_t npix, int32_t ntot,
+                                             hipStream_t stream) {
+  return run_centroids(this, img, intensities, centroids, nvalid, npix, ntot, stream);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderTcog<Tin, T>::get_cog(float* intensities, T* slopes, bool noise) {
+  if (this->wfs != nullptr) {
+    CarmaObj<float>* img = (noise || !this->wfs->roket) ? this->wfs->d_binimg
+                                                        : this->wfs->d_binimg_notnoisy;
+    return get_cog(img->get_data(), intensities, slopes, (int32_t)this->wfs->nvalid,
+                   (int32_t)this->wfs->npix, (int32_t)this->wfs->d_binimg->get_dims(1),
+                   this->wfs->stream);
+  }
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderTcog<Tin, T>::get_cog() {
+  if (this->wfs != nullptr)
+    return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+// ============================================================ Wcog
+// reference: libsutra/src.cpp/sutra_centroider_wcog.cpp:26-159
+template <class Tin, class T>
+SutraCentroiderWcog<Tin, T>::SutraCentroiderWcog(CarmaContext* context, SutraWfs* wfs,
+                                                 int64_t nvalid, float offset, float scale,
+                                                 bool filter_TT, int32_t device)
+    : SutraCentroider<Tin, T>(context, wfs, nvalid, offset, scale, filter_TT, device),
+      threshold(0.f),
+      d_weights(nullptr) {}
+template <class Tin, class T>
+SutraCentroiderWcog<Tin, T>::~SutraCentroiderWcog() {
+  delete d_weights;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::init_weights() {
+  this->current_context->set_active_device(this->device, 1);
+  delete d_weights;
+  int64_t d[4] = {3, this->npix, this->npix, this->nvalid};
+  d_weights = new CarmaObj<float>(this->current_context, d);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::load_weights(float* weights, int32_t ndim) {
+  if (d_weights == nullptr) throw std::runtime_error("load_weights: init_weights has not been called");
+  if (ndim == 3) {
+    d_weights->host2device(weights);
+  } else {
+    int64_t d[3] = {2, this->npix, this->npix};
+    CarmaObj<float> tmp(this->current_context, d, weights);
+    shwfs_throw_if(shwfs_fill_weights(d_weights->get_data(), tmp.get_data(), this->npix,
+                                      (int32_t)d_weights->get_nb_elements(), nullptr),
+                   "load_weights");
+    carma_safe_call(hipDeviceSynchronize());
+  }
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::set_threshold(float t) {
+  threshold = t;
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+shwfs_centroid_args SutraCentroiderWcog<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                               int32_t nvalid, int32_t npix,
+                                                               int32_t size) {
+  shwfs_centroid_args a = base_args(this, SHWFS_CENTROID_WCOG, intensities, slopes, nvalid, npix, size);
+  a.threshold = threshold;
+  a.d_weights = d_weights ? d_weights->get_data() : nullptr;
+  return a;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::get_cog(float* img, float* intensities, T* centroids,
+                                             int32_t nvalid, int32_t npix, int32_t ntot,
+                                             hipStream_t stream) {
+  return run_centroids(this, img, intensities, centroids, nvalid, npix, ntot, stream);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::get_cog(float* intensities, T* slopes, bool noise) {
+  if (this->wfs != nullptr) {
+    CarmaObj<float>* img = (noise || !this->wfs->roket) ? this->wfs->d_binimg
+                                                        : this->wfs->d_binimg_notnoisy;
+    return get_cog(img->get_data(), intensities, slopes, (int32_t)this->wfs->nvalid,
+                   (int32_t)this->wfs->npix, (int32_t)this->wfs->d_binimg->get_dims(1),
+                   this->wfs->stream);
+  }
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderWcog<Tin, T>::get_cog() {
+  if (this->wfs != nullptr)
+    return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+// ============================================================ fused frame
+int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* intensities,
+                   float* slopes, bool noise) {
+  if (c == nullptr || c->wfs != wfs)
+    throw std::runtime_error("comp_frame: the centroider must be attached to this WFS");
+  if (!wfs->arrays_loaded || wfs->d_gs == nullptr)
+    throw std::runtime_error("comp_frame: load_arrays / initgs have not been called");
+  if (c->npix != wfs->npix || c->nvalid != wfs->nvalid)
+    throw std::runtime_error("comp_frame: centroider and WFS sizes differ");
+  if (wfs->d_gs->d_phase->d_screen->get_dims(1) != wfs->d_pupil->get_dims(1))
+    throw std::runtime_error("comp_frame: phase screen and pupil sizes differ");
+  if (c->get_type() == "wcog") {
+    auto* wc = dynamic_cast<SutraCentroiderWcog<float, float>*>(c);
+    if (!wc || !wc->d_weights || wc->d_weights->get_nb_elements() < wfs->npix * wfs->npix * wfs->nvalid)
+      throw std::runtime_error("comp_frame: WCoG weights not initialised for this WFS");
+  }
+  wfs->current_context->set_active_device(wfs->device, 1);
+  shwfs_sh_geom g = wfs->geom();
+  shwfs_frame_args f = wfs->frame_args(noise);
+  shwfs_centroid_args a =
+      c->centroid_args(intensities, slopes, (int32_t)wfs->nvalid, (int32_t)wfs->npix,
+                       (int32_t)wfs->d_binimg->get_dims(1));
+  f.centroid = &a;
+  shwfs_throw_if(shwfs_sh_comp_image(&g, &f, wfs->stream), "comp_frame");
+  wfs->noise_call++;
+  if (c->filter_TT) c->apply_TT_filter(slopes);
+  return EXIT_SUCCESS;
+}
+
+double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
+                  int32_t nframes, bool noise, bool fused) {
+  hipEvent_t e0, e1;
+  carma_safe_call(hipEventCreate(&e0));
+  carma_safe_call(hipEventCreate(&e1));
+  carma_safe_call(hipEventRecord(e0, wfs->stream));
+  for (int32_t k = 0; k < nframes; ++k) {
+    if (fused) {
+      comp_frame(wfs, c, c->d_intensities->get_data(), slopes, noise);
+    } else {
+      wfs->comp_image(noise);
+      c->get_cog(c->d_intensities->get_data(), slopes, noise);
+    }
+  }
+  carma_safe_call(hipEventRecord(e1, wfs->stream));
+  carma_safe_call(hipEventSynchronize(e1));
+  float ms = 0.f;
+  carma_safe_call(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
+template class SutraCentroider<float, float>;
+template class SutraCentroiderCog<float, float>;
+template class SutraCentroiderTcog<float, float>;
+template class SutraCentroiderWcog<float, float>;
+
+// ============================================================ controller / rtc
+template <class T, class Tout>
+SutraController<T, Tout>::SutraController(CarmaContext* context, string type_, int32_t nslope_,
+                                          int32_t nactu_, float delay_, int32_t device_,
+                                          const int32_t* idx_centro, int32_t ncentro)
+    : type(type_),
+      device(device_),
+      nslope(nslope_),
+      nactu(nactu_),
+      delay(delay_),
+      gain(0.f),
+      current_context(context) {
+  int64_t d[2] = {1, nslope};
+  d_centroids = new CarmaObj<T>(context, d);
+  d[1] = nactu > 0 ? nactu : 1;
+  d_com = new CarmaObj<T>(context, d);
+  for (int32_t i = 0; i < ncentro; ++i) centro_idx.push_back(idx_centro ? idx_centro[i] : i);
+}
+
+template <class T, class Tout>
+SutraController<T, Tout>::~SutraController() {
+  delete d_centroids;
+  delete d_com;
+}
+
+template class SutraController<float, float>;
+
+template <typename Tin, typename T, typename Tout>
+SutraRtc<Tin, T, Tout>::SutraRtc() {}
+
+template <typename Tin, typename T, typename Tout>
+SutraRtc<Tin, T, Tout>::~SutraRtc() {
+  for (auto* c : d_centro) delete c;
+  for (auto* c : d_control) delete c;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nvalid, float offset,
+                                               float scale, bool filter_TT, int64_t device,
+                                               string typec) {
+  return add_centroider(context, nvalid, offset, scale, filter_TT, device, typec, nullptr);
+}
+
+// reference: libsutra/src.cpp/sutra_rtc.cpp:73-115 (factory by type string)
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nvalid, float offset,
+                                               float scale, bool filter_TT, int64_t device,
+                                               string typec, SutraWfs* wfs) {
+  if (typec == "cog")
+    d_centro.push_back(new SutraCentroiderCog<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                      filter_TT, (int32_t)device));
+  else if (typec == "tcog")
+    d_centro.push_back(new SutraCentroiderTcog<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                       filter_TT, (int32_t)device));
+  else if (typec == "wcog")
+    d_centro.push_back(new SutraCentroiderWcog<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                       filter_TT, (int32_t)device));
+  else {
+    std::cerr << "centroider unknown" << std::endl;
+    return EXIT_FAILURE;
+  }
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::add_controller(CarmaContext* context, string typec, int64_t device,
+                                               float delay, int32_t nslope, int32_t nactu,
+                                               const int32_t* idx_centro, int32_t ncentro) {
+  if (typec != "generic")
+    throw std::runtime_error("add_controller: only the slope-holding \"generic\" controller is part of this build");
+  d_control.push_back(new SutraController<T, Tout>(context, typec, nslope, nactu, delay,
+                                                   (int32_t)device, idx_centro, ncentro));
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::remove_centroider(int32_t ncentro) {
+  delete d_centro.at(ncentro);
+  d_centro.erase(d_centro.begin() + ncentro);
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::do_calibrate_img(int32_t ncntrl) {
+  for (int32_t idx : d_control.at(ncntrl)->centro_idx) d_centro.at(idx)->calibrate_img();
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::do_centroids() {
+  for (auto* c : d_centro) c->get_cog();
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::do_centroids(int32_t ncntrl) {
+  return do_centroids(ncntrl, true);
+}
+
+// reference: libsutra/src.cpp/sutra_rtc.cpp:485-511
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::do_centroids(int32_t ncntrl, bool noise) {
+  SutraController<T, Tout>* ctl = d_control.at(ncntrl);
+  int32_t indslope = 0;
+  for (int32_t idx : ctl->centro_idx) {
+    SutraCentroider<Tin, T>* c = d_centro.at(idx);
+    if (indslope + c->nslopes > ctl->nslope)
+      throw std::runtime_error("do_centroids: the controller slope vector is too short");
+    if (c->wfs != nullptr) {
+      c->get_cog(c->d_intensities->get_data(), ctl->d_centroids->get_data_at(indslope), noise);
+    } else {
+      if (c->d_img == nullptr) throw std::runtime_error("do_centroids: no image loaded");
+      c->get_cog(c->d_img->get_data(), c->d_intensities->get_data(),
+                 ctl->d_centroids->get_data_at(indslope), c->nvalid, c->npix,
+                 (int32_t)c->d_img->get_dims(1));
+    }
+    indslope += c->nslopes;
+  }
+  return EXIT_SUCCESS;
+}
+
+template <typename Tin, typename T, typename Tout>
+int32_t SutraRtc<Tin, T, Tout>::set_centroids_ref(float* centroids_ref) {
+  int32_t off = 0;
+  for (auto* c : d_centro) {
+    c->set_centroids_ref(centroids_ref + off);
+    off += c->nslopes;
+  }
+  return EXIT_SUCCESS;
+}
+
+template class SutraRtc<float, float, float>;

@@ -1,0 +1,159 @@
+/* shwfs.h - C ABI of the MI355X (gfx950) Shack-Hartmann WFS hot path.
+ *
+ * Plain C, plain pointers and sizes.  Every device pointer is owned by the
+ * caller (the C++ sutra/carma layer, or any FFI binding); the library never
+ * frees what it is given.  Every entry point returns SHWFS_OK (0) or a
+ * negative SHWFS_ERR_* code; the message of the last error on the calling
+ * host thread is available from shwfs_last_error().  Nothing here throws or
+ * aborts.  `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Array layout is the reference's: column-major ("x fast"), e.g. a phase
+ * screen of side npup holds phase(x, y) at x + y*npup; the spot cube holds
+ * bincube(px, py, s) at px + py*npix + s*npix*npix.
+ */
+#ifndef SHWFS_H
+#define SHWFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SHWFS_OK = 0,
+  SHWFS_ERR_ARG = -1,         /* null pointer / inconsistent sizes */
+  SHWFS_ERR_UNSUPPORTED = -2, /* shape outside what the kernels implement */
+  SHWFS_ERR_HIP = -3,         /* a HIP runtime call failed */
+  SHWFS_ERR_STATE = -4,       /* call order violated (e.g. image before tables) */
+  SHWFS_ERR_NOMEM = -5
+};
+
+/* Computation method of the spot image (shwfs_frame_args.method). */
+enum {
+  SHWFS_METHOD_FFT_FP32 = 0,   /* plain FP32: in-register radix-2 FFT */
+  SHWFS_METHOD_DFT_BF16X3 = 1, /* pruned DFT on MFMA, split-BF16x3, FP32 accumulate */
+  SHWFS_METHOD_DFT_FP16 = 2    /* pruned DFT on MFMA, FP16 with per-subap scale */
+};
+
+/* Centroider kinds (shwfs_centroid_args.kind). */
+enum {
+  SHWFS_CENTROID_NONE = 0,
+  SHWFS_CENTROID_COG = 1,
+  SHWFS_CENTROID_TCOG = 2,
+  SHWFS_CENTROID_WCOG = 3
+};
+
+/* Geometry of one SH WFS: sizes + device tables (all owned by the caller).
+ * Filled by the C++ layer after SutraWfsSH::load_arrays
+ * (reference: libsutra/src.cpp/sutra_wfs_sh.cpp:308-331). */
+typedef struct shwfs_sh_geom {
+  int32_t nvalid;  /* valid sub-apertures */
+  int32_t nphase;  /* phase points per sub-aperture side (pdiam) */
+  int32_t npup;    /* side of the phase screen / pupil arrays */
+  int32_t nfft;    /* FFT support side (8, 16, 32 or 64) */
+  int32_t ntot;    /* HR support side; must equal nfft here */
+  int32_t npix;    /* detector pixels per sub-aperture side */
+  int32_t nrebin;  /* HR pixels per detector pixel side */
+  int32_t nxsub;   /* sub-apertures across the pupil; binimg side = npix*nxsub */
+  const int32_t* d_istart;     /* [nvalid] validpuppixx (x of the subap corner in the screen) */
+  const int32_t* d_jstart;     /* [nvalid] validpuppixy */
+  const int32_t* d_validsubsx; /* [nvalid] x of the spot corner in binimg (already *npix) */
+  const int32_t* d_validsubsy; /* [nvalid] */
+  const float* d_offsets;      /* [nphase^2] half-pixel tip-tilt phase (halfxy) */
+  const float* d_fluxPerSub;   /* [nvalid] */
+  const int32_t* d_binmap;     /* [nrebin^2 * npix^2] HR flat index per binned pixel */
+} shwfs_sh_geom;
+
+/* A centroider applied to a detector image (or fused after the spot stage).
+ * Reference kernels: libsutra/src.cu/sutra_centroider_{cog,wcog,tcog}.cu. */
+typedef struct shwfs_centroid_args {
+  int32_t kind;            /* SHWFS_CENTROID_* */
+  int32_t nvalid;
+  int32_t npix;
+  int32_t size;            /* side of the image */
+  const int32_t* d_validx; /* [nvalid] spot corner x in the image */
+  const int32_t* d_validy; /* [nvalid] */
+  float offset;            /* pixel offset subtracted (npix/2 - 0.5) */
+  float scale;             /* arcsec per pixel */
+  float threshold;         /* TCOG / WCOG */
+  const float* d_weights;  /* WCOG: [npix^2 * nvalid] */
+  const float* d_ref;      /* [2*nvalid] reference slopes (may be NULL = zeros) */
+  float* d_slopes;         /* [2*nvalid] */
+  float* d_intensities;    /* [nvalid] centroider flux */
+  int32_t slope_order;     /* 0 untied (x..., y...), 1 interlaced (x, y, x, y...) */
+  int32_t fp64_moments;    /* 1: flux and first moments summed in float64 */
+} shwfs_centroid_args;
+
+/* One frame of one SH WFS. */
+typedef struct shwfs_frame_args {
+  const float* d_phase;      /* [npup^2] phase screen in micrometres */
+  const float* d_pupil;      /* [npup^2] amplitude mask */
+  float scale;               /* 2*pi / lambda[um] */
+  float nphot;               /* photons per fully illuminated sub-aperture */
+  float noise;               /* spot noise: < 0 none, 0 Poisson, > 0 Poisson + readout sigma */
+  float noise_background;    /* > 0: binimg background = sigma*N(0,1) before the spots */
+  uint64_t seed;             /* Philox key */
+  uint32_t call;             /* Philox call counter (advance once per frame) */
+  int32_t method;            /* SHWFS_METHOD_* */
+  float* d_bincube;          /* [npix^2 * nvalid] */
+  float* d_binimg;           /* [(npix*nxsub)^2] or NULL to skip the image */
+  float* d_intensities;      /* [nvalid] raw per-subap flux before normalisation */
+  const shwfs_centroid_args* centroid; /* NULL or fused centroider (validx/size ignored) */
+} shwfs_frame_args;
+
+/* ---- spot image (SutraWfsSH::comp_generic + fill_binimage, fused) ---------
+ * Replaces the launch sequence of libsutra/src.cpp/sutra_wfs_sh.cpp:337-621:
+ * fillcamplipup (src.cu/sutra_wfs.cu:55-85), CarmaFFT (libcarma/src.cpp/
+ * carma_fft.cpp:164-180), abs2c (src.cu/sutra_utils.cu:157-168), fillbincube
+ * (src.cu/sutra_wfs.cu:396-414), subap_reduce (696-718), subap_norm
+ * (src.cu/sutra_utils.cu:181-193), prng 'P'/'N' (libcarma/src.cpp/
+ * carma_obj.cpp:490-527) and fillbinimg (src.cu/sutra_wfs.cu:205-229),
+ * plus an optional fused get_centroids. */
+int32_t shwfs_sh_comp_image(const shwfs_sh_geom* geom, const shwfs_frame_args* frame, void* stream);
+
+/* Validates a geometry against what the kernels implement (no launch). */
+int32_t shwfs_sh_check_geom(const shwfs_sh_geom* geom, int32_t method);
+
+/* ---- centroiders on a detector image ---------------------------------------
+ * Replaces get_centroids of libsutra/src.cu/sutra_centroider_cog.cu:83-150,
+ * sutra_centroider_wcog.cu:118-170 and sutra_centroider_tcog.cu:78-141
+ * (TCOG writes the thresholded pixels back into the image, as the reference). */
+int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_centroid_args* args,
+                        void* stream);
+
+/* Broadcast a npix x npix weight map to every sub-aperture:
+ * replaces fill_weights, libsutra/src.cu/sutra_centroider_wcog.cu:27-50. */
+int32_t shwfs_fill_weights(float* d_out, const float* d_in, int32_t npix, int32_t n_total,
+                           void* stream);
+
+/* ---- counter-based noise on a device array ----------------------------------
+ * Replaces CarmaObj<float>::prng (libcarma/src.cpp/carma_obj.cpp:490-527,
+ * src.cu/carma_rng.cu:98-141,287-303) with Philox4x32-10 streams:
+ * gtype 'P': d = Poisson(d); 'N': d = beta*d + alpha*N(0,1); 'U': d = beta*d + U(0,1). */
+int32_t shwfs_prng(float* d_data, int64_t n, char gtype, float alpha, float beta, uint64_t seed,
+                   uint32_t call, uint32_t domain, void* stream);
+
+/* ---- detector calibration ----------------------------------------------------
+ * cal[i] = (raw[lutPix[i]] - dark[i]) * flat[i]; replaces calib_krnl,
+ * libsutra/src.cu/sutra_centroider.cu:107-114 (float input). */
+int32_t shwfs_calibrate(const float* d_raw, float* d_cal, const float* d_dark, const float* d_flat,
+                        const int32_t* d_lutPix, int64_t n, void* stream);
+
+/* ---- geometric slopes (mean phase gradient per sub-aperture) -----------------
+ * type 0: edge differences, reduce_phasex/y (libsutra/src.cu/sutra_wfs.cu:868-913);
+ * type 2: projection on ttprojmat (1036-1090).  phase in micrometres; phasemap
+ * [nphase^2 * nvalid] flat screen indices; slopes [2*nvalid] (x then y). */
+int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
+                          const int32_t* d_phasemap, const float* d_ttprojmat, int32_t nphase,
+                          int32_t nvalid, float alpha, void* stream);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+int32_t shwfs_last_error(char* buf, size_t len);
+const char* shwfs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHWFS_H */

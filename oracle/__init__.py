@@ -1,0 +1,41 @@
+"""CPU oracle for the Shack-Hartmann WFS hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the C++
+``sutra``/``carma`` layer, the ``shwfs`` host package) may import, call or link
+anything from this package.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker / the CPU
+baseline, never as the thing measured or shipped.
+
+The oracle restates, in float64 NumPy (pocketfft for the FFT) and plain Python
+loops for the small cases, the algorithm of the reference path:
+
+* ``geometry``  - pupil, valid sub-apertures, index maps (binmap, halfxy, ...)
+                  (shesha/shesha/init/geom_init.py, shesha/shesha/util/make_pupil.py)
+* ``wfs``       - SutraWfsSH::comp_image (libsutra/src.cpp/sutra_wfs_sh.cpp:337-643)
+* ``centroid``  - CoG / weighted CoG / thresholded CoG
+                  (libsutra/src.cu/sutra_centroider_{cog,wcog,tcog}.cu)
+* ``philox``    - Philox4x32-10 counter RNG (Salmon et al., SC'11) and the
+                  Poisson / Gaussian samplers the noise stage draws with
+* ``turbulence``- von Karman phase structure function (closed form used by the
+                  reference's ``rodconan``, shesha/shesha/util/iterkolmo.py:313-351)
+* ``mlp``       - float64 forward/backward of the per-subaperture MLP centroider
+
+Pinning status of every function is listed in ``PINNING`` below; each entry
+names the tests that pin it.  Any function marked "parity unpinned" has no
+independent check.
+"""
+
+PINNING = {
+    "geometry.wfs_size": "pinned: tests/test_oracle_geometry.py (stated Nfft/nrebin/Ntot of the configs)",
+    "geometry.sh_geometry": "pinned: c1 known answer (52 valid), closed-form halfxy, "
+                            "binmap injection/fftshift invariants, brute-force recount",
+    "wfs.comp_image": "pinned: brute-force DFT, Parseval, flux normalisation, Fourier-shift "
+                      "theorem slopes, flat-wavefront centring (tests/test_oracle_wfs.py)",
+    "centroid.cog/wcog/tcog": "pinned: scipy.ndimage.center_of_mass (as the reference tests do), "
+                              "threshold/epsilon special cases (tests/test_oracle_centroid.py)",
+    "philox.philox4x32": "pinned: Random123 known-answer vectors (tests/test_oracle_philox.py)",
+    "philox.poisson/normal": "pinned: mean/variance/KS tests against scipy.stats",
+    "turbulence.structure_function": "pinned: Kolmogorov limit 6.88 (r/r0)^(5/3) and "
+                                     "saturation 2*variance for r >> L0",
+    "mlp": "pinned: central finite differences of the float64 loss",
+}

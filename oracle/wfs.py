@@ -1,0 +1,103 @@
+"""Oracle for SutraWfsSH::comp_image in float64 (test infrastructure only).
+
+Restates libsutra/src.cpp/sutra_wfs_sh.cpp:337-643 for the NGS, Ntot == Nfft,
+no-field-stop, no-kernel-convolution case, kernel by kernel:
+
+* camplipup_krnl   libsutra/src.cu/sutra_wfs.cu:27-53
+      E[x + y*Nfft + s*Nfft^2] = mask[p] * exp(i*(phase[p]*scale - offset[x + y*pdiam]))
+      with p = x + y*npup + istart[s] + jstart[s]*npup, x, y < pdiam, zero elsewhere.
+* CarmaFFT(dir=1)  libcarma/src.cpp/carma_fft.cpp:164-180 -> forward, unnormalised,
+      exp(-2*pi*i*k*n/N)  (numpy.fft.fft2 convention).
+* abs2c_krnl       libsutra/src.cu/sutra_utils.cu:144-168
+* fillbincube_krnl libsutra/src.cu/sutra_wfs.cu:369-414
+      bincube[p + s*npix^2] = sum_i hr[binmap[i + p*nrebin^2] + s*Nfft^2]
+* reduce2          libsutra/src.cu/sutra_wfs.cu:556-590  (per-subap flux)
+* subapnorm_krnl   libsutra/src.cu/sutra_utils.cu:170-193
+      b = b * fluxPerSub[s] / I[s] * nphot, skipped when I[s] == 0
+* bimg_krnl        libsutra/src.cu/sutra_wfs.cu:183-229 (alpha = 0)
+      binimg[x + y*nsub + vx[s] + vy[s]*nsub] = bincube[x + y*npix + s*npix^2]
+
+Arrays are indexed [x, y] (x = fast device axis).
+"""
+import numpy as np
+
+
+def fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft):
+    """Zero-padded complex fields, shape (nvalid, Nfft, Nfft) indexed [s, x, y]."""
+    nvalid = len(istart)
+    E = np.zeros((nvalid, Nfft, Nfft), dtype=np.complex128)
+    ph = np.asarray(phase, dtype=np.float64)
+    pu = np.asarray(pupil, dtype=np.float64)
+    off = np.asarray(offsets, dtype=np.float64)
+    sc = float(np.float32(scale))
+    for s in range(nvalid):
+        i0, j0 = int(istart[s]), int(jstart[s])
+        sub = ph[i0:i0 + pdiam, j0:j0 + pdiam]
+        msk = pu[i0:i0 + pdiam, j0:j0 + pdiam]
+        E[s, :pdiam, :pdiam] = msk * np.exp(1j * (sub * sc - off))
+    return E
+
+
+def hr_images(E):
+    """|FFT2|^2 of each field (forward, unnormalised)."""
+    F = np.fft.fft2(E, axes=(1, 2))
+    return (F.real ** 2 + F.imag ** 2)
+
+
+def bin_images(hr, binmap, npix):
+    """Gather-sum of the HR pixels listed by binmap, returns [px, py, s]."""
+    nvalid, N, _ = hr.shape
+    flat = hr.transpose(0, 2, 1).reshape(nvalid, N * N)   # F-order flat index u + v*N
+    bm = np.asarray(binmap)
+    b = np.zeros((nvalid, npix * npix))
+    for i in range(bm.shape[0]):
+        b += flat[:, bm[i, :]]
+    return b.reshape(nvalid, npix, npix).transpose(2, 1, 0)  # p = px + py*npix -> [px, py, s]
+
+
+def normalise(bincube, fluxPerSub, nphot):
+    """Per-subap flux (intensities) and photometric normalisation."""
+    inten = bincube.sum(axis=(0, 1))
+    out = bincube.copy()
+    nz = inten != 0
+    out[:, :, nz] = out[:, :, nz] * (np.asarray(fluxPerSub, np.float64)[nz] / inten[nz] * float(nphot))
+    return out, inten
+
+
+def binimg_from_cube(bincube, validsubsx, validsubsy, nsub, base=None):
+    """Scatter each sub-aperture spot into the detector image at (vx, vy)."""
+    npix = bincube.shape[0]
+    img = np.zeros((nsub, nsub)) if base is None else np.array(base, dtype=np.float64)
+    for s in range(bincube.shape[2]):
+        vx, vy = int(validsubsx[s]), int(validsubsy[s])
+        img[vx:vx + npix, vy:vy + npix] = bincube[:, :, s]
+    return img
+
+
+def comp_image(phase, pupil, scale, offsets, istart, jstart, validsubsx, validsubsy,
+               binmap, fluxPerSub, nphot, pdiam, Nfft, npix, nxsub):
+    """Noise-free SH image: returns (bincube [px,py,s], intensities_raw, binimg [x,y]).
+
+    ``intensities_raw`` is what the reference leaves in d_intensities: the
+    per-subap sum *before* normalisation.
+    """
+    E = fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft)
+    hr = hr_images(E)
+    b = bin_images(hr, binmap, npix)
+    bn, inten = normalise(b, fluxPerSub, nphot)
+    img = binimg_from_cube(bn, validsubsx, validsubsy, npix * nxsub)
+    return bn, inten, img
+
+
+def brute_dft2(E):
+    """O(N^4) textbook forward DFT of one 2-D field (pins the FFT convention)."""
+    N = E.shape[0]
+    out = np.zeros((N, N), dtype=np.complex128)
+    for k in range(N):
+        for l in range(N):
+            acc = 0j
+            for n in range(N):
+                for m in range(N):
+                    acc += E[n, m] * np.exp(-2j * np.pi * (k * n + l * m) / N)
+            out[k, l] = acc
+    return out
