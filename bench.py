@@ -45,11 +45,13 @@ def sync():
         pass
 
 
-def measure(method, cfg_name, steps, warmup, phase, thr):
+def measure(method, cfg_name, steps, warmup, phase, thr, wg=0):
     import sutra
     from shwfs import configs, system
     c = configs.get(cfg_name)
     s = system.ShSystem(c, method=method, fp64_moments=True, threshold=thr)
+    if wg:
+        s.wfs.set_wg_size(wg)
     s.set_phase(phase)
     slopes = s.rtc.d_control[0].d_centroids
     sutra.run_frames(s.wfs, s.centro, slopes, warmup, False, True)
@@ -96,6 +98,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--methods", default="0,1,2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--wg", type=int, default=0, help="spot-kernel workgroup size (0 = default)")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run a device copy of this many MiB first (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path has no multi-GPU stage); "
@@ -107,6 +112,19 @@ def main():
 
     from shwfs import configs, system
     import sutra  # noqa: F401
+
+    if a.copy_calib_mib > 0:
+        # a plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+        # counters give the scale of the spot kernel's counters
+        import carma
+        ctx = carma.context.get_instance()
+        n = a.copy_calib_mib * (1 << 20) // 4
+        src = carma.obj_float(ctx, np.ones(n, dtype=np.float32))
+        dst = carma.obj_float(ctx, np.zeros(n, dtype=np.float32))
+        for _ in range(3):
+            dst.copy_from(src, n)
+        sync()
+        del src, dst
 
     cfg = system.geometry(configs.get("large"))
     w = cfg["p_wfs"]
@@ -123,7 +141,7 @@ def main():
     results = {}
     for m in [int(x) for x in a.methods.split(",") if x != ""]:
         try:
-            r, s = measure(m, "large", a.steps, a.warmup, phase, thr)
+            r, s = measure(m, "large", a.steps, a.warmup, phase, thr, a.wg)
         except RuntimeError as e:
             print("method %s unavailable: %s" % (METHOD_NAMES[m], e), file=sys.stderr)
             continue

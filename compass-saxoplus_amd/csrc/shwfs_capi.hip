@@ -1,7 +1,9 @@
 // C ABI entry points of libshwfs.so, the standalone centroid kernels, the
 // counter-based noise kernel and the weight broadcast.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "shwfs_device.hpp"
 #include "shwfs_internal.hpp"
@@ -205,7 +207,40 @@ int32_t shwfs_last_error(char* buf, size_t len) {
 
 const char* shwfs_version(void) { return "shwfs-mi355x 0.1 (gfx950)"; }
 
+int32_t shwfs_mfma_probe(const float* d_A, const float* d_B, float* d_D, int32_t method,
+                         void* stream) {
+  if (!d_A || !d_B || !d_D) return fail(SHWFS_ERR_ARG, "mfma_probe: NULL argument");
+  return launch_mfma_probe(d_A, d_B, d_D, method, (hipStream_t)stream);
+}
+
 int32_t shwfs_sh_check_geom(const shwfs_sh_geom* g, int32_t method) { return check_geom(g, method); }
+
+int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t npix,
+                                  int32_t nrebin, int32_t* origin) {
+  if (!h_binmap || !origin || nfft <= 0 || npix <= 0 || nrebin <= 0 || npix * nrebin > nfft)
+    return fail(SHWFS_ERR_ARG, "binmap_block_origin: bad arguments");
+  if (!is_pow2(nfft)) return 0;
+  const int nr2 = nrebin * nrebin;
+  std::vector<int32_t> want(nr2), have(nr2);
+  for (int o = 0; o < nfft; ++o) {
+    bool ok = true;
+    for (int p = 0; p < npix * npix && ok; ++p) {
+      const int px = p % npix, py = p / npix;
+      for (int b = 0; b < nrebin; ++b)
+        for (int a = 0; a < nrebin; ++a)
+          want[a + b * nrebin] = ((o + px * nrebin + a) % nfft) + ((o + py * nrebin + b) % nfft) * nfft;
+      for (int i = 0; i < nr2; ++i) have[i] = h_binmap[i + p * nr2];
+      std::sort(want.begin(), want.end());
+      std::sort(have.begin(), have.end());
+      ok = (want == have);
+    }
+    if (ok) {
+      *origin = o;
+      return 1;
+    }
+  }
+  return 0;
+}
 
 int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, void* stream) {
   int32_t rc = check_geom(g, f ? f->method : 0);
@@ -231,6 +266,13 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, v
   p.npix = g->npix;
   p.nrebin = g->nrebin;
   p.nsub = g->npix * g->nxsub;
+  p.bin_block = g->binmap_block ? 1 : 0;
+  p.bin_origin = g->bin_origin;
+  if (p.bin_block && (g->bin_origin < 0 || g->bin_origin >= g->nfft))
+    return fail(SHWFS_ERR_ARG, "bin_origin outside [0, nfft)");
+  if (f->wg_size != 0 && f->wg_size != 64 && f->wg_size != 128 && f->wg_size != 256)
+    return fail(SHWFS_ERR_ARG, "wg_size must be 0, 64, 128 or 256");
+  p.tpb = f->wg_size;
   p.scale = f->scale;
   p.nphot = f->nphot;
   p.noise = f->noise;

@@ -176,19 +176,44 @@ struct PreTw {
 };
 
 // ---------------------------------------------------------------- phase -> field
-// 2*pi split for Cody-Waite: C1 = float(2*pi), C2 = float(2*pi - C1).
+// sin / cos of theta = phase*scale - offset, accurate for |phase*scale| up to
+// ~1e4 rad:
+//  * the float product keeps its exact rounding error e (FMA two-product);
+//  * q = nearest integer to theta / (pi/2); the multiple q*pi/2 is removed
+//    from the product with a two-constant Cody-Waite split of pi/2
+//    (H1 = float(pi/2), H2 = float(pi/2 - H1)), then the offset and e are
+//    applied, leaving |t| <= pi/4 (+ rounding);
+//  * sin t, cos t by their Taylor series to t^11 / t^10 (truncation < 3e-9 on
+//    [-pi/4, pi/4]), then the quadrant rotation by q.
+__device__ __forceinline__ void sincos_reduced(float t, int qi, float& s, float& c) {
+  const float t2 = t * t;
+  float sp = __builtin_fmaf(t2, 2.75573192e-06f, -1.98412698e-04f);  // 1/9!, -1/7!
+  sp = __builtin_fmaf(t2, sp, 8.33333333e-03f);                      // 1/5!
+  sp = __builtin_fmaf(t2, sp, -1.66666667e-01f);                     // -1/3!
+  const float sv = __builtin_fmaf(t * t2, sp, t);
+  float cp = __builtin_fmaf(t2, -2.75573192e-07f, 2.48015873e-05f);  // -1/10!, 1/8!
+  cp = __builtin_fmaf(t2, cp, -1.38888889e-03f);                     // -1/6!
+  cp = __builtin_fmaf(t2, cp, 4.16666667e-02f);                      // 1/4!
+  cp = __builtin_fmaf(t2, cp, -0.5f);
+  const float cv = __builtin_fmaf(t2, cp, 1.0f);
+  const float a = (qi & 1) ? cv : sv;
+  const float b = (qi & 1) ? sv : cv;
+  s = (qi & 2) ? -a : a;
+  c = ((qi + 1) & 2) ? -b : b;
+}
+
 __device__ __forceinline__ void phase_sincos(float phase, float scale, float offset, float& s,
                                              float& c) {
-  constexpr float kInv2Pi = 0.15915494309189535f;
-  constexpr float kC1 = 6.28318548202514648f;        // float nearest 2*pi
-  constexpr float kC2 = -1.7484556000744883e-07f;    // 2*pi - kC1
-  float p = phase * scale;
-  float e = __builtin_fmaf(phase, scale, -p);  // exact rounding error of the product
-  float k = __builtin_rintf(p * kInv2Pi);
-  float r = __builtin_fmaf(-k, kC1, p);
-  r = __builtin_fmaf(-k, kC2, r);
-  r = (r + e) - offset;
-  sincosf(r, &s, &c);
+  constexpr float kInvHalfPi = 0.636619772367581343f;
+  constexpr float kH1 = 1.57079637050628662f;      // float nearest pi/2
+  constexpr float kH2 = -4.37113900018624283e-08f;  // pi/2 - kH1
+  const float p = phase * scale;
+  const float e = __builtin_fmaf(phase, scale, -p);  // exact rounding error of the product
+  const float q = __builtin_rintf((p - offset) * kInvHalfPi);
+  float t = __builtin_fmaf(-q, kH1, p);
+  t = __builtin_fmaf(-q, kH2, t);
+  t = (t - offset) + e;
+  sincos_reduced(t, (int)q, s, c);
 }
 
 // ---------------------------------------------------------------- Philox4x32-10

@@ -1,15 +1,17 @@
-// Plain FP32 spot-image kernel: one workgroup of 256 lanes handles 256/N
-// sub-apertures (N = FFT support).  Per sub-aperture, entirely on chip:
+// Plain FP32 spot-image kernel.  A workgroup of TPB lanes handles TPB/N
+// sub-apertures (N = FFT support); one group of N consecutive lanes owns one
+// sub-aperture and everything below stays on chip:
 //
-//   0. coalesced load of the pdiam x pdiam phase / pupil window, compensated
-//      phase reduction + sincos, field E(x, y) -> LDS
+//   0. the pdiam x pdiam phase / pupil window is loaded coalesced, every load
+//      of a lane issued before the first use; compensated phase reduction +
+//      sincos; field E(x, y) -> LDS
 //   1. row transforms along x: lane (y, r) computes the M = N/R point FFT of
 //      E(., y) * W_N^(x r), i.e. the outputs kx = R k' + r (R = 2 when
-//      pdiam <= N/2: the zero padding halves every transform), -> LDS G(kx, y)
+//      pdiam <= N/2: the zero padding halves every transform) -> LDS G(kx, y)
 //   2. column transforms along y: lane kx computes all N outputs ky of its
 //      column in registers, |F|^2 -> LDS hr(kx + ky N)
-//   3. the shared tail (binning via binmap, flux, normalisation, noise,
-//      bincube / binimg writes, optional fused centroider).
+//   3. the shared tail (binning, flux, normalisation, noise, bincube /
+//      binimg writes, optional fused centroider).
 //
 // Semantics: libsutra/src.cu/sutra_wfs.cu:27-53 (camplipup_krnl), cuFFT
 // forward C2C (libcarma/src.cpp/carma_fft.cpp:164-180), sutra_utils.cu:144-193,
@@ -20,9 +22,9 @@ namespace shwfs {
 
 template <int N, int R>
 struct FftLayout {
-  static constexpr int M = N / R;      // row transform size
-  static constexpr int S = 256 / N;    // sub-apertures per workgroup
-  static constexpr int PS = M + 1;     // G row stride in float2 (odd: fewer bank conflicts)
+  static constexpr int M = N / R;    // row transform size
+  static constexpr int PS = M + 1;   // G row stride in float2 (odd: fewer bank conflicts)
+  static constexpr int EPL = (M * M + N - 1) / N;  // field elements per lane (pdiam <= M)
   static __host__ __device__ int pe(int pdiam) { return pdiam + 1; }  // E row stride
   static __host__ __device__ int slot_floats(int pdiam, int npix) {
     int e = pdiam * pe(pdiam) * 2;
@@ -35,11 +37,11 @@ struct FftLayout {
   }
 };
 
-template <int N, int R, bool NOISE>
-__global__ __launch_bounds__(256) void sh_image_fft_kernel(ImageParams P) {
+template <int N, int R, int TPB, bool NOISE>
+__global__ __launch_bounds__(TPB) void sh_image_fft_kernel(ImageParams P) {
   using L = FftLayout<N, R>;
   constexpr int M = L::M;
-  constexpr int S = L::S;
+  constexpr int S = TPB / N;
   constexpr int PS = L::PS;
   extern __shared__ float smem[];
 
@@ -47,12 +49,14 @@ __global__ __launch_bounds__(256) void sh_image_fft_kernel(ImageParams P) {
   const int PE = L::pe(pdiam);
   const int slot_f = L::slot_floats(pdiam, P.npix);
   int* s_binmap = (int*)(smem + S * slot_f);
-  const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
-  for (int i = threadIdx.x; i < nbm; i += 256) s_binmap[i] = P.binmap[i];
+  if (!P.bin_block) {
+    const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
+    for (int i = threadIdx.x; i < nbm; i += TPB) s_binmap[i] = P.binmap[i];
+  }
 
   const int slot = threadIdx.x / N;
   const int lane = threadIdx.x % N;
-  const int s = blockIdx.x * S + slot;
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * S + slot;
   const bool valid = s < P.nvalid;
   float* base = smem + slot * slot_f;
   float2* E = (float2*)base;  // E[y*PE + x]
@@ -61,18 +65,7 @@ __global__ __launch_bounds__(256) void sh_image_fft_kernel(ImageParams P) {
   float* bin = base + (slot_f - ((P.npix * P.npix + 3) & ~3));
 
   // ---- 0. fields
-  if (valid) {
-    const int i0 = P.istart[s], j0 = P.jstart[s];
-    const int np2 = pdiam * pdiam;
-    for (int idx = lane; idx < np2; idx += N) {
-      const int x = idx % pdiam, y = idx / pdiam;
-      const size_t g = (size_t)(i0 + x) + (size_t)(j0 + y) * P.npup;
-      float sn, cs;
-      phase_sincos(P.phase[g], P.scale, P.offsets[idx], sn, cs);
-      const float m = P.pupil[g];
-      E[y * PE + x] = make_float2(cs * m, sn * m);
-    }
-  }
+  load_fields<N, L::EPL, false>(P, s, valid, lane, (float*)E, PE, 0);
   __syncthreads();
 
   // ---- 1. row transforms
@@ -126,23 +119,33 @@ __global__ __launch_bounds__(256) void sh_image_fft_kernel(ImageParams P) {
   __syncthreads();
 
   // ---- 3. tail
-  spot_tail<N, NOISE>(P, s, valid, lane, hr, bin, s_binmap);
+  spot_tail<N, N, NOISE>(P, s, valid, lane, hr, bin, s_binmap, 1.0f);
+}
+
+template <int N, int R, int TPB>
+static int32_t launch_fft_nrt(const ImageParams& p, hipStream_t st) {
+  using L = FftLayout<N, R>;
+  constexpr int S = TPB / N;
+  const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * p.npix * p.npix;
+  const size_t lds = sizeof(float) * ((size_t)S * L::slot_floats(p.pdiam, p.npix) + nbm);
+  if (lds > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "spot kernel: LDS footprint exceeds the per-workgroup limit");
+  const int grid = (p.nvalid + S - 1) / S;
+  if (p.noise > -1.0f)
+    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, true>), dim3(grid), dim3(TPB), lds, st, p);
+  else
+    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, false>), dim3(grid), dim3(TPB), lds, st, p);
+  SHWFS_LAUNCHED("sh_image_fft_kernel");
+  return SHWFS_OK;
 }
 
 template <int N, int R>
 static int32_t launch_fft_nr(const ImageParams& p, hipStream_t st) {
-  using L = FftLayout<N, R>;
-  const int nbm = p.nrebin * p.nrebin * p.npix * p.npix;
-  const size_t lds = sizeof(float) * ((size_t)L::S * L::slot_floats(p.pdiam, p.npix) + nbm);
-  if (lds > max_lds_per_block())
-    return fail(SHWFS_ERR_UNSUPPORTED, "spot kernel: LDS footprint exceeds the per-workgroup limit");
-  const int grid = (p.nvalid + L::S - 1) / L::S;
-  if (p.noise > -1.0f)
-    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, true>), dim3(grid), dim3(256), lds, st, p);
-  else
-    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, false>), dim3(grid), dim3(256), lds, st, p);
-  SHWFS_LAUNCHED("sh_image_fft_kernel");
-  return SHWFS_OK;
+  // workgroup size: 64 lanes by default (more, smaller workgroups spread the
+  // sub-apertures evenly over the 256 CUs); 256 on request (tuning)
+  if (p.tpb == 256) return launch_fft_nrt<N, R, 256>(p, st);
+  if (p.tpb == 128 && N <= 128) return launch_fft_nrt<N, R, 128>(p, st);
+  return launch_fft_nrt<N, R, 64>(p, st);
 }
 
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t st) {

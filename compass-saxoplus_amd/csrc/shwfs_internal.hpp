@@ -61,6 +61,9 @@ struct ImageParams {
   const float* flux;
   const int* binmap;
   int npup, nvalid, pdiam, npix, nrebin, nsub;
+  int bin_block;   // 1: binmap is the (shifted) block form, indices computed from bin_origin
+  int bin_origin;
+  int tpb;         // threads per workgroup of the spot kernel (0 = default)
   float scale, nphot, noise;
   unsigned long long seed;
   unsigned call;
@@ -78,6 +81,7 @@ struct ImageParams {
 
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s);
+int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, hipStream_t s);
 int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
                           hipStream_t s);
 

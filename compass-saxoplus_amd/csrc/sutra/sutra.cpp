@@ -265,6 +265,8 @@ SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
       noise_call(0),
       arrays_loaded(false),
       own_plan(false),
+      bin_block(0),
+      bin_origin(0),
       noise_this_call(true) {
   if (fakecam_) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
 }
@@ -396,6 +398,11 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
   d_validpuppixx->host2device(istart);
   d_validpuppixy->host2device(jstart);
   d_ftkernel->host2device(kernel);
+  int32_t origin = 0;
+  int32_t rc = shwfs_binmap_block_origin(binmap, (int32_t)nfft, (int32_t)npix, (int32_t)nrebin, &origin);
+  if (rc < 0) shwfs_throw_if(rc, "load_arrays");
+  bin_block = rc;
+  bin_origin = rc ? origin : 0;
   arrays_loaded = true;
   return EXIT_SUCCESS;
 }
@@ -418,6 +425,8 @@ shwfs_sh_geom SutraWfsSH::geom() const {
     g.d_offsets = d_offsets->get_data();
     g.d_fluxPerSub = d_fluxPerSub->get_data();
     g.d_binmap = d_binmap->get_data();
+    g.binmap_block = bin_block;
+    g.bin_origin = bin_origin;
   }
   return g;
 }
@@ -433,6 +442,7 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
   f.seed = noise_seed;
   f.call = noise_call;
   f.method = get_method();
+  f.wg_size = get_wg_size();
   f.d_bincube = d_bincube->get_data();
   f.d_binimg = d_binimg ? d_binimg->get_data() : nullptr;
   f.d_intensities = d_intensities->get_data();
@@ -506,6 +516,18 @@ int32_t SutraWfsSH::set_field_stop(PlanMap& campli_plans, float* field_stop, int
   (void)field_stop;
   (void)N;
   throw std::runtime_error("field stops are not part of this build");
+}
+
+int32_t SutraWfsSH::set_wg_size(int32_t wg_size) {
+  if (wg_size != 0 && wg_size != 64 && wg_size != 128 && wg_size != 256)
+    throw std::runtime_error("set_wg_size: 0, 64, 128 or 256");
+  if (!campli_plan) throw std::runtime_error("set_wg_size: allocate_buffers has not run");
+  if (!own_plan) {
+    campli_plan = new ShwfsPlan(*campli_plan);
+    own_plan = true;
+  }
+  campli_plan->wg_size = wg_size;
+  return EXIT_SUCCESS;
 }
 
 int32_t SutraWfsSH::set_method(int32_t method) {

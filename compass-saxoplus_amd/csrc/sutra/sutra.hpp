@@ -28,6 +28,7 @@ struct ShwfsPlan {
   int32_t nfft = 0;
   int32_t nvalid = 0;
   int32_t method = SHWFS_METHOD_FFT_FP32;
+  int32_t wg_size = 0;  // threads per workgroup of the spot kernel (0 = library default)
 };
 using PlanMap = map<vector<int32_t>, ShwfsPlan*>;
 
@@ -167,6 +168,7 @@ class SutraWfsSH : public SutraWfs {
   uint32_t noise_call;
   bool arrays_loaded;
   bool own_plan;  // campli_plan was cloned by set_method and belongs to this WFS
+  int32_t bin_block, bin_origin;  // binmap analysis done at load_arrays
 
   SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel, CarmaObj<hipFloatComplex>* d_camplipup,
              CarmaObj<hipFloatComplex>* d_camplifoc, CarmaObj<hipFloatComplex>* d_fttotim,
@@ -191,6 +193,8 @@ class SutraWfsSH : public SutraWfs {
   // Spot computation method (SHWFS_METHOD_*); FP32 FFT by default.
   int32_t set_method(int32_t method);
   int32_t get_method() const { return campli_plan ? campli_plan->method : SHWFS_METHOD_FFT_FP32; }
+  int32_t set_wg_size(int32_t wg_size);
+  int32_t get_wg_size() const { return campli_plan ? campli_plan->wg_size : 0; }
   // Geometry descriptor handed to the C ABI.
   shwfs_sh_geom geom() const;
   // Frame arguments of the next comp_image (advances nothing).

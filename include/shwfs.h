@@ -64,6 +64,10 @@ typedef struct shwfs_sh_geom {
   const float* d_offsets;      /* [nphase^2] half-pixel tip-tilt phase (halfxy) */
   const float* d_fluxPerSub;   /* [nvalid] */
   const int32_t* d_binmap;     /* [nrebin^2 * npix^2] HR flat index per binned pixel */
+  int32_t binmap_block;        /* 1: d_binmap is the block form found by
+                                  shwfs_binmap_block_origin (kernels compute the
+                                  indices instead of reading the table) */
+  int32_t bin_origin;          /* origin returned by shwfs_binmap_block_origin */
 } shwfs_sh_geom;
 
 /* A centroider applied to a detector image (or fused after the spot stage).
@@ -97,6 +101,7 @@ typedef struct shwfs_frame_args {
   uint64_t seed;             /* Philox key */
   uint32_t call;             /* Philox call counter (advance once per frame) */
   int32_t method;            /* SHWFS_METHOD_* */
+  int32_t wg_size;           /* threads per workgroup of the spot kernel: 0 (default), 64, 128, 256 */
   float* d_bincube;          /* [npix^2 * nvalid] */
   float* d_binimg;           /* [(npix*nxsub)^2] or NULL to skip the image */
   float* d_intensities;      /* [nvalid] raw per-subap flux before normalisation */
@@ -115,6 +120,16 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* geom, const shwfs_frame_args* f
 
 /* Validates a geometry against what the kernels implement (no launch). */
 int32_t shwfs_sh_check_geom(const shwfs_sh_geom* geom, int32_t method);
+
+/* Host-side analysis of a binmap (no GPU): returns 1 and sets *origin when,
+ * for every binned pixel p = px + py*npix, the set of its nrebin^2 HR indices
+ * is {((o + px*nrebin + a) mod nfft) + ((o + py*nrebin + b) mod nfft) * nfft},
+ * the form geom_init.py:872-899 builds for an NGS (centred window, fftshift);
+ * returns 0 otherwise, negative on bad arguments.  Block form lets the
+ * kernels skip the table; the summation order inside a wrapped block may
+ * differ from the table's (float rounding only). */
+int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t npix,
+                                  int32_t nrebin, int32_t* origin);
 
 /* ---- centroiders on a detector image ---------------------------------------
  * Replaces get_centroids of libsutra/src.cu/sutra_centroider_cog.cu:83-150,
@@ -150,6 +165,12 @@ int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
                           int32_t nvalid, float alpha, void* stream);
 
 /* ---- diagnostics ------------------------------------------------------------ */
+/* One 16x32 by 32x16 product on the matrix cores with the operand layouts the
+ * DFT kernels use: d_A (16x32) and d_B (32x16) row-major float, rounded to the
+ * method's input type (fp16, or bf16 hi/lo for BF16X3); d_D (16x16) row-major.
+ * No counterpart in the reference: it checks the MFMA lane layout. */
+int32_t shwfs_mfma_probe(const float* d_A, const float* d_B, float* d_D, int32_t method,
+                         void* stream);
 int32_t shwfs_last_error(char* buf, size_t len);
 const char* shwfs_version(void);
 

@@ -1,0 +1,67 @@
+"""C ABI checks that need no GPU: the library loads, exports every declared
+symbol, validates its arguments on the host and reports errors."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import capi
+from oracle import geometry as og
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = capi.declared_functions()
+    assert {"shwfs_sh_comp_image", "shwfs_centroids", "shwfs_prng", "shwfs_last_error"} <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_version_and_error_reporting(lib):
+    assert b"gfx950" in lib.shwfs_version()
+    g = capi.ShGeom()
+    g.nvalid = 4
+    g.nphase, g.npup, g.nfft, g.ntot, g.npix, g.nrebin, g.nxsub = 8, 68, 12, 12, 6, 2, 8
+    assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_UNSUPPORTED
+    assert "nfft" in capi.last_error(lib)
+    g.nfft = g.ntot = 16
+    assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_STATE   # tables not loaded
+    g.ntot = 32
+    assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_UNSUPPORTED
+    assert lib.shwfs_sh_check_geom(None, 0) == capi.SHWFS_ERR_ARG
+
+
+def test_null_and_size_arguments_rejected_without_launch(lib):
+    assert lib.shwfs_sh_comp_image(None, None, None) == capi.SHWFS_ERR_ARG
+    a = capi.CentroidArgs()
+    assert lib.shwfs_centroids(None, None, C.byref(a), None) == capi.SHWFS_ERR_ARG
+    a.nvalid, a.npix, a.size = 4, 8, 64
+    assert lib.shwfs_centroids(C.c_void_p(16), None, C.byref(a), None) == capi.SHWFS_ERR_STATE
+    assert lib.shwfs_prng(None, 10, b"P", 1.0, 0.0, 1, 0, 0, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_prng(C.c_void_p(16), 10, b"X", 1.0, 0.0, 1, 0, 0, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_fill_weights(None, None, 8, 64, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_calibrate(None, None, None, None, None, 4, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_slopes_geom(1, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), None, 8, 4,
+                                 1.0, None) == capi.SHWFS_ERR_UNSUPPORTED
+    assert lib.shwfs_mfma_probe(None, None, None, 1, None) == capi.SHWFS_ERR_ARG
+
+
+@pytest.mark.parametrize("Ntot,npix,nrebin", [(16, 8, 2), (32, 8, 4), (16, 6, 2), (64, 16, 4), (16, 5, 3)])
+def test_binmap_block_analysis_recognises_the_ngs_binmap(lib, Ntot, npix, nrebin):
+    bm = np.ascontiguousarray(og.binmap_table(Ntot, npix, nrebin).T)  # [p][i] = F-order of [i, p]
+    o = C.c_int32(-1)
+    rc = lib.shwfs_binmap_block_origin(bm.ctypes.data_as(C.c_void_p), Ntot, npix, nrebin, C.byref(o))
+    assert rc == 1
+    # the origin reproduces the table: block of pixel 0 starts at (o, o)
+    w = nrebin * npix
+    indi = (Ntot - w) // 2 + (1 if (w % 2) != (Ntot % 2) else 0)
+    assert o.value == (indi + Ntot // 2) % Ntot
+    # a permuted table is not block form
+    bad = bm.copy()
+    bad[[0, 1]] = bad[[1, 0]]
+    assert lib.shwfs_binmap_block_origin(bad.ctypes.data_as(C.c_void_p), Ntot, npix, nrebin,
+                                         C.byref(o)) == 0
