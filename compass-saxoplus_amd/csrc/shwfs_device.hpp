@@ -305,4 +305,53 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Whole-wave sums with DPP moves (no LDS crossbar): quad exchanges, half-row
+// and row mirrors give every lane its 16-lane row sum; row_bcast:15 / :31
+// carry the row sums up to lane 63, which is read back as a scalar.
+__device__ __forceinline__ int dpp_int(int v, int ctrl_sel) {
+  switch (ctrl_sel) {
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    case 3: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+    case 4: return __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false); // row_bcast:31
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v += __int_as_float(dpp_int(__float_as_int(v), c));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_int((int)(b & 0xffffffffLL), c);
+    const int hi = dpp_int((int)(b >> 32), c);
+    v += __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v = fmaxf(v, __int_as_float(dpp_int(__float_as_int(v), c)));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Sum over groups of G lanes; G == 64 uses the DPP wave sum.
+template <int G, typename T>
+__device__ __forceinline__ T lane_group_sum(T v) {
+  if constexpr (G == 64)
+    return wave_sum(v);
+  else
+    return group_sum<G>(v);
+}
+
 }  // namespace shwfs

@@ -78,9 +78,9 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
 template <int LANES, typename Acc>
 __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, bool valid, int lane,
                                                 Acc I, Acc Sx, Acc Sy) {
-  I = group_sum<LANES>(I);
-  Sx = group_sum<LANES>(Sx);
-  Sy = group_sum<LANES>(Sy);
+  I = lane_group_sum<LANES>(I);
+  Sx = lane_group_sum<LANES>(Sx);
+  Sy = lane_group_sum<LANES>(Sy);
   if (valid && lane == 0) {
     const int ix = P.slope_order == 0 ? s : 2 * s;
     const int iy = P.slope_order == 0 ? s + P.nvalid : 2 * s + 1;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
       }
     }
   }
-  const float I = group_sum<LANES>(part);
+  const float I = lane_group_sum<LANES>(part);
   const float fl = valid ? P.flux[s] : 0.0f;
   const int vx = valid ? P.validsubsx[s] : 0;
   const int vy = valid ? P.validsubsy[s] : 0;
