@@ -21,7 +21,9 @@ LIB = os.path.join(HERE, "lib")
 OBJ = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SHWFS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["shwfs_capi.hip", "shwfs_image_fft.hip", "shwfs_image_dft.hip"]
+HIP_SOURCES = ["shwfs_capi.hip", "shwfs_image_fft.hip", "shwfs_image_dft.hip",
+               "shwfs_dft_bf16x3_q1.hip", "shwfs_dft_bf16x3_q2.hip", "shwfs_dft_fp16_q1.hip",
+               "shwfs_dft_fp16_q2.hip"]
 CPP_SOURCES = ["carma/carma.cpp", "sutra/sutra.cpp", "sutra/sutra_centroider.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",

@@ -1,0 +1,6 @@
+// MFMA spot kernels, bf16x3 operands, padded phase side 16.
+#include "shwfs_image_dft.hpp"
+
+namespace shwfs {
+template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 1>(const ImageParams&, int, hipStream_t);
+}  // namespace shwfs

@@ -1,0 +1,439 @@
+// MFMA spot kernels: the zero-padded 2-D FFT of each sub-aperture field
+// becomes two dense products with the pruned DFT matrix on the matrix cores.
+// With W[k][n] = exp(-2 pi i k n / N) (k < N, n < P16 = 16 Q >= pdiam; the
+// zero rows of the padded field are never multiplied) and the real stacking
+//     A_W = [[Wr, -Wi], [Wi, Wr]]                 (2N x 2 P16)
+// the two stages are
+//     stage A   [Gr | Gi]  = [Er | Ei] A_W^T       rows y,         cols (re/im, kx)
+//     stage B   [Fr ; Fi]  = A_W [Gr ; Gi]         rows (re/im, ky), cols kx
+// on v_mfma_f32_16x16x32 tiles with FP32 accumulation, one wave per
+// sub-aperture.  A_W's A-operand fragments are, register for register, the
+// B-operand fragments of A_W^T, so one prebuilt table feeds both stages.
+// The field is evaluated with x fastest over the lanes (coalesced window
+// reads) and reaches stage A's operand layout through one LDS pass; stage A's
+// accumulators become stage B's operand in registers, through two cross-row
+// lane swaps (v_permlane32_swap, v_permlane16_swap).
+// Operand precisions:
+//   * SHWFS_METHOD_DFT_BF16X3: every operand x = hi + lo (bf16 each),
+//     product = hi*hi + hi*lo + lo*hi (lo*lo dropped), 3 MFMA per tile;
+//   * SHWFS_METHOD_DFT_FP16: fp16 operands, 1 MFMA per tile; the field of a
+//     sub-aperture is scaled by a power of two s such that the bound
+//     pdiam * max|E| >= max|G| lands in [2^14, 2^15), and the flux by 1/s^2.
+// The field uses the hardware sine and cosine on the angle in turns (about
+// 1e-6 absolute error): these methods trade accuracy for speed and state their
+// error against the float64 oracle.  Binning, noise, detector image and
+// centroids follow the same definitions as the FFT kernel.
+#pragma once
+
+#include <map>
+#include <mutex>
+
+#include "shwfs_mfma.hpp"
+#include "shwfs_tail.hpp"
+
+namespace shwfs {
+
+template <int N, int Q>
+struct DftLayout {
+  static constexpr int P16 = 16 * Q;             // padded phase side (pdiam <= P16 <= N)
+  static constexpr int MT = N / 8;               // 16-row tiles of A_W: (re/im, k) = 2N rows
+  static constexpr int KS = Q;                   // 32-wide K steps of A_W: (re/im, n) = 2 P16
+  static constexpr int NTB = N / 16;             // kx tiles
+  static constexpr bool kAwInRegs = MT * KS <= 8;  // operand table held in registers
+  static constexpr int PE = P16 + 4;             // field row stride (floats), 16-byte rows
+  static constexpr int EPLANE = P16 * PE;        // Er plane, then Ei plane
+  static constexpr int EPL = (P16 * P16 + 63) / 64;  // field elements per lane
+  // LDS per wave: the field planes; the general-binmap tail adds the |F|^2
+  // image (over the planes) and the spot
+  static __host__ __device__ int wave_floats(int npix, bool gather) {
+    const int spot = (npix * npix + 3) & ~3;
+    if (!gather) return 2 * EPLANE > spot ? 2 * EPLANE : spot;
+    return (2 * EPLANE > N * N ? 2 * EPLANE : N * N) + spot;
+  }
+};
+
+template <int PREC>
+struct Frag;
+template <>
+struct Frag<SHWFS_METHOD_DFT_FP16> {
+  f16x8 v;
+  __device__ void set(int e, float x) { v[e] = (_Float16)x; }
+};
+template <>
+struct Frag<SHWFS_METHOD_DFT_BF16X3> {
+  bf16x8 hi, lo;
+  __device__ void set(int e, float x) {
+    __bf16 h, l;
+    split_bf16(x, h, l);
+    hi[e] = h;
+    lo[e] = l;
+  }
+};
+
+__device__ __forceinline__ f32x4 mma(const Frag<SHWFS_METHOD_DFT_FP16>& a,
+                                     const Frag<SHWFS_METHOD_DFT_FP16>& b, f32x4 c) {
+  return mfma_f16(a.v, b.v, c);
+}
+__device__ __forceinline__ f32x4 mma(const Frag<SHWFS_METHOD_DFT_BF16X3>& a,
+                                     const Frag<SHWFS_METHOD_DFT_BF16X3>& b, f32x4 c) {
+  c = mfma_bf16(a.lo, b.hi, c);
+  c = mfma_bf16(a.hi, b.lo, c);
+  return mfma_bf16(a.hi, b.hi, c);
+}
+
+// Element e of lane l of the A_W fragment (mt, ks).
+template <int N, int Q>
+__device__ __forceinline__ float aw_value(int mt, int ks, int lane, int e) {
+  constexpr int P16 = 16 * Q;
+  const int m = 16 * mt + (lane & 15);
+  const int k = 32 * ks + 8 * (lane >> 4) + e;
+  const bool row_im = m >= N, col_im = k >= P16;
+  const int kx = row_im ? m - N : m;
+  const int x = col_im ? k - P16 : k;
+  const int idx = (kx * x) & (N - 1);
+  const float c = kTw<N>.c[idx];   // cos(2 pi idx / N)
+  const float s = kTw<N>.s[idx];   // -sin(2 pi idx / N)
+  if (!row_im) return col_im ? -s : c;  // [Wr, -Wi] with Wi = -sin
+  return col_im ? c : s;                // [Wi,  Wr]
+}
+
+// The A_W operand as per-lane fragments, built once per device and kept in a
+// small device table.
+template <int N, int Q, int PREC>
+__global__ void build_wfrag_kernel(Frag<PREC>* out) {
+  using L = DftLayout<N, Q>;
+  const int lane = threadIdx.x;
+  for (int mt = 0; mt < L::MT; ++mt)
+    for (int ks = 0; ks < L::KS; ++ks) {
+      Frag<PREC> f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) f.set(e, aw_value<N, Q>(mt, ks, lane, e));
+      out[(mt * L::KS + ks) * 64 + lane] = f;
+    }
+}
+
+template <int N, int Q, int PREC>
+static const Frag<PREC>* wfrag_table(hipStream_t st) {
+  static std::mutex mu;
+  static std::map<int, Frag<PREC>*> tables;  // device -> table
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tables.find(dev);
+  if (it != tables.end()) return it->second;
+  using L = DftLayout<N, Q>;
+  Frag<PREC>* t = nullptr;
+  if (hipMalloc((void**)&t, sizeof(Frag<PREC>) * L::MT * L::KS * 64) != hipSuccess) return nullptr;
+  hipLaunchKernelGGL((build_wfrag_kernel<N, Q, PREC>), dim3(1), dim3(64), 0, st, t);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipFree(t);
+    return nullptr;
+  }
+  tables[dev] = t;
+  return t;
+}
+
+// Source of the A_W fragments: registers (small tables) or the device table
+// (reads that stay in L1/L2).
+template <int N, int Q, int PREC, bool REGS = DftLayout<N, Q>::kAwInRegs>
+struct AwOperand {
+  using L = DftLayout<N, Q>;
+  Frag<PREC> f[L::MT][L::KS];
+  __device__ __forceinline__ AwOperand(const Frag<PREC>* t, int lane) {
+#pragma unroll
+    for (int mt = 0; mt < L::MT; ++mt)
+#pragma unroll
+      for (int ks = 0; ks < L::KS; ++ks) f[mt][ks] = t[(mt * L::KS + ks) * 64 + lane];
+  }
+  __device__ __forceinline__ Frag<PREC> get(int mt, int ks) const { return f[mt][ks]; }
+};
+template <int N, int Q, int PREC>
+struct AwOperand<N, Q, PREC, false> {
+  using L = DftLayout<N, Q>;
+  const Frag<PREC>* t;
+  __device__ __forceinline__ AwOperand(const Frag<PREC>* tab, int lane) : t(tab + lane) {}
+  __device__ __forceinline__ Frag<PREC> get(int mt, int ks) const {
+    return t[(mt * L::KS + ks) * 64];
+  }
+};
+
+// Two accumulator tiles X, Y of 16 rows each (D layout: lane j + 16 (i/4)
+// holds rows 4 (i/4) .. +3 of column j) -> one B operand of 32 rows
+// [X ; Y] (lane j + 16 (k/8) holds rows 8 (k/8) .. +7 of column j).  With
+// row groups Xg, Yg (g = lane >> 4): permlane32_swap gives
+// X' = [X0 X1 Y0 Y1], Y' = [X2 X3 Y2 Y3]; permlane16_swap then gives
+// X'' = [X0 X2 Y0 Y2], Y'' = [X1 X3 Y1 Y3], i.e. (elements 0-3, 4-7) of
+// group g = (X'', Y'') = rows (8g .. 8g+3, 8g+4 .. 8g+7) of [X ; Y].
+template <int PREC>
+__device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x4& Y, float scale) {
+  Frag<PREC> f;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[v]), __float_as_uint(Y[v]),
+                                                    false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(a[0], a[1], false, false);
+    f.set(v, __uint_as_float(b[0]) * scale);
+    f.set(4 + v, __uint_as_float(b[1]) * scale);
+  }
+  return f;
+}
+
+// Register binning of the |F|^2 values a lane holds after stage B (kx of the
+// lane, four consecutive ky per accumulator): the NR x NR block of a binned
+// pixel is NR consecutive v of one lane times NR consecutive lanes (NR in
+// {1, 2, 4}, block origin a multiple of NR).  Summation: within the lane
+// over ky, then across lanes over kx.
+template <int N, int NR>
+struct RegBin {
+  static constexpr int MT = N / 8, NTB = N / 16;
+  static constexpr int VPL = (MT / 2) * NTB * (4 / NR);  // binned values a lane may own
+};
+
+// Spot, detector pixels and centroid of sub-aperture s (< nvalid, uniform
+// over the wave).  smem: the wave's LDS slot (general-binmap tail only).
+template <int N, int Q, int PREC, bool NOISE, int NR>
+__device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
+                                          float* smem, const int* s_binmap,
+                                          const AwOperand<N, Q, PREC>& aw) {
+  using L = DftLayout<N, Q>;
+  using F = Frag<PREC>;
+  const int pdiam = P.pdiam;
+
+  // ---- field: coalesced window loads (x fastest over lanes) -> Er, Ei planes
+  // in LDS -> stage A operand, A_E[y][k] with y = 16 ma + (lane & 15) and
+  // k = 32 ks + 8 (lane >> 4) + e; k < P16 holds Er at x = k, k >= P16 holds
+  // Ei at x = k - P16 (0 outside the pdiam window)
+  load_fields<64, L::EPL, true, true>(P, s, true, lane, smem, L::PE, L::EPLANE);
+  __syncthreads();
+  const bool full = (pdiam == L::P16);
+  float ev[Q][Q][8];
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+    for (int ks = 0; ks < Q; ++ks) {
+      const int y = 16 * ma + (lane & 15);
+      const int k0 = 32 * ks + 8 * (lane >> 4);
+      const int im = k0 >= L::P16 ? 1 : 0;
+      const int x0 = k0 - im * L::P16;
+      const float* src = smem + im * L::EPLANE + y * L::PE + x0;
+      const float4 a = *reinterpret_cast<const float4*>(src);
+      const float4 b = *reinterpret_cast<const float4*>(src + 4);
+      const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        ev[ma][ks][e] = (full || (x0 + e < pdiam && y < pdiam)) ? v[e] : 0.0f;
+    }
+  float escale = 1.0f;
+  if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
+    float amax = 0.0f;
+#pragma unroll
+    for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+      for (int ks = 0; ks < Q; ++ks)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) amax = fmaxf(amax, fabsf(ev[ma][ks][e]));
+    amax = wave_max(amax) * (float)pdiam;  // >= max |G|
+    if (amax > 0.0f) escale = ldexpf(1.0f, 14 - ilogbf(amax));
+  }
+  F ae[Q][Q];
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+    for (int ks = 0; ks < Q; ++ks)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ae[ma][ks].set(e, ev[ma][ks][e] * escale);
+
+  // ---- stage A: G (rows y, cols (re/im, kx)) = E A_W^T
+  f32x4 ga[Q][L::MT];
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+    for (int nt = 0; nt < L::MT; ++nt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
+      ga[ma][nt] = acc;
+    }
+  const float inten_scale = 1.0f / (escale * escale);
+
+  // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
+  // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles
+  using RB = RegBin<N, (NR > 0 ? NR : 1)>;
+  float bv[RB::VPL];
+  int bx[RB::VPL], by[RB::VPL];
+  const int o = P.bin_origin, npix = P.npix;
+  float* hr = smem;  // general-binmap tail: |F|^2 over the field planes
+  if constexpr (NR == 0) __syncthreads();
+#pragma unroll
+  for (int nt = 0; nt < L::NTB; ++nt) {
+    F bg[L::KS];
+#pragma unroll
+    for (int ks = 0; ks < L::KS; ++ks) {
+      constexpr int H = L::MT / 2;  // im column tiles start at H
+      const int r0 = 32 * ks, r1 = 32 * ks + 16;
+      const f32x4& X = ga[(r0 % L::P16) / 16][(r0 / L::P16) * H + nt];
+      const f32x4& Y = ga[(r1 % L::P16) / 16][(r1 / L::P16) * H + nt];
+      bg[ks] = rows_to_operand<PREC>(X, Y, 1.0f);
+    }
+    f32x4 fb[L::MT];
+#pragma unroll
+    for (int mt = 0; mt < L::MT; ++mt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < L::KS; ++ks) acc = mma(aw.get(mt, ks), bg[ks], acc);
+      fb[mt] = acc;
+    }
+    const int kx = 16 * nt + (lane & 15);
+    if constexpr (NR == 0) {
+#pragma unroll
+      for (int mt = 0; mt < L::MT / 2; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ky = 16 * mt + 4 * (lane >> 4) + v;
+          const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
+          hr[kx + ky * N] = fr * fr + fi * fi;
+        }
+    } else {
+      const int u = (kx - o) & (N - 1);
+#pragma unroll
+      for (int mt = 0; mt < L::MT / 2; ++mt) {
+        float iv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
+          iv[v] = fr * fr + fi * fi;
+        }
+#pragma unroll
+        for (int g = 0; g < 4 / NR; ++g) {
+          float b = 0.0f;
+#pragma unroll
+          for (int v = 0; v < NR; ++v) b += iv[g * NR + v];
+          if constexpr (NR >= 2) b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
+          if constexpr (NR >= 4) b += __int_as_float(dpp_int(__float_as_int(b), 1));  // lane ^ 2
+          const int wv = (16 * mt + 4 * (lane >> 4) + g * NR - o) & (N - 1);
+          const int px = u / NR, py = wv / NR;
+          const bool own = ((lane & (NR - 1)) == 0) && px < npix && py < npix;
+          const int k = (nt * (L::MT / 2) + mt) * (4 / NR) + g;
+          bv[k] = own ? b : 0.0f;
+          bx[k] = own ? px : -1;
+          by[k] = py;
+        }
+      }
+    }
+  }
+
+  if constexpr (NR == 0) {
+    __syncthreads();
+    float* bin = smem + (L::wave_floats(npix, true) - ((npix * npix + 3) & ~3));
+    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, s_binmap, inten_scale);
+    __syncthreads();
+  } else {
+    // ---- binned spot: owned values -> LDS (over the field planes), then the
+    // shared per-pixel tail with one pixel per lane
+    float part = 0.0f;
+#pragma unroll
+    for (int k = 0; k < RB::VPL; ++k) part += bv[k];
+    const float I = wave_sum(part);
+    __syncthreads();
+    float* bin = smem;
+#pragma unroll
+    for (int k = 0; k < RB::VPL; ++k)
+      if (bx[k] >= 0) bin[bx[k] + by[k] * npix] = bv[k];
+    __syncthreads();
+    finish_spot<64, NOISE>(P, s, true, lane, bin, I, inten_scale);
+    __syncthreads();
+  }
+}
+
+// Noise-free variants are held to 96 VGPRs (5 waves per SIMD, so a frame of
+// ~5000 sub-apertures is resident at once); the noise samplers keep their
+// registers.
+template <int N, int Q, int PREC, bool NOISE, int NR>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((NOISE || N == 64) ? 4 : 5)))
+void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
+  using L = DftLayout<N, Q>;
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  int* s_binmap = (int*)(smem + L::wave_floats(P.npix, NR == 0));
+  if (NR == 0 && !P.bin_block) {
+    const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
+    for (int i = lane; i < nbm; i += 64) s_binmap[i] = P.binmap[i];
+  }
+  const AwOperand<N, Q, PREC> aw(wtab, lane);
+  // persistent waves: workgroup g handles sub-apertures g, g + G, ...;
+  // workgroups of one XCD hold contiguous g
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
+  for (int it = 0; it < iters; ++it) {
+    const int s = g + it * (int)gridDim.x;
+    if (s >= P.nvalid) break;  // uniform over the (single-wave) workgroup
+    dft_subap<N, Q, PREC, NOISE, NR>(P, s, lane, smem, s_binmap, aw);
+  }
+}
+
+// Largest grid of the DFT spot kernel (workgroups).
+constexpr int kDftMaxGrid = 65536;
+
+template <int N, int Q, int PREC, int NR>
+static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
+  using L = DftLayout<N, Q>;
+  const int nbm = (NR == 0 && !p.bin_block) ? p.nrebin * p.nrebin * p.npix * p.npix : 0;
+  const size_t lds = sizeof(float) * ((size_t)L::wave_floats(p.npix, NR == 0) + nbm);
+  if (lds > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "DFT spot kernel: LDS footprint exceeds the per-workgroup limit");
+  // grid: one single-wave workgroup per sub-aperture (the dispatcher balances
+  // them over the CUs as they drain), up to kDftMaxGrid; beyond that each
+  // workgroup loops over sub-apertures g, g + grid, ...
+  const bool noisy = p.noise > -1.0f;
+  auto kern = noisy ? sh_image_dft_kernel<N, Q, PREC, true, NR>
+                    : sh_image_dft_kernel<N, Q, PREC, false, NR>;
+  int grid = p.nvalid < kDftMaxGrid ? p.nvalid : kDftMaxGrid;
+  if (grid < 1) grid = 1;
+  const int iters = (p.nvalid + grid - 1) / grid;
+  const Frag<PREC>* wtab = wfrag_table<N, Q, PREC>(st);
+  if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, p, iters, wtab);
+  SHWFS_LAUNCHED("sh_image_dft_kernel");
+  return SHWFS_OK;
+}
+
+template <int N, int Q, int PREC>
+static int32_t launch_dft_nr(const ImageParams& p, hipStream_t st) {
+  // register binning when the binmap is the NGS block form with a block
+  // origin aligned on the rebin factor; otherwise the LDS gather
+  const bool blk = p.bin_block && (p.bin_origin % (p.nrebin > 0 ? p.nrebin : 1)) == 0;
+  if (blk && p.nrebin == 4) return launch_dft_t<N, Q, PREC, 4>(p, st);
+  if (blk && p.nrebin == 2) return launch_dft_t<N, Q, PREC, 2>(p, st);
+  if (blk && p.nrebin == 1) return launch_dft_t<N, Q, PREC, 1>(p, st);
+  return launch_dft_t<N, Q, PREC, 0>(p, st);
+}
+
+// Entry point per operand precision and padded phase side 16*Q; each
+// (PREC, Q) pair is instantiated in a translation unit of its own.
+template <int PREC, int Q>
+int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
+  if constexpr (Q == 1) {
+    switch (nfft) {
+      case 16: return launch_dft_nr<16, 1, PREC>(p, st);
+      case 32: return launch_dft_nr<32, 1, PREC>(p, st);
+      case 64: return launch_dft_nr<64, 1, PREC>(p, st);
+      default: break;
+    }
+  } else {
+    switch (nfft) {
+      case 32: return launch_dft_nr<32, 2, PREC>(p, st);
+      case 64: return launch_dft_nr<64, 2, PREC>(p, st);
+      default: break;
+    }
+  }
+  return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
+}
+
+#define SHWFS_DFT_EXTERN(PREC, Q) \
+  extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t);
+SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 1)
+SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 2)
+SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 1)
+SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 2)
+#undef SHWFS_DFT_EXTERN
+
+}  // namespace shwfs

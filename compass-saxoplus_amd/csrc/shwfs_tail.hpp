@@ -23,18 +23,40 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return x * q + (x < r ? x : r) + b / kXcd;
 }
 
+// exp(i theta) for theta = phase*scale - offset.  FAST: the hardware sine and
+// cosine on the angle in turns (theta / 2 pi reduced to [-1/2, 1/2]), about
+// 1e-6 absolute error for the phase ranges of a sub-aperture; otherwise the
+// float-exact reduction of phase_sincos.
+template <bool FAST>
+__device__ __forceinline__ void field_sincos(float ph, float scale, float turn_scale, float off,
+                                             float& sn, float& cs) {
+  if constexpr (FAST) {
+    constexpr float kInv2Pi = 0.15915494309189533577f;
+    float r = fmaf(ph, turn_scale, -off * kInv2Pi);
+    r -= rintf(r);
+    sn = __builtin_amdgcn_sinf(r);
+    cs = __builtin_amdgcn_cosf(r);
+  } else {
+    phase_sincos(ph, scale, off, sn, cs);
+  }
+}
+
 // Field E(x, y) = mask * exp(i (phase*scale - offset)) of one sub-aperture.
 // PLANAR: Er at E[y*pe + x], Ei at E[plane + y*pe + x] (floats); otherwise
 // interleaved float2 at E2[y*pe + x].  All global loads of a chunk are issued
-// before the first use.
-template <int LANES, int EPL, bool PLANAR>
+// unconditionally (addresses clamped into the window) before the first use,
+// so their latencies overlap.
+template <int LANES, int EPL, bool PLANAR, bool FAST = false>
 __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool valid, int lane,
                                             float* E, int pe, int plane) {
   const int pdiam = P.pdiam;
   const int np2 = pdiam * pdiam;
   constexpr int CH = EPL < 16 ? EPL : 16;  // elements in flight per lane (register bound)
   if (valid) {
-    const int i0 = P.istart[s], j0 = P.jstart[s];
+    const int npup = P.npup;
+    const float* phase = P.phase + P.istart[s] + (size_t)P.jstart[s] * npup;
+    const float* pupil = P.pupil + P.istart[s] + (size_t)P.jstart[s] * npup;
+    const float turn_scale = P.scale * 0.15915494309189533577f;
     int x = lane % pdiam, y = lane / pdiam;
     const int dx = LANES % pdiam, dy = LANES / pdiam;
     for (int c0 = 0; c0 < EPL; c0 += CH) {
@@ -43,14 +65,15 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
 #pragma unroll
       for (int e = 0; e < CH; ++e) {
         const int idx = lane + (c0 + e) * LANES;
+        const bool in = idx < np2;
         xs[e] = x;
         ys[e] = y;
-        if (idx < np2) {
-          const size_t g = (size_t)(i0 + x) + (size_t)(j0 + y) * P.npup;
-          ph[e] = P.phase[g];
-          pu[e] = P.pupil[g];
-          of[e] = P.offsets[idx];
-        }
+        const int yc = in ? y : pdiam - 1;
+        const int xc = in ? x : pdiam - 1;
+        const int g = xc + yc * npup;
+        ph[e] = phase[g];
+        pu[e] = pupil[g];
+        of[e] = P.offsets[in ? idx : np2 - 1];
         x += dx;
         y += dy;
         if (x >= pdiam) {
@@ -62,7 +85,7 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
       for (int e = 0; e < CH; ++e) {
         if (lane + (c0 + e) * LANES < np2) {
           float sn, cs;
-          phase_sincos(ph[e], P.scale, of[e], sn, cs);
+          field_sincos<FAST>(ph[e], P.scale, turn_scale, of[e], sn, cs);
           if constexpr (PLANAR) {
             E[ys[e] * pe + xs[e]] = cs * pu[e];
             E[plane + ys[e] * pe + xs[e]] = sn * pu[e];
@@ -91,6 +114,60 @@ __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, boo
     P.slopes[ix] = ((float)((double)Sx / ((double)I + 1.e-6)) - P.coffset) * P.cscale - rx;
     P.slopes[iy] = ((float)((double)Sy / ((double)I + 1.e-6)) - P.coffset) * P.cscale - ry;
     P.cintens[s] = (float)I;
+  }
+}
+
+// Binned spot -> flux normalisation, noise, bincube / binimg, fused
+// centroid.  bin: the npix^2 binned pixels of sub-aperture s in LDS (x
+// fastest), in units of 1/inten_scale; I: their sum (the same units).
+template <int LANES, bool NOISE>
+__device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool valid, int lane,
+                                            const float* bin, float I, float inten_scale) {
+  const int npix = P.npix;
+  const int npix2 = npix * npix;
+  float cI = 0.0f, cX = 0.0f, cY = 0.0f;
+  double dI = 0.0, dX = 0.0, dY = 0.0;
+  if (valid) {
+    const float fl = P.flux[s];
+    const int vx = P.validsubsx[s];
+    const int vy = P.validsubsy[s];
+    if (lane == 0) P.intensities[s] = I * inten_scale;
+    // photometric normalisation b * flux / I * nphot as one factor
+    const float fnorm = (I != 0.0f) ? fl / I * P.nphot : 1.0f;
+    for (int p = lane; p < npix2; p += LANES) {
+      float b = bin[p] * fnorm;
+      const unsigned e = (unsigned)(p + s * npix2);
+      if constexpr (NOISE) {
+        if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, P.call, 0u, P.seed);
+        if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, P.call, 1u, P.seed);
+      }
+      P.bincube[e] = b;
+      const int py = p / npix, px = p - py * npix;
+      float out = b;
+      if (P.ckind != SHWFS_CENTROID_NONE) {
+        float d = b;
+        if (P.ckind != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
+        if (P.ckind == SHWFS_CENTROID_TCOG) out = d;
+        const float fx = (float)px, fy = (float)py;
+        const float w = (P.ckind == SHWFS_CENTROID_WCOG) ? P.weights[e] : 1.0f;
+        if (P.fp64) {
+          dI += (double)d * w;
+          dX += (double)d * fx * w;
+          dY += (double)d * fy * w;
+        } else {
+          cI += d * w;
+          cX += d * fx * w;
+          cY += d * fy * w;
+        }
+      }
+      if (P.binimg) P.binimg[(vx + px) + (size_t)(vy + py) * P.nsub] = out;
+    }
+  }
+  if (P.ckind != SHWFS_CENTROID_NONE) {
+    if (P.fp64)
+      centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY);
+    else
+      centroid_finish<LANES, float>(P, s, valid, lane, cI, cX, cY);
   }
 }
 
@@ -131,61 +208,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
     }
   }
   const float I = lane_group_sum<LANES>(part);
-  const float fl = valid ? P.flux[s] : 0.0f;
-  const int vx = valid ? P.validsubsx[s] : 0;
-  const int vy = valid ? P.validsubsy[s] : 0;
-  float cI = 0.0f, cX = 0.0f, cY = 0.0f;
-  double dI = 0.0, dX = 0.0, dY = 0.0;
-  if (valid) {
-    if (lane == 0) P.intensities[s] = I * inten_scale;
-    for (int p = lane; p < npix2; p += LANES) {
-      float b = bin[p];
-      if (I != 0.0f) b = b * fl / I * P.nphot;
-      const unsigned e = (unsigned)(p + s * npix2);
-      if constexpr (NOISE) {
-        if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, P.call, 0u, P.seed);
-        if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, P.call, 1u, P.seed);
-      }
-      P.bincube[e] = b;
-      const int px = p % P.npix, py = p / P.npix;
-      float out = b;
-      if (P.ckind != SHWFS_CENTROID_NONE) {
-        float d = b;
-        if (P.ckind != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
-        if (P.ckind == SHWFS_CENTROID_TCOG) out = d;
-        const float fx = (float)px, fy = (float)py;
-        if (P.ckind == SHWFS_CENTROID_WCOG) {
-          const float w = P.weights[e];
-          if (P.fp64) {
-            dI += (double)d * w;
-            dX += (double)d * fx * w;
-            dY += (double)d * fy * w;
-          } else {
-            cI += d * w;
-            cX += d * fx * w;
-            cY += d * fy * w;
-          }
-        } else {
-          if (P.fp64) {
-            dI += (double)d;
-            dX += (double)d * fx;
-            dY += (double)d * fy;
-          } else {
-            cI += d;
-            cX += d * fx;
-            cY += d * fy;
-          }
-        }
-      }
-      if (P.binimg) P.binimg[(vx + px) + (size_t)(vy + py) * P.nsub] = out;
-    }
-  }
-  if (P.ckind != SHWFS_CENTROID_NONE) {
-    if (P.fp64)
-      centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY);
-    else
-      centroid_finish<LANES, float>(P, s, valid, lane, cI, cX, cY);
-  }
+  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale);
 }
 
 }  // namespace shwfs
