@@ -190,22 +190,23 @@ struct RegBin {
 };
 
 // Spot, detector pixels and centroid of sub-aperture s (< nvalid, uniform
-// over the wave).  smem: the wave's LDS slot (general-binmap tail only).
-template <int N, int Q, int PREC, bool NOISE, int NR>
+// over the wave).  smem: the wave's LDS slot.  FULL: pdiam == P16 (the
+// window fills the operand tiles; no masking).
+template <int N, int Q, int PREC, bool NOISE, int NR, bool FULL>
 __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
                                           float* smem, const int* s_binmap,
                                           const AwOperand<N, Q, PREC>& aw) {
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
-  const int pdiam = P.pdiam;
+  const int pdiam = FULL ? L::P16 : P.pdiam;
 
   // ---- field: coalesced window loads (x fastest over lanes) -> Er, Ei planes
   // in LDS -> stage A operand, A_E[y][k] with y = 16 ma + (lane & 15) and
   // k = 32 ks + 8 (lane >> 4) + e; k < P16 holds Er at x = k, k >= P16 holds
   // Ei at x = k - P16 (0 outside the pdiam window)
-  load_fields<64, L::EPL, true, true>(P, s, true, lane, smem, L::PE, L::EPLANE);
+  load_fields<64, L::EPL, true, true, FULL ? L::P16 : 0>(P, s, true, lane, smem, L::PE,
+                                                          L::EPLANE);
   __syncthreads();
-  const bool full = (pdiam == L::P16);
   float ev[Q][Q][8];
 #pragma unroll
   for (int ma = 0; ma < Q; ++ma)
@@ -221,7 +222,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
       for (int e = 0; e < 8; ++e)
-        ev[ma][ks][e] = (full || (x0 + e < pdiam && y < pdiam)) ? v[e] : 0.0f;
+        ev[ma][ks][e] = (FULL || (x0 + e < pdiam && y < pdiam)) ? v[e] : 0.0f;
     }
   float escale = 1.0f;
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
@@ -348,7 +349,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
 // Noise-free variants are held to 96 VGPRs (5 waves per SIMD, so a frame of
 // ~5000 sub-apertures is resident at once); the noise samplers keep their
 // registers.
-template <int N, int Q, int PREC, bool NOISE, int NR>
+template <int N, int Q, int PREC, bool NOISE, int NR, bool FULL>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((NOISE || N == 64) ? 4 : 5)))
 void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
   using L = DftLayout<N, Q>;
@@ -366,7 +367,7 @@ void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
   for (int it = 0; it < iters; ++it) {
     const int s = g + it * (int)gridDim.x;
     if (s >= P.nvalid) break;  // uniform over the (single-wave) workgroup
-    dft_subap<N, Q, PREC, NOISE, NR>(P, s, lane, smem, s_binmap, aw);
+    dft_subap<N, Q, PREC, NOISE, NR, FULL>(P, s, lane, smem, s_binmap, aw);
   }
 }
 
@@ -384,8 +385,11 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
   // them over the CUs as they drain), up to kDftMaxGrid; beyond that each
   // workgroup loops over sub-apertures g, g + grid, ...
   const bool noisy = p.noise > -1.0f;
-  auto kern = noisy ? sh_image_dft_kernel<N, Q, PREC, true, NR>
-                    : sh_image_dft_kernel<N, Q, PREC, false, NR>;
+  const bool full = p.pdiam == L::P16;
+  auto kern = noisy ? (full ? sh_image_dft_kernel<N, Q, PREC, true, NR, true>
+                            : sh_image_dft_kernel<N, Q, PREC, true, NR, false>)
+                    : (full ? sh_image_dft_kernel<N, Q, PREC, false, NR, true>
+                            : sh_image_dft_kernel<N, Q, PREC, false, NR, false>);
   int grid = p.nvalid < kDftMaxGrid ? p.nvalid : kDftMaxGrid;
   if (grid < 1) grid = 1;
   const int iters = (p.nvalid + grid - 1) / grid;

@@ -46,10 +46,10 @@ __device__ __forceinline__ void field_sincos(float ph, float scale, float turn_s
 // interleaved float2 at E2[y*pe + x].  All global loads of a chunk are issued
 // unconditionally (addresses clamped into the window) before the first use,
 // so their latencies overlap.
-template <int LANES, int EPL, bool PLANAR, bool FAST = false>
+template <int LANES, int EPL, bool PLANAR, bool FAST = false, int PD = 0>
 __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool valid, int lane,
                                             float* E, int pe, int plane) {
-  const int pdiam = P.pdiam;
+  const int pdiam = PD > 0 ? PD : P.pdiam;  // PD: pdiam known at compile time
   const int np2 = pdiam * pdiam;
   constexpr int CH = EPL < 16 ? EPL : 16;  // elements in flight per lane (register bound)
   if (valid) {
