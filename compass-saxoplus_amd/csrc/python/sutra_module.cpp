@@ -449,6 +449,52 @@ PYBIND11_MODULE(sutra, m) {
         py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
         py::arg("noise") = true, py::arg("fused") = true,
         "issues nframes frames back to back; returns hipEvent-timed milliseconds");
+  py::class_<SutraWfsBatch>(m, "WfsBatch")
+      .def(py::init([](CarmaContext& ctx, std::vector<SutraWfsSH*> wfs,
+                       std::vector<Centro*> centro, bool noise) {
+             return new SutraWfsBatch(&ctx, wfs, centro, noise);
+           }),
+           py::arg("context"), py::arg("wfs"), py::arg("centroiders") = std::vector<Centro*>(),
+           py::arg("noise") = false, py::keep_alive<1, 3>(), py::keep_alive<1, 4>(),
+           "several SH WFS (and their fused centroiders, None allowed) computed by one "
+           "launch per frame")
+      .def_property_readonly("d_slopes",
+                             [](SutraWfsBatch& b) {
+                               py::list l;
+                               for (auto* s : b.d_slopes)
+                                 l.append(py::cast(s, py::return_value_policy::reference_internal,
+                                                   py::cast(&b)));
+                               return l;
+                             })
+      .def_property_readonly("methods", &SutraWfsBatch::methods)
+      .def_property_readonly("splits", &SutraWfsBatch::splits)
+      .def_property_readonly("grid", &SutraWfsBatch::grid)
+      .def_property_readonly("remap", &SutraWfsBatch::remap)
+      .def_property_readonly("frame", &SutraWfsBatch::frame)
+      .def("set_tiles", &SutraWfsBatch::set_tiles, py::arg("methods"), py::arg("splits"),
+           py::arg("grid") = 0, py::arg("remap") = 0)
+      .def("set_frame", &SutraWfsBatch::set_frame, py::arg("frame"))
+      .def("autotune",
+           [](SutraWfsBatch& b, uint32_t mask, int32_t reps) {
+             py::gil_scoped_release nogil;
+             return b.autotune(mask, reps);
+           },
+           py::arg("method_mask") = 3u, py::arg("reps") = 20,
+           "times the candidate tiles, keeps the fastest; returns us per frame")
+      .def("launch", &SutraWfsBatch::launch)
+      .def("capture", &SutraWfsBatch::capture, py::arg("frames"))
+      .def("replay",
+           [](SutraWfsBatch& b, int32_t n) {
+             py::gil_scoped_release nogil;
+             return b.replay(n);
+           },
+           py::arg("ngraphs"), "launches the captured graph ngraphs times; returns ms")
+      .def("run",
+           [](SutraWfsBatch& b, int32_t n) {
+             py::gil_scoped_release nogil;
+             return b.run(n);
+           },
+           py::arg("nframes"), "nframes single-frame launches; returns ms");
   m.attr("SHWFS_METHOD_FFT_FP32") = (int)SHWFS_METHOD_FFT_FP32;
   m.attr("SHWFS_METHOD_DFT_BF16X3") = (int)SHWFS_METHOD_DFT_BF16X3;
   m.attr("SHWFS_METHOD_DFT_FP16") = (int)SHWFS_METHOD_DFT_FP16;

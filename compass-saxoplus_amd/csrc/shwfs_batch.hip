@@ -1,0 +1,444 @@
+// Batched frames over several SH WFS: task table, one launch per frame,
+// hipGraph capture of consecutive frames, and the tile autotuner.
+//
+// Replaces the sequential per-WFS model of the reference (SutraSensors with
+// shared scratch and one cuFFT plan per WFS shape, libsutra/src.cpp/
+// sutra_sensors.cpp:42-91; the supervisor's loop over WFS,
+// shesha/supervisor/compassSupervisor.py:256-265).
+#include <algorithm>
+#include <vector>
+
+#include "shwfs_batch.hpp"
+
+using namespace shwfs;
+
+struct shwfs_batch {
+  int device = 0;
+  std::vector<shwfs_sh_geom> geoms;
+  std::vector<shwfs_frame_args> frames;  // centroid pointer not kept
+  std::vector<ImageParams> params;
+  std::vector<shwfs_batch_tile> tiles;
+  int grid = 0;   // 0: one workgroup per tile
+  int remap = 0;  // XCD-contiguous tile ranges
+  bool noisy = false;
+  // device task table
+  BatchTask* d_tasks = nullptr;
+  int ntasks = 0, total_tiles = 0;
+  size_t lds = 0;
+  uint32_t* d_frame = nullptr;
+  hipStream_t own_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_frames = 0;
+};
+
+namespace {
+
+__global__ void frame_advance_kernel(uint32_t* frame, uint32_t n) { frame[0] += n; }
+__global__ void frame_set_kernel(uint32_t* frame, uint32_t v) { frame[0] = v; }
+
+bool fft_kind(int nfft, int split, int* kind) {
+  const int base = nfft == 16 ? BK_FFT16_1 : nfft == 32 ? BK_FFT32_1 : nfft == 64 ? BK_FFT64_1 : -1;
+  if (base < 0) return false;
+  *kind = base + (split == 2 ? 1 : 0);
+  return true;
+}
+
+bool dft_kind(int nfft, int pdiam, int method, int* kind) {
+  const bool bf = method == SHWFS_METHOD_DFT_BF16X3;
+  if (!bf && method != SHWFS_METHOD_DFT_FP16) return false;
+  const int off = bf ? BK_BF16_16 : BK_FP16_16;
+  if (pdiam <= 16) {
+    if (nfft == 16) return *kind = off + 0, true;
+    if (nfft == 32) return *kind = off + 1, true;
+    if (nfft == 64) return *kind = off + 2, true;
+  } else if (pdiam <= 32) {
+    if (nfft == 32) return *kind = off + 3, true;
+    if (nfft == 64) return *kind = off + 4, true;
+  }
+  return false;
+}
+
+// Per-tile cost estimate used to put the heaviest tiles first.
+double tile_cost(const BatchTask& t, int nfft) {
+  return (double)t.per_tile * nfft * nfft;
+}
+
+int32_t fill_task(shwfs_batch* b, int i, const shwfs_batch_tile& tile, hipStream_t st,
+                  BatchTask* out, size_t* lds) {
+  const shwfs_sh_geom& g = b->geoms[i];
+  BatchTask t{};
+  t.P = b->params[i];
+  const int nfft = g.nfft, pdiam = g.nphase, npix = g.npix;
+  int32_t rc = check_geom(&g, tile.method);
+  if (rc) return rc;
+  if (tile.method == SHWFS_METHOD_FFT_FP32) {
+    int split = tile.split;
+    if (split == 0) split = 2 * pdiam <= nfft ? 2 : 1;
+    if (split != 1 && split != 2) return fail(SHWFS_ERR_ARG, "batch: FFT split must be 0, 1 or 2");
+    if (split == 2 && 2 * pdiam > nfft)
+      return fail(SHWFS_ERR_ARG, "batch: FFT split 2 needs 2*nphase <= nfft");
+    if (!fft_kind(nfft, split, &t.kind))
+      return fail(SHWFS_ERR_UNSUPPORTED, "batch: FFT stage needs nfft in {16, 32, 64}");
+    t.per_tile = 64 / nfft;
+    int slot = 0;
+    switch (t.kind) {
+      case BK_FFT16_1: slot = FftLayout<16, 1>::slot_floats(pdiam, npix); break;
+      case BK_FFT16_2: slot = FftLayout<16, 2>::slot_floats(pdiam, npix); break;
+      case BK_FFT32_1: slot = FftLayout<32, 1>::slot_floats(pdiam, npix); break;
+      case BK_FFT32_2: slot = FftLayout<32, 2>::slot_floats(pdiam, npix); break;
+      case BK_FFT64_1: slot = FftLayout<64, 1>::slot_floats(pdiam, npix); break;
+      default: slot = FftLayout<64, 2>::slot_floats(pdiam, npix); break;
+    }
+    *lds = sizeof(float) * (size_t)t.per_tile * slot;
+  } else {
+    if (!dft_kind(nfft, pdiam, tile.method, &t.kind))
+      return fail(SHWFS_ERR_UNSUPPORTED, "batch: DFT stage needs nfft in {16, 32, 64}, pdiam <= 32");
+    t.per_tile = 1;
+    t.nr = dft_register_rebin(t.P);
+    int wf = 0;
+    switch (t.kind - (tile.method == SHWFS_METHOD_DFT_BF16X3 ? BK_BF16_16 : BK_FP16_16)) {
+      case 0: wf = DftLayout<16, 1>::wave_floats(npix, t.nr == 0); break;
+      case 1: wf = DftLayout<32, 1>::wave_floats(npix, t.nr == 0); break;
+      case 2: wf = DftLayout<64, 1>::wave_floats(npix, t.nr == 0); break;
+      case 3: wf = DftLayout<32, 2>::wave_floats(npix, t.nr == 0); break;
+      default: wf = DftLayout<64, 2>::wave_floats(npix, t.nr == 0); break;
+    }
+    *lds = sizeof(float) * (size_t)wf;
+    t.wtab = dft_operand_table(nfft, pdiam, tile.method, st);
+    if (!t.wtab) return SHWFS_ERR_HIP;
+  }
+  t.ntiles = (g.nvalid + t.per_tile - 1) / t.per_tile;
+  *out = t;
+  return SHWFS_OK;
+}
+
+int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
+  const int n = (int)b->geoms.size();
+  // the table may still be read by frames in flight on any stream
+  SHWFS_HIP(hipDeviceSynchronize());
+  std::vector<BatchTask> tasks(n);
+  std::vector<double> cost(n);
+  size_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    size_t l = 0;
+    int32_t rc = fill_task(b, i, b->tiles[i], st, &tasks[i], &l);
+    if (rc) return rc;
+    lds = std::max(lds, l);
+    cost[i] = tile_cost(tasks[i], b->geoms[i].nfft);
+  }
+  if (lds > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "batch: LDS footprint exceeds the per-workgroup limit");
+  // heaviest tiles first (they start first and the light ones fill in)
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  std::vector<BatchTask> sorted(n);
+  int first = 0;
+  for (int k = 0; k < n; ++k) {
+    sorted[k] = tasks[order[k]];
+    sorted[k].first_tile = first;
+    first += sorted[k].ntiles;
+  }
+  if (b->d_tasks == nullptr) SHWFS_HIP(hipMalloc((void**)&b->d_tasks, sizeof(BatchTask) * n));
+  SHWFS_HIP(hipMemcpy(b->d_tasks, sorted.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice));
+  b->ntasks = n;
+  b->total_tiles = first;
+  b->lds = lds;
+  return SHWFS_OK;
+}
+
+void drop_graph(shwfs_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_frames = 0;
+}
+
+hipStream_t pick_stream(shwfs_batch* b, void* stream) {
+  return stream ? (hipStream_t)stream : b->own_stream;
+}
+
+// One frame (frame counter + frame_off) without advancing the counter.
+int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
+  for (size_t i = 0; i < b->frames.size(); ++i) {
+    const shwfs_frame_args& f = b->frames[i];
+    if (f.d_binimg && f.noise_background > 0.0f) {
+      const long n = (long)b->params[i].nsub * b->params[i].nsub;
+      int32_t rc = launch_background(f.d_binimg, n, f.noise_background, f.seed, f.call + frame_off,
+                                     st, b->d_frame);
+      if (rc) return rc;
+    }
+  }
+  const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
+  if (b->noisy)
+    return launch_batch_noisy(b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds, b->d_frame,
+                              frame_off, b->remap, st);
+  return launch_batch_quiet(b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds, b->d_frame,
+                            frame_off, b->remap, st);
+}
+
+int32_t advance(shwfs_batch* b, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(frame_advance_kernel, dim3(1), dim3(1), 0, st, b->d_frame, n);
+  SHWFS_LAUNCHED("frame_advance_kernel");
+  return SHWFS_OK;
+}
+
+int resident_blocks(shwfs_batch* b) {
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  const int per = b->noisy ? batch_blocks_per_cu_noisy(b->lds) : batch_blocks_per_cu_quiet(b->lds);
+  return cus * (per > 0 ? per : 1);
+}
+
+// Time per frame (us) of the current configuration: median of 3 runs of
+// `reps` back-to-back frames.
+int32_t time_config(shwfs_batch* b, int reps, hipStream_t st, float* us) {
+  int32_t rc = build_tasks(b, st);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  SHWFS_HIP(hipEventCreate(&e0));
+  SHWFS_HIP(hipEventCreate(&e1));
+  float runs[3];
+  for (int w = 0; w < 2 && rc == SHWFS_OK; ++w) rc = launch_frame(b, 0, st);
+  for (int r = 0; r < 3 && rc == SHWFS_OK; ++r) {
+    (void)hipEventRecord(e0, st);
+    for (int k = 0; k < reps && rc == SHWFS_OK; ++k) rc = launch_frame(b, 0, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(SHWFS_ERR_HIP, "batch: timing failed");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    runs[r] = 1e3f * ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  std::sort(runs, runs + 3);
+  *us = runs[1];
+  return SHWFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* frames,
+                           int32_t nwfs, shwfs_batch** out) {
+  if (!out) return fail(SHWFS_ERR_ARG, "batch_create: out is NULL");
+  *out = nullptr;
+  if (!geoms || !frames || nwfs <= 0) return fail(SHWFS_ERR_ARG, "batch_create: no WFS given");
+  auto* b = new (std::nothrow) shwfs_batch();
+  if (!b) return fail(SHWFS_ERR_NOMEM, "batch_create: out of host memory");
+  int32_t rc = SHWFS_OK;
+  if (hipGetDevice(&b->device) != hipSuccess) rc = fail(SHWFS_ERR_HIP, "batch_create: no device");
+  for (int i = 0; i < nwfs && rc == SHWFS_OK; ++i) {
+    ImageParams p;
+    rc = image_params(&geoms[i], &frames[i], &p);
+    if (rc) break;
+    b->geoms.push_back(geoms[i]);
+    shwfs_frame_args f = frames[i];
+    f.centroid = nullptr;
+    b->frames.push_back(f);
+    b->params.push_back(p);
+    b->tiles.push_back(shwfs_batch_tile{frames[i].method, 0});
+    if (frames[i].noise > -1.0f) b->noisy = true;
+  }
+  if (rc == SHWFS_OK && hipMalloc((void**)&b->d_frame, sizeof(uint32_t)) != hipSuccess)
+    rc = fail(SHWFS_ERR_HIP, "batch_create: hipMalloc failed");
+  if (rc == SHWFS_OK && hipStreamCreate(&b->own_stream) != hipSuccess)
+    rc = fail(SHWFS_ERR_HIP, "batch_create: hipStreamCreate failed");
+  if (rc == SHWFS_OK) {
+    hipLaunchKernelGGL(frame_set_kernel, dim3(1), dim3(1), 0, b->own_stream, b->d_frame, 0u);
+    if (hipGetLastError() != hipSuccess) rc = fail(SHWFS_ERR_HIP, "batch_create: counter init failed");
+  }
+  if (rc == SHWFS_OK) rc = build_tasks(b, b->own_stream);
+  if (rc == SHWFS_OK && hipStreamSynchronize(b->own_stream) != hipSuccess)
+    rc = fail(SHWFS_ERR_HIP, "batch_create: synchronisation failed");
+  if (rc) {
+    shwfs_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_destroy(shwfs_batch* b) {
+  if (!b) return SHWFS_OK;
+  drop_graph(b);
+  if (b->own_stream) (void)hipStreamSynchronize(b->own_stream);
+  if (b->d_tasks) (void)hipFree(b->d_tasks);
+  if (b->d_frame) (void)hipFree(b->d_frame);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+  delete b;
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_set_tiles(shwfs_batch* b, const shwfs_batch_tile* tiles, int32_t grid,
+                              int32_t remap) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_set_tiles: NULL batch");
+  if (grid < 0) return fail(SHWFS_ERR_ARG, "batch_set_tiles: grid must be >= 0");
+  std::vector<shwfs_batch_tile> old = b->tiles;
+  if (tiles) b->tiles.assign(tiles, tiles + b->geoms.size());
+  int32_t rc = build_tasks(b, b->own_stream);
+  if (rc) {
+    b->tiles = old;
+    (void)build_tasks(b, b->own_stream);
+    return rc;
+  }
+  b->grid = grid;
+  b->remap = remap ? 1 : 0;
+  drop_graph(b);
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_get_tiles(const shwfs_batch* b, shwfs_batch_tile* tiles, int32_t* grid,
+                              int32_t* remap) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_get_tiles: NULL batch");
+  if (tiles) std::copy(b->tiles.begin(), b->tiles.end(), tiles);
+  if (grid) *grid = b->grid;
+  if (remap) *remap = b->remap;
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_set_frame(shwfs_batch* b, uint32_t frame, void* stream) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_set_frame: NULL batch");
+  hipLaunchKernelGGL(frame_set_kernel, dim3(1), dim3(1), 0, pick_stream(b, stream), b->d_frame,
+                     frame);
+  SHWFS_LAUNCHED("frame_set_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_get_frame(shwfs_batch* b, uint32_t* frame) {
+  if (!b || !frame) return fail(SHWFS_ERR_ARG, "batch_get_frame: NULL argument");
+  SHWFS_HIP(hipDeviceSynchronize());
+  SHWFS_HIP(hipMemcpy(frame, b->d_frame, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_launch(shwfs_batch* b, void* stream) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_launch: NULL batch");
+  hipStream_t st = pick_stream(b, stream);
+  int32_t rc = launch_frame(b, 0, st);
+  if (rc) return rc;
+  return advance(b, 1, st);
+}
+
+int32_t shwfs_batch_capture(shwfs_batch* b, int32_t frames, void* stream) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_capture: NULL batch");
+  if (frames <= 0) return fail(SHWFS_ERR_ARG, "batch_capture: frames must be positive");
+  hipStream_t st = pick_stream(b, stream);
+  drop_graph(b);
+  SHWFS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int32_t rc = SHWFS_OK;
+  for (int k = 0; k < frames && rc == SHWFS_OK; ++k) rc = launch_frame(b, (uint32_t)k, st);
+  if (rc == SHWFS_OK) rc = advance(b, (uint32_t)frames, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g) return fail(SHWFS_ERR_HIP, "batch_capture: stream capture failed");
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    drop_graph(b);
+    return fail(SHWFS_ERR_HIP, "batch_capture: graph instantiation failed");
+  }
+  b->graph_frames = frames;
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_replay(shwfs_batch* b, int32_t ngraphs, void* stream) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_replay: NULL batch");
+  if (!b->exec) return fail(SHWFS_ERR_STATE, "batch_replay: no captured graph");
+  if (ngraphs < 0) return fail(SHWFS_ERR_ARG, "batch_replay: negative count");
+  hipStream_t st = pick_stream(b, stream);
+  for (int k = 0; k < ngraphs; ++k) SHWFS_HIP(hipGraphLaunch(b->exec, st));
+  return SHWFS_OK;
+}
+
+int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps, void* stream,
+                             float* best_us) {
+  if (!b) return fail(SHWFS_ERR_ARG, "batch_autotune: NULL batch");
+  if (reps <= 0) return fail(SHWFS_ERR_ARG, "batch_autotune: reps must be positive");
+  if ((method_mask & 7u) == 0) return fail(SHWFS_ERR_ARG, "batch_autotune: empty method mask");
+  hipStream_t st = pick_stream(b, stream);
+  drop_graph(b);
+  const int n = (int)b->geoms.size();
+  // start from a valid configuration: the first allowed method per WFS
+  for (int i = 0; i < n; ++i) {
+    if ((method_mask >> b->tiles[i].method) & 1u) continue;
+    for (int m = 0; m < 3; ++m)
+      if (((method_mask >> m) & 1u) && check_geom(&b->geoms[i], m) == SHWFS_OK) {
+        b->tiles[i] = shwfs_batch_tile{m, 0};
+        break;
+      }
+  }
+  float best = 0.f;
+  int32_t rc = time_config(b, reps, st, &best);
+  if (rc) return rc;
+  // coordinate descent over the WFS, heaviest first
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    return (double)b->geoms[a].nvalid * b->geoms[a].nfft * b->geoms[a].nfft >
+           (double)b->geoms[c].nvalid * b->geoms[c].nfft * b->geoms[c].nfft;
+  });
+  for (int i : order) {
+    const shwfs_sh_geom& g = b->geoms[i];
+    std::vector<shwfs_batch_tile> cands;
+    for (int m = 0; m < 3; ++m) {
+      if (!((method_mask >> m) & 1u) || check_geom(&g, m) != SHWFS_OK) continue;
+      if (m == SHWFS_METHOD_FFT_FP32) {
+        if (g.nfft != 16 && g.nfft != 32 && g.nfft != 64) continue;
+        cands.push_back(shwfs_batch_tile{m, 1});
+        if (2 * g.nphase <= g.nfft) cands.push_back(shwfs_batch_tile{m, 2});
+      } else {
+        int k;
+        if (dft_kind(g.nfft, g.nphase, m, &k)) cands.push_back(shwfs_batch_tile{m, 0});
+      }
+    }
+    shwfs_batch_tile keep = b->tiles[i];
+    for (const auto& c : cands) {
+      if (c.method == keep.method && c.split == keep.split) continue;
+      b->tiles[i] = c;
+      float us = 0.f;
+      if (time_config(b, reps, st, &us) == SHWFS_OK && us < best) {
+        best = us;
+        keep = c;
+      }
+    }
+    b->tiles[i] = keep;
+  }
+  rc = build_tasks(b, st);
+  if (rc) return rc;
+  // launch shape: one workgroup per tile or persistent grids; XCD mapping
+  const int res = resident_blocks(b);
+  const int grids[3] = {0, res, 2 * res};
+  int best_grid = b->grid, best_remap = b->remap;
+  for (int gi = 0; gi < 3; ++gi)
+    for (int rm = 0; rm < 2; ++rm) {
+      if (grids[gi] == b->grid && rm == b->remap) continue;
+      const int sg = b->grid, sr = b->remap;
+      b->grid = grids[gi];
+      b->remap = rm;
+      float us = 0.f;
+      if (time_config(b, reps, st, &us) == SHWFS_OK && us < best) {
+        best = us;
+        best_grid = grids[gi];
+        best_remap = rm;
+      }
+      b->grid = sg;
+      b->remap = sr;
+    }
+  b->grid = best_grid;
+  b->remap = best_remap;
+  rc = build_tasks(b, st);
+  if (rc) return rc;
+  SHWFS_HIP(hipStreamSynchronize(st));
+  if (best_us) *best_us = best;
+  return SHWFS_OK;
+}
+
+}  // extern "C"

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float*
   }
 }
 
-static int32_t check_geom(const shwfs_sh_geom* g, int method) {
+int32_t check_geom(const shwfs_sh_geom* g, int method) {
   if (!g) return fail(SHWFS_ERR_ARG, "geometry is NULL");
   if (g->nvalid <= 0 || g->nphase <= 0 || g->npup <= 0 || g->npix <= 0 || g->nrebin <= 0 ||
       g->nxsub <= 0)
@@ -189,6 +189,69 @@ static int32_t check_geom(const shwfs_sh_geom* g, int method) {
   } else if (method != SHWFS_METHOD_FFT_FP32) {
     return fail(SHWFS_ERR_ARG, "unknown method");
   }
+  return SHWFS_OK;
+}
+
+int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* out) {
+  int32_t rc = check_geom(g, f ? f->method : 0);
+  if (rc) return rc;
+  if (!f || !f->d_phase || !f->d_pupil || !f->d_bincube || !f->d_intensities)
+    return fail(SHWFS_ERR_ARG, "frame: phase, pupil, bincube and intensities are required");
+  // every sub-aperture window must lie inside the screen: checked by the caller
+  // when the tables are loaded (the tables live on the device).
+  ImageParams p{};
+  p.phase = f->d_phase;
+  p.pupil = f->d_pupil;
+  p.offsets = g->d_offsets;
+  p.istart = g->d_istart;
+  p.jstart = g->d_jstart;
+  p.validsubsx = g->d_validsubsx;
+  p.validsubsy = g->d_validsubsy;
+  p.flux = g->d_fluxPerSub;
+  p.binmap = g->d_binmap;
+  p.npup = g->npup;
+  p.nvalid = g->nvalid;
+  p.pdiam = g->nphase;
+  p.npix = g->npix;
+  p.nrebin = g->nrebin;
+  p.nsub = g->npix * g->nxsub;
+  p.bin_block = g->binmap_block ? 1 : 0;
+  p.bin_origin = g->bin_origin;
+  if (p.bin_block && (g->bin_origin < 0 || g->bin_origin >= g->nfft))
+    return fail(SHWFS_ERR_ARG, "bin_origin outside [0, nfft)");
+  if (f->wg_size != 0 && f->wg_size != 64 && f->wg_size != 128 && f->wg_size != 256)
+    return fail(SHWFS_ERR_ARG, "wg_size must be 0, 64, 128 or 256");
+  p.tpb = f->wg_size;
+  p.scale = f->scale;
+  p.nphot = f->nphot;
+  p.noise = f->noise;
+  p.seed = f->seed;
+  p.call = f->call;
+  p.bincube = f->d_bincube;
+  p.binimg = f->d_binimg;
+  p.intensities = f->d_intensities;
+  p.ckind = SHWFS_CENTROID_NONE;
+  if (f->centroid && f->centroid->kind != SHWFS_CENTROID_NONE) {
+    const shwfs_centroid_args* c = f->centroid;
+    if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_WCOG)
+      return fail(SHWFS_ERR_ARG, "unknown centroider kind");
+    if (!c->d_slopes || !c->d_intensities) return fail(SHWFS_ERR_ARG, "centroid outputs are NULL");
+    if (c->kind == SHWFS_CENTROID_WCOG && !c->d_weights)
+      return fail(SHWFS_ERR_ARG, "WCOG needs weights");
+    if (c->nvalid != g->nvalid || c->npix != g->npix)
+      return fail(SHWFS_ERR_ARG, "fused centroider sizes differ from the WFS");
+    p.ckind = c->kind;
+    p.fp64 = c->fp64_moments;
+    p.slope_order = c->slope_order;
+    p.threshold = c->threshold;
+    p.coffset = c->offset;
+    p.cscale = c->scale;
+    p.weights = c->d_weights;
+    p.ref = c->d_ref;
+    p.slopes = c->d_slopes;
+    p.cintens = c->d_intensities;
+  }
+  *out = p;
   return SHWFS_OK;
 }
 
@@ -243,65 +306,10 @@ int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t
 }
 
 int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, void* stream) {
-  int32_t rc = check_geom(g, f ? f->method : 0);
+  ImageParams p;
+  int32_t rc = image_params(g, f, &p);
   if (rc) return rc;
-  if (!f || !f->d_phase || !f->d_pupil || !f->d_bincube || !f->d_intensities)
-    return fail(SHWFS_ERR_ARG, "frame: phase, pupil, bincube and intensities are required");
-  // every sub-aperture window must lie inside the screen: checked by the caller
-  // when the tables are loaded (the tables live on the device).
   hipStream_t st = (hipStream_t)stream;
-  ImageParams p{};
-  p.phase = f->d_phase;
-  p.pupil = f->d_pupil;
-  p.offsets = g->d_offsets;
-  p.istart = g->d_istart;
-  p.jstart = g->d_jstart;
-  p.validsubsx = g->d_validsubsx;
-  p.validsubsy = g->d_validsubsy;
-  p.flux = g->d_fluxPerSub;
-  p.binmap = g->d_binmap;
-  p.npup = g->npup;
-  p.nvalid = g->nvalid;
-  p.pdiam = g->nphase;
-  p.npix = g->npix;
-  p.nrebin = g->nrebin;
-  p.nsub = g->npix * g->nxsub;
-  p.bin_block = g->binmap_block ? 1 : 0;
-  p.bin_origin = g->bin_origin;
-  if (p.bin_block && (g->bin_origin < 0 || g->bin_origin >= g->nfft))
-    return fail(SHWFS_ERR_ARG, "bin_origin outside [0, nfft)");
-  if (f->wg_size != 0 && f->wg_size != 64 && f->wg_size != 128 && f->wg_size != 256)
-    return fail(SHWFS_ERR_ARG, "wg_size must be 0, 64, 128 or 256");
-  p.tpb = f->wg_size;
-  p.scale = f->scale;
-  p.nphot = f->nphot;
-  p.noise = f->noise;
-  p.seed = f->seed;
-  p.call = f->call;
-  p.bincube = f->d_bincube;
-  p.binimg = f->d_binimg;
-  p.intensities = f->d_intensities;
-  p.ckind = SHWFS_CENTROID_NONE;
-  if (f->centroid && f->centroid->kind != SHWFS_CENTROID_NONE) {
-    const shwfs_centroid_args* c = f->centroid;
-    if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_WCOG)
-      return fail(SHWFS_ERR_ARG, "unknown centroider kind");
-    if (!c->d_slopes || !c->d_intensities) return fail(SHWFS_ERR_ARG, "centroid outputs are NULL");
-    if (c->kind == SHWFS_CENTROID_WCOG && !c->d_weights)
-      return fail(SHWFS_ERR_ARG, "WCOG needs weights");
-    if (c->nvalid != g->nvalid || c->npix != g->npix)
-      return fail(SHWFS_ERR_ARG, "fused centroider sizes differ from the WFS");
-    p.ckind = c->kind;
-    p.fp64 = c->fp64_moments;
-    p.slope_order = c->slope_order;
-    p.threshold = c->threshold;
-    p.coffset = c->offset;
-    p.cscale = c->scale;
-    p.weights = c->d_weights;
-    p.ref = c->d_ref;
-    p.slopes = c->d_slopes;
-    p.cintens = c->d_intensities;
-  }
   if (f->d_binimg && f->noise_background > 0.0f) {
     const long n = (long)p.nsub * p.nsub;
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);

@@ -3,4 +3,5 @@
 
 namespace shwfs {
 template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 1>(const ImageParams&, int, hipStream_t);
+template const void* dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, hipStream_t);
 }  // namespace shwfs

@@ -3,4 +3,5 @@
 
 namespace shwfs {
 template int32_t launch_dft_pq<SHWFS_METHOD_DFT_FP16, 2>(const ImageParams&, int, hipStream_t);
+template const void* dft_table_pq<SHWFS_METHOD_DFT_FP16, 2>(int, hipStream_t);
 }  // namespace shwfs

@@ -11,6 +11,22 @@ static int32_t launch_dft_p(const ImageParams& p, int nfft, hipStream_t st) {
   return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
 }
 
+const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t st) {
+  const void* t = nullptr;
+  const bool q1 = pdiam <= 16;
+  if (pdiam > 32) {
+    t = nullptr;
+  } else if (method == SHWFS_METHOD_DFT_FP16) {
+    t = q1 ? dft_table_pq<SHWFS_METHOD_DFT_FP16, 1>(nfft, st)
+           : dft_table_pq<SHWFS_METHOD_DFT_FP16, 2>(nfft, st);
+  } else if (method == SHWFS_METHOD_DFT_BF16X3) {
+    t = q1 ? dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(nfft, st)
+           : dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 2>(nfft, st);
+  }
+  if (!t) set_error("no DFT operand table for nfft %d, pdiam %d, method %d", nfft, pdiam, method);
+  return t;
+}
+
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t st) {
   if (method == SHWFS_METHOD_DFT_FP16) return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st);
   if (method == SHWFS_METHOD_DFT_BF16X3) return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st);

@@ -178,24 +178,16 @@ __device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x
   return f;
 }
 
-// Register binning of the |F|^2 values a lane holds after stage B (kx of the
-// lane, four consecutive ky per accumulator): the NR x NR block of a binned
-// pixel is NR consecutive v of one lane times NR consecutive lanes (NR in
-// {1, 2, 4}, block origin a multiple of NR).  Summation: within the lane
-// over ky, then across lanes over kx.
-template <int N, int NR>
-struct RegBin {
-  static constexpr int MT = N / 8, NTB = N / 16;
-  static constexpr int VPL = (MT / 2) * NTB * (4 / NR);  // binned values a lane may own
-};
-
 // Spot, detector pixels and centroid of sub-aperture s (< nvalid, uniform
-// over the wave).  smem: the wave's LDS slot.  FULL: pdiam == P16 (the
-// window fills the operand tiles; no masking).
-template <int N, int Q, int PREC, bool NOISE, int NR, bool FULL>
+// over the wave).  smem: the wave's LDS slot; bm: the binmap table (LDS or
+// global) used when nr == 0; nr: register rebin factor (dft_register_rebin),
+// fixed at compile time by NRC >= 0; FULL: pdiam == P16 (the window fills the
+// operand tiles; no masking).
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1>
 __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
-                                          float* smem, const int* s_binmap,
+                                          float* smem, const int* bm, const int nr_rt,
                                           const AwOperand<N, Q, PREC>& aw) {
+  const int nr = NRC >= 0 ? NRC : nr_rt;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
@@ -258,13 +250,14 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   const float inten_scale = 1.0f / (escale * escale);
 
   // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
-  // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles
-  using RB = RegBin<N, (NR > 0 ? NR : 1)>;
-  float bv[RB::VPL];
-  int bx[RB::VPL], by[RB::VPL];
+  // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
+  // |F|^2 goes to LDS: binned in registers (nr > 0) or as the whole image for
+  // the general binmap gather (nr == 0).  The field planes are dead by now.
   const int o = P.bin_origin, npix = P.npix;
-  float* hr = smem;  // general-binmap tail: |F|^2 over the field planes
-  if constexpr (NR == 0) __syncthreads();
+  float* hr = smem;   // nr == 0: hr[kx + ky*N]
+  float* bin = nr > 0 ? smem : smem + (L::wave_floats(npix, true) - ((npix * npix + 3) & ~3));
+  float part = 0.0f;  // this lane's share of the binned flux (nr > 0)
+  __syncthreads();
 #pragma unroll
   for (int nt = 0; nt < L::NTB; ++nt) {
     F bg[L::KS];
@@ -285,94 +278,97 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       fb[mt] = acc;
     }
     const int kx = 16 * nt + (lane & 15);
-    if constexpr (NR == 0) {
+    const int u = (kx - o) & (N - 1);
 #pragma unroll
-      for (int mt = 0; mt < L::MT / 2; ++mt)
+    for (int mt = 0; mt < L::MT / 2; ++mt) {
+      float iv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
+        iv[v] = fr * fr + fi * fi;
+      }
+      const int ky0 = 16 * mt + 4 * (lane >> 4);  // this lane's 4 consecutive ky
+      if (nr == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) hr[kx + (ky0 + v) * N] = iv[v];
+      } else if (nr == 4) {
+        // the 4 ky of the lane x lanes kx .. kx+3 form one binned pixel
+        float b = (iv[0] + iv[1]) + (iv[2] + iv[3]);
+        b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
+        b += __int_as_float(dpp_int(__float_as_int(b), 1));  // lane ^ 2
+        const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
+        if ((lane & 3) == 0 && px < npix && py < npix) {
+          bin[px + py * npix] = b;
+          part += b;
+        }
+      } else if (nr == 2) {
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          float b = iv[2 * g] + iv[2 * g + 1];
+          b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
+          const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
+          if ((lane & 1) == 0 && px < npix && py < npix) {
+            bin[px + py * npix] = b;
+            part += b;
+          }
+        }
+      } else {  // nr == 1
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int ky = 16 * mt + 4 * (lane >> 4) + v;
-          const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
-          hr[kx + ky * N] = fr * fr + fi * fi;
-        }
-    } else {
-      const int u = (kx - o) & (N - 1);
-#pragma unroll
-      for (int mt = 0; mt < L::MT / 2; ++mt) {
-        float iv[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
-          iv[v] = fr * fr + fi * fi;
-        }
-#pragma unroll
-        for (int g = 0; g < 4 / NR; ++g) {
-          float b = 0.0f;
-#pragma unroll
-          for (int v = 0; v < NR; ++v) b += iv[g * NR + v];
-          if constexpr (NR >= 2) b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
-          if constexpr (NR >= 4) b += __int_as_float(dpp_int(__float_as_int(b), 1));  // lane ^ 2
-          const int wv = (16 * mt + 4 * (lane >> 4) + g * NR - o) & (N - 1);
-          const int px = u / NR, py = wv / NR;
-          const bool own = ((lane & (NR - 1)) == 0) && px < npix && py < npix;
-          const int k = (nt * (L::MT / 2) + mt) * (4 / NR) + g;
-          bv[k] = own ? b : 0.0f;
-          bx[k] = own ? px : -1;
-          by[k] = py;
+          const int px = u, py = (ky0 + v - o) & (N - 1);
+          if (px < npix && py < npix) {
+            bin[px + py * npix] = iv[v];
+            part += iv[v];
+          }
         }
       }
     }
   }
-
-  if constexpr (NR == 0) {
-    __syncthreads();
-    float* bin = smem + (L::wave_floats(npix, true) - ((npix * npix + 3) & ~3));
-    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, s_binmap, inten_scale);
-    __syncthreads();
+  __syncthreads();
+  if (nr == 0) {
+    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale);
   } else {
-    // ---- binned spot: owned values -> LDS (over the field planes), then the
-    // shared per-pixel tail with one pixel per lane
-    float part = 0.0f;
-#pragma unroll
-    for (int k = 0; k < RB::VPL; ++k) part += bv[k];
-    const float I = wave_sum(part);
-    __syncthreads();
-    float* bin = smem;
-#pragma unroll
-    for (int k = 0; k < RB::VPL; ++k)
-      if (bx[k] >= 0) bin[bx[k] + by[k] * npix] = bv[k];
-    __syncthreads();
-    finish_spot<64, NOISE>(P, s, true, lane, bin, I, inten_scale);
-    __syncthreads();
+    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale);
   }
+  __syncthreads();
 }
 
 // Noise-free variants are held to 96 VGPRs (5 waves per SIMD, so a frame of
 // ~5000 sub-apertures is resident at once); the noise samplers keep their
 // registers.
-template <int N, int Q, int PREC, bool NOISE, int NR, bool FULL>
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((NOISE || N == 64) ? 4 : 5)))
 void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
+  constexpr int nr = NR;
   using L = DftLayout<N, Q>;
   extern __shared__ float smem[];
   const int lane = threadIdx.x;
-  int* s_binmap = (int*)(smem + L::wave_floats(P.npix, NR == 0));
-  if (NR == 0 && !P.bin_block) {
+  int* s_binmap = (int*)(smem + L::wave_floats(P.npix, nr == 0));
+  if (nr == 0 && !P.bin_block) {
     const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
     for (int i = lane; i < nbm; i += 64) s_binmap[i] = P.binmap[i];
   }
   const AwOperand<N, Q, PREC> aw(wtab, lane);
-  // persistent waves: workgroup g handles sub-apertures g, g + G, ...;
-  // workgroups of one XCD hold contiguous g
+  // workgroup g handles sub-apertures g, g + G, ...; workgroups of one XCD
+  // hold contiguous g
   const int g = xcd_remap(blockIdx.x, gridDim.x);
   for (int it = 0; it < iters; ++it) {
     const int s = g + it * (int)gridDim.x;
     if (s >= P.nvalid) break;  // uniform over the (single-wave) workgroup
-    dft_subap<N, Q, PREC, NOISE, NR, FULL>(P, s, lane, smem, s_binmap, aw);
+    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem, s_binmap, NR, aw);
   }
 }
 
 // Largest grid of the DFT spot kernel (workgroups).
 constexpr int kDftMaxGrid = 65536;
+
+// Register binning applies when the binmap is the NGS block form with a block
+// origin aligned on the rebin factor (nrebin in {1, 2, 4}); otherwise 0, the
+// general gather over |F|^2 in LDS.
+__host__ __device__ inline int dft_register_rebin(const ImageParams& p) {
+  const bool blk = p.bin_block && p.nrebin > 0 && (p.bin_origin % p.nrebin) == 0;
+  return (blk && (p.nrebin == 1 || p.nrebin == 2 || p.nrebin == 4)) ? p.nrebin : 0;
+}
 
 template <int N, int Q, int PREC, int NR>
 static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
@@ -386,10 +382,10 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
   // workgroup loops over sub-apertures g, g + grid, ...
   const bool noisy = p.noise > -1.0f;
   const bool full = p.pdiam == L::P16;
-  auto kern = noisy ? (full ? sh_image_dft_kernel<N, Q, PREC, true, NR, true>
-                            : sh_image_dft_kernel<N, Q, PREC, true, NR, false>)
-                    : (full ? sh_image_dft_kernel<N, Q, PREC, false, NR, true>
-                            : sh_image_dft_kernel<N, Q, PREC, false, NR, false>);
+  auto kern = noisy ? (full ? sh_image_dft_kernel<N, Q, PREC, true, true, NR>
+                            : sh_image_dft_kernel<N, Q, PREC, true, false, NR>)
+                    : (full ? sh_image_dft_kernel<N, Q, PREC, false, true, NR>
+                            : sh_image_dft_kernel<N, Q, PREC, false, false, NR>);
   int grid = p.nvalid < kDftMaxGrid ? p.nvalid : kDftMaxGrid;
   if (grid < 1) grid = 1;
   const int iters = (p.nvalid + grid - 1) / grid;
@@ -401,14 +397,13 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
 }
 
 template <int N, int Q, int PREC>
-static int32_t launch_dft_nr(const ImageParams& p, hipStream_t st) {
-  // register binning when the binmap is the NGS block form with a block
-  // origin aligned on the rebin factor; otherwise the LDS gather
-  const bool blk = p.bin_block && (p.bin_origin % (p.nrebin > 0 ? p.nrebin : 1)) == 0;
-  if (blk && p.nrebin == 4) return launch_dft_t<N, Q, PREC, 4>(p, st);
-  if (blk && p.nrebin == 2) return launch_dft_t<N, Q, PREC, 2>(p, st);
-  if (blk && p.nrebin == 1) return launch_dft_t<N, Q, PREC, 1>(p, st);
-  return launch_dft_t<N, Q, PREC, 0>(p, st);
+static int32_t launch_dft_n(const ImageParams& p, hipStream_t st) {
+  switch (dft_register_rebin(p)) {
+    case 4: return launch_dft_t<N, Q, PREC, 4>(p, st);
+    case 2: return launch_dft_t<N, Q, PREC, 2>(p, st);
+    case 1: return launch_dft_t<N, Q, PREC, 1>(p, st);
+    default: return launch_dft_t<N, Q, PREC, 0>(p, st);
+  }
 }
 
 // Entry point per operand precision and padded phase side 16*Q; each
@@ -417,23 +412,42 @@ template <int PREC, int Q>
 int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
   if constexpr (Q == 1) {
     switch (nfft) {
-      case 16: return launch_dft_nr<16, 1, PREC>(p, st);
-      case 32: return launch_dft_nr<32, 1, PREC>(p, st);
-      case 64: return launch_dft_nr<64, 1, PREC>(p, st);
+      case 16: return launch_dft_n<16, 1, PREC>(p, st);
+      case 32: return launch_dft_n<32, 1, PREC>(p, st);
+      case 64: return launch_dft_n<64, 1, PREC>(p, st);
       default: break;
     }
   } else {
     switch (nfft) {
-      case 32: return launch_dft_nr<32, 2, PREC>(p, st);
-      case 64: return launch_dft_nr<64, 2, PREC>(p, st);
+      case 32: return launch_dft_n<32, 2, PREC>(p, st);
+      case 64: return launch_dft_n<64, 2, PREC>(p, st);
       default: break;
     }
   }
   return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
 }
 
-#define SHWFS_DFT_EXTERN(PREC, Q) \
-  extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t);
+template <int PREC, int Q>
+const void* dft_table_pq(int nfft, hipStream_t st) {
+  if constexpr (Q == 1) {
+    switch (nfft) {
+      case 16: return wfrag_table<16, 1, PREC>(st);
+      case 32: return wfrag_table<32, 1, PREC>(st);
+      case 64: return wfrag_table<64, 1, PREC>(st);
+      default: return nullptr;
+    }
+  } else {
+    switch (nfft) {
+      case 32: return wfrag_table<32, 2, PREC>(st);
+      case 64: return wfrag_table<64, 2, PREC>(st);
+      default: return nullptr;
+    }
+  }
+}
+
+#define SHWFS_DFT_EXTERN(PREC, Q)                                                         \
+  extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t); \
+  extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 1)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 2)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 1)

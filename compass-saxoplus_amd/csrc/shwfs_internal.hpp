@@ -79,10 +79,19 @@ struct ImageParams {
   float* cintens;
 };
 
+// Validated kernel parameters of one frame of one WFS (no launch).
+int32_t check_geom(const shwfs_sh_geom* g, int method);
+int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
+
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s);
 int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, hipStream_t s);
+// img = sigma * N(0,1); the Philox call is `call` plus *call_base when
+// call_base is not NULL (device frame counter of a batch)
 int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
-                          hipStream_t s);
+                          hipStream_t s, const unsigned* call_base = nullptr);
+// Device table of the DFT operand fragments for (nfft, pdiam, method), built
+// on first use on the current device; NULL (and the error set) otherwise.
+const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t s);
 
 }  // namespace shwfs

@@ -403,6 +403,45 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
 double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
                   int32_t nframes, bool noise, bool fused);
 
+// Several SH WFS (uneven sizes) computed by one launch per frame, with their
+// fused centroiders; replaces the per-WFS loop over SutraSensors::d_wfs.
+// Slopes go to buffers owned by the batch (d_slopes[i], 2*nvalid each).
+// Spot noise of WFS i at batch frame f uses the Philox call
+// wfs[i]->noise_call + f (noise_call as it was at construction).
+class SutraWfsBatch {
+ public:
+  vector<SutraWfsSH*> wfs;
+  vector<SutraCentroider<float, float>*> centro;
+  vector<CarmaObj<float>*> d_slopes;
+  bool noise;
+  int32_t device;
+  CarmaContext* current_context;
+
+  SutraWfsBatch(CarmaContext* context, vector<SutraWfsSH*> wfs,
+                vector<SutraCentroider<float, float>*> centro, bool noise);
+  ~SutraWfsBatch();
+  SutraWfsBatch(const SutraWfsBatch&) = delete;
+  SutraWfsBatch& operator=(const SutraWfsBatch&) = delete;
+
+  // per-WFS (method, split), launch grid (0 = one workgroup per tile), XCD remap
+  void set_tiles(const vector<int32_t>& methods, const vector<int32_t>& splits, int32_t grid,
+                 int32_t remap);
+  vector<int32_t> methods() const;
+  vector<int32_t> splits() const;
+  int32_t grid() const;
+  int32_t remap() const;
+  float autotune(uint32_t method_mask, int32_t reps);  // us per frame of the kept tiles
+  void set_frame(uint32_t frame);
+  uint32_t frame();
+  void launch();                         // one frame
+  void capture(int32_t frames);          // hipGraph of `frames` frames
+  double replay(int32_t ngraphs);        // ms between hipEvents around the replays
+  double run(int32_t nframes);           // ms for nframes plain launches
+
+ private:
+  shwfs_batch* handle;
+};
+
 template <typename Tin, typename T, typename Tout>
 class SutraRtc {
  public:

@@ -80,3 +80,22 @@ class ShSystem:
                     validsubsx=w._validsubsx, validsubsy=w._validsubsy, binmap=w._binmap,
                     fluxPerSub=geom.flux_per_valid_subap(w), nphot=np.float32(w._nphotons),
                     pdiam=w._pdiam, Nfft=w._Nfft, npix=w.npix, nxsub=w.nxsub)
+
+
+class BatchSystem:
+    """Several independent SH WFS (one ShSystem each: own pupil, screen and
+    centroider) computed by one launch per frame through ``sutra.WfsBatch``."""
+
+    def __init__(self, cfgs, method=0, noise=False):
+        self.systems = [ShSystem(c, method=method) for c in cfgs]
+        self.carma, self.sutra = native()
+        ctx = self.systems[0].ctx
+        self.batch = self.sutra.WfsBatch(ctx, [s.wfs for s in self.systems],
+                                         [s.centro for s in self.systems], noise)
+
+    def set_phases(self, phases):
+        for s, ph in zip(self.systems, phases):
+            s.set_phase(ph)
+
+    def slopes(self, i):
+        return np.array(self.batch.d_slopes[i])

@@ -164,6 +164,53 @@ int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
                           const int32_t* d_phasemap, const float* d_ttprojmat, int32_t nphase,
                           int32_t nvalid, float alpha, void* stream);
 
+/* ---- batched frames over several SH WFS ---------------------------------------
+ * One launch per frame covers every sub-aperture of every WFS of the batch
+ * (uneven sizes and methods mixed), replacing the sequential per-WFS launch
+ * sequences of SutraSensors (libsutra/src.cpp/sutra_sensors.cpp:42-91, one
+ * cuFFT plan per WFS shape) driven by the supervisor's loop over WFS
+ * (shesha/supervisor/compassSupervisor.py:256-265).  The spot noise of WFS i
+ * at frame f uses the Philox call frames[i].call + f, f being a device-side
+ * frame counter (starts at 0, advanced once per frame), so captured graphs
+ * replay successive frames.  All pointers in geoms / frames (and their fused
+ * centroiders) are captured at creation and must stay valid.  A NULL stream
+ * means an internal stream of the batch that is ordered with the default
+ * stream. */
+typedef struct shwfs_batch shwfs_batch; /* opaque */
+typedef struct shwfs_batch_tile {
+  int32_t method; /* SHWFS_METHOD_* used for this WFS in the batch */
+  int32_t split;  /* FFT stage: 1, or 2 (row transforms split in two, needs
+                     2*nphase <= nfft); 0 = 2 when allowed.  Ignored by DFT. */
+} shwfs_batch_tile;
+/* frames[i].method is the initial method of WFS i (wg_size is ignored). */
+int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* frames,
+                           int32_t nwfs, shwfs_batch** out);
+int32_t shwfs_batch_destroy(shwfs_batch* batch);
+/* tiles: [nwfs] or NULL (keep); grid: workgroups of the launch, 0 = one per
+ * tile (otherwise each loops over tiles); remap: 1 gives each XCD a
+ * contiguous range of tiles.  Drops a captured graph. */
+int32_t shwfs_batch_set_tiles(shwfs_batch* batch, const shwfs_batch_tile* tiles, int32_t grid,
+                              int32_t remap);
+int32_t shwfs_batch_get_tiles(const shwfs_batch* batch, shwfs_batch_tile* tiles, int32_t* grid,
+                              int32_t* remap);
+int32_t shwfs_batch_set_frame(shwfs_batch* batch, uint32_t frame, void* stream);
+/* Synchronises the device, then reads the frame counter. */
+int32_t shwfs_batch_get_frame(shwfs_batch* batch, uint32_t* frame);
+/* One frame of every WFS (+ the frame counter advance). */
+int32_t shwfs_batch_launch(shwfs_batch* batch, void* stream);
+/* Captures `frames` consecutive frames into a hipGraph (replacing any
+ * previous one); nothing runs until shwfs_batch_replay. */
+int32_t shwfs_batch_capture(shwfs_batch* batch, int32_t frames, void* stream);
+/* Launches the captured graph `ngraphs` times (ngraphs * frames frames). */
+int32_t shwfs_batch_replay(shwfs_batch* batch, int32_t ngraphs, void* stream);
+/* Tile autotuner: per WFS, the methods allowed by method_mask (bit m =
+ * SHWFS_METHOD_m) and FFT splits; then the launch grid and XCD mapping; each
+ * candidate timed over `reps` frames (median of 3).  Keeps the fastest
+ * configuration and returns its time per frame.  Overwrites the outputs and
+ * drops a captured graph; the frame counter is left unchanged. */
+int32_t shwfs_batch_autotune(shwfs_batch* batch, uint32_t method_mask, int32_t reps, void* stream,
+                             float* best_us);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* One 16x32 by 32x16 product on the matrix cores with the operand layouts the
  * DFT kernels use: d_A (16x32) and d_B (32x16) row-major float, rounded to the

@@ -41,6 +41,10 @@ class FrameArgs(C.Structure):
                 ("centroid", C.POINTER(CentroidArgs))]
 
 
+class BatchTile(C.Structure):
+    _fields_ = [("method", i32), ("split", i32)]
+
+
 def declared_functions():
     """Names of every function declared in include/shwfs.h."""
     txt = open(HEADER).read()
@@ -58,6 +62,17 @@ def load():
     lib.shwfs_calibrate.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
     lib.shwfs_slopes_geom.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp]
     lib.shwfs_mfma_probe.argtypes = [vp, vp, vp, i32, vp]
+    lib.shwfs_batch_create.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), i32,
+                                       C.POINTER(vp)]
+    lib.shwfs_batch_destroy.argtypes = [vp]
+    lib.shwfs_batch_set_tiles.argtypes = [vp, C.POINTER(BatchTile), i32, i32]
+    lib.shwfs_batch_get_tiles.argtypes = [vp, C.POINTER(BatchTile), C.POINTER(i32), C.POINTER(i32)]
+    lib.shwfs_batch_set_frame.argtypes = [vp, u32, vp]
+    lib.shwfs_batch_get_frame.argtypes = [vp, C.POINTER(u32)]
+    lib.shwfs_batch_launch.argtypes = [vp, vp]
+    lib.shwfs_batch_capture.argtypes = [vp, i32, vp]
+    lib.shwfs_batch_replay.argtypes = [vp, i32, vp]
+    lib.shwfs_batch_autotune.argtypes = [vp, u32, i32, vp, C.POINTER(C.c_float)]
     lib.shwfs_last_error.argtypes = [C.c_char_p, C.c_size_t]
     lib.shwfs_version.restype = C.c_char_p
     return lib

@@ -65,3 +65,23 @@ def test_binmap_block_analysis_recognises_the_ngs_binmap(lib, Ntot, npix, nrebin
     bad[[0, 1]] = bad[[1, 0]]
     assert lib.shwfs_binmap_block_origin(bad.ctypes.data_as(C.c_void_p), Ntot, npix, nrebin,
                                          C.byref(o)) == 0
+
+
+def test_batch_arguments_rejected_without_launch(lib):
+    out = C.c_void_p(123)
+    assert lib.shwfs_batch_create(None, None, 2, C.byref(out)) == capi.SHWFS_ERR_ARG
+    assert out.value is None  # cleared on failure
+    g, f = capi.ShGeom(), capi.FrameArgs()
+    assert lib.shwfs_batch_create(C.byref(g), C.byref(f), 0, C.byref(out)) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_batch_create(C.byref(g), C.byref(f), 1, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_batch_destroy(None) == capi.SHWFS_OK
+    for rc in (lib.shwfs_batch_set_tiles(None, None, 0, 0),
+               lib.shwfs_batch_get_tiles(None, None, None, None),
+               lib.shwfs_batch_set_frame(None, 0, None),
+               lib.shwfs_batch_get_frame(None, None),
+               lib.shwfs_batch_launch(None, None),
+               lib.shwfs_batch_capture(None, 4, None),
+               lib.shwfs_batch_replay(None, 1, None),
+               lib.shwfs_batch_autotune(None, 1, 10, None, None)):
+        assert rc == capi.SHWFS_ERR_ARG
+    assert "batch" in capi.last_error(lib)
