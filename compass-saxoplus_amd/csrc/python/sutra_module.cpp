@@ -342,6 +342,33 @@ static void declare_centroiders(py::module& m) {
            },
            py::arg("weights"), py::arg("ndim"))
       .def("set_threshold", &Wcog::set_threshold, py::arg("thresh"));
+
+  using Mlp = SutraCentroiderMlp<float, float>;
+  py::class_<Mlp, Centro>(m, "CentroiderMLP_FF")
+      .def_property_readonly("d_params", [](Mlp& c) { return c.d_params; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_grad", [](Mlp& c) { return c.d_grad; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_fallback", [](Mlp& c) { return c.d_fallback; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("x_scale", [](Mlp& c) { return c.x_scale; })
+      .def_property_readonly("flux_range", [](Mlp& c) { return py::make_tuple(c.flux_lo, c.flux_hi); })
+      .def_property_readonly("step", [](Mlp& c) { return c.step; })
+      .def("init_params", &Mlp::init_params, py::arg("seed"))
+      .def("set_input_scale", &Mlp::set_input_scale, py::arg("scale"))
+      .def("set_flux_range", &Mlp::set_flux_range, py::arg("lo"), py::arg("hi"))
+      .def("loss_grad", &Mlp::loss_grad, py::arg("x"), py::arg("t"), py::arg("n"),
+           py::arg("t_scale") = 1.0f, "mean loss; the gradient is left in d_grad")
+      .def("forward", &Mlp::forward, py::arg("x"), py::arg("n"), py::arg("y"))
+      .def("train",
+           [](Mlp& c, CarmaObj<float>* x, CarmaObj<float>* t, int64_t n, float t_scale,
+              int32_t batch, int32_t epochs, float lr, uint64_t seed) {
+             py::gil_scoped_release nogil;
+             return c.train(x, t, n, t_scale, batch, epochs, lr, seed);
+           },
+           py::arg("x"), py::arg("t"), py::arg("n"), py::arg("t_scale") = 1.0f,
+           py::arg("batch") = 4096, py::arg("epochs") = 1, py::arg("lr") = 1e-3f,
+           py::arg("seed") = 1u, "Adam epochs over shuffled batches; last epoch's mean loss");
 }
 
 static void declare_rtc(py::module& m) {
@@ -495,6 +522,17 @@ PYBIND11_MODULE(sutra, m) {
              return b.run(n);
            },
            py::arg("nframes"), "nframes single-frame launches; returns ms");
+  m.def("make_mlp_training_set",
+        [](SutraWfsSH& w, CarmaObj<float>& base, float r0_base, float r0_min, float r0_max,
+           int32_t nframes, uint64_t seed, int32_t geom_type, CarmaObj<float>& X,
+           CarmaObj<float>& T) {
+          py::gil_scoped_release nogil;
+          return make_mlp_training_set(&w, &base, r0_base, r0_min, r0_max, nframes, seed,
+                                       geom_type, &X, &T);
+        },
+        py::arg("wfs"), py::arg("base"), py::arg("r0_base"), py::arg("r0_min"), py::arg("r0_max"),
+        py::arg("nframes"), py::arg("seed"), py::arg("geom_type"), py::arg("X"), py::arg("T"),
+        "noisy spots and geometric slopes of nframes shifted / rescaled screen windows");
   m.attr("SHWFS_METHOD_FFT_FP32") = (int)SHWFS_METHOD_FFT_FP32;
   m.attr("SHWFS_METHOD_DFT_BF16X3") = (int)SHWFS_METHOD_DFT_BF16X3;
   m.attr("SHWFS_METHOD_DFT_FP16") = (int)SHWFS_METHOD_DFT_FP16;

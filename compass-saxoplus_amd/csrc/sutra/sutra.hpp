@@ -372,6 +372,58 @@ class SutraCentroiderWcog : public SutraCentroider<Tin, T> {
                                     int32_t size) override;
 };
 
+// Learned centroider: a per-sub-aperture MLP (64 -> 128 -> 128 -> 2, GELU)
+// on 8x8-pixel spots, bf16 on the matrix cores, trained on the GPU with
+// Adam; CoG of the same pixels as fallback when the spot flux is outside the
+// training range.  No counterpart in the reference (a centroider type next to
+// the CoG family, created by SutraRtc::add_centroider with type "mlp").
+template <class Tin, class T>
+class SutraCentroiderMlp : public SutraCentroider<Tin, T> {
+ public:
+  CarmaObj<float>* d_params;  // SHWFS_MLP_NPARAM fp32 master parameters
+  CarmaObj<float>* d_grad;
+  CarmaObj<float>* d_m;       // Adam moments
+  CarmaObj<float>* d_v;
+  CarmaObj<float>* d_work;    // bf16 weight copies
+  CarmaObj<float>* d_loss;
+  CarmaObj<int32_t>* d_fallback;  // [nvalid] 1 where the CoG was used
+  float x_scale;              // input scale (1 / photons per sub-aperture)
+  float flux_lo, flux_hi;     // in-distribution spot flux
+  int32_t step;               // Adam steps taken
+
+  SutraCentroiderMlp(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                     float scale, bool filter_TT, int32_t device);
+  ~SutraCentroiderMlp();
+  string get_type() override { return "mlp"; }
+  int32_t init_params(uint64_t seed);
+  int32_t set_input_scale(float s);
+  int32_t set_flux_range(float lo, float hi);
+  // mean loss of the targets t * t_scale; gradient left in d_grad
+  float loss_grad(CarmaObj<float>* x, CarmaObj<float>* t, int64_t n, float t_scale);
+  int32_t forward(CarmaObj<float>* x, int64_t n, CarmaObj<float>* y);
+  // epochs over shuffled batches (host shuffle, seeded); returns the mean
+  // loss of the last epoch
+  float train(CarmaObj<float>* x, CarmaObj<float>* t, int64_t n, float t_scale, int32_t batch,
+              int32_t epochs, float lr, uint64_t seed);
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  // not fusable into the spot kernel: NONE (comp_frame runs spots then get_cog)
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+};
+
+// Training pairs generated by the WFS itself: for each of nframes frames, a
+// window of `base` (side base->get_dims(1), phase in um at r0_base) at a
+// random shift, rescaled to r0 ~ U(r0_min, r0_max) ((r0/r0_base)^(-5/6)),
+// becomes the guide-star phase; the noisy spots (the WFS noise) go to rows
+// of X [nframes*nvalid][64] and slopes_geom(geom_type) (arcsec) to T
+// [nframes*nvalid][2].  Returns the number of pairs.
+int64_t make_mlp_training_set(SutraWfsSH* wfs, CarmaObj<float>* base, float r0_base, float r0_min,
+                              float r0_max, int32_t nframes, uint64_t seed, int32_t geom_type,
+                              CarmaObj<float>* X, CarmaObj<float>* T);
+
 // Slope sink of the RTC: the reference's controllers are outside this build;
 // the generic controller keeps what do_centroids needs (its slope vector and
 // the indices of the centroiders that feed it).

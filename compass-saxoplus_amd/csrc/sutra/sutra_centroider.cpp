@@ -503,6 +503,10 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
       throw std::runtime_error("comp_frame: WCoG weights not initialised for this WFS");
   }
   wfs->current_context->set_active_device(wfs->device, 1);
+  if (c->get_type() == "mlp") {  // not fusable: spots, then the network
+    wfs->comp_image(noise);
+    return c->get_cog(intensities, slopes, noise);
+  }
   shwfs_sh_geom g = wfs->geom();
   shwfs_frame_args f = wfs->frame_args(noise);
   shwfs_centroid_args a =
@@ -600,6 +604,9 @@ int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nv
   else if (typec == "wcog")
     d_centro.push_back(new SutraCentroiderWcog<Tin, T>(context, wfs, nvalid, offset, scale,
                                                        filter_TT, (int32_t)device));
+  else if (typec == "mlp")  // learned centroider (this build's addition)
+    d_centro.push_back(new SutraCentroiderMlp<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                      filter_TT, (int32_t)device));
   else {
     std::cerr << "centroider unknown" << std::endl;
     return EXIT_FAILURE;

@@ -211,6 +211,52 @@ int32_t shwfs_batch_replay(shwfs_batch* batch, int32_t ngraphs, void* stream);
 int32_t shwfs_batch_autotune(shwfs_batch* batch, uint32_t method_mask, int32_t reps, void* stream,
                              float* best_us);
 
+/* ---- learned centroider (MLP 64 -> 128 -> 128 -> 2, GELU) ---------------------
+ * No counterpart in the reference: a centroider type next to CoG / WCoG /
+ * TCoG (libsutra/src.cu/sutra_centroider_{cog,wcog,tcog}.cu), trained on spot
+ * / true-gradient pairs produced by the WFS itself.  bf16 operands on the
+ * matrix cores, FP32 accumulation, FP32 master parameters.
+ * d_params: flat [SHWFS_MLP_NPARAM] = W1 (128x64 row-major), b1 (128),
+ * W2 (128x128), b2 (128), W3 (2x128), b3 (2).  d_work: shwfs_mlp_work_bytes()
+ * of device scratch for the bf16 weight copies.  Inputs x: [n][64] spots
+ * (pixel px + 8*py), multiplied by x_scale; outputs / targets [n][2] in
+ * pixels.  Loss: mean over samples and outputs of (y - t)^2. */
+#define SHWFS_MLP_NPARAM 25090
+size_t shwfs_mlp_work_bytes(void);
+/* LeCun-normal weights from a Philox stream, zero biases. */
+int32_t shwfs_mlp_init(float* d_params, uint64_t seed, void* stream);
+int32_t shwfs_mlp_forward(const float* d_params, void* d_work, const float* d_x, int64_t n,
+                          float x_scale, float* d_y, void* stream);
+/* Mean loss into *d_loss and its gradient into d_grad (both overwritten);
+ * targets are d_t * t_scale.  d_index: optional [n] rows of d_x / d_t to use
+ * (a shuffled batch; each row must exist), or NULL. */
+int32_t shwfs_mlp_loss_grad(const float* d_params, void* d_work, const float* d_x,
+                            const float* d_t, const int32_t* d_index, int64_t n, float x_scale,
+                            float t_scale, float* d_grad, float* d_loss, void* stream);
+/* One Adam step (step >= 1, bias-corrected) on d_params; zeroes d_grad. */
+int32_t shwfs_mlp_adam(float* d_params, float* d_grad, float* d_m, float* d_v, int32_t step,
+                       float lr, float beta1, float beta2, float eps, void* stream);
+/* Slopes of 8x8-pixel spots: the MLP output * y_scale - ref, or the CoG of
+ * the same pixels, (Sx/(I+1e-6) - offset) * scale - ref, when the spot flux I
+ * is outside [flux_lo, flux_hi] (d_fallback[s] = 1 then; may be NULL).  Spots
+ * are the windows at (d_validx[s], d_validy[s]) of a detector image of side
+ * `size`, or, with both valid arrays NULL, a spot cube [nvalid][64].
+ * d_intensities (may be NULL) receives I.  slope_order as
+ * shwfs_centroid_args. */
+int32_t shwfs_mlp_centroids(const float* d_params, void* d_work, const float* d_img,
+                            int32_t size, const int32_t* d_validx, const int32_t* d_validy,
+                            int32_t nvalid, int32_t npix, float x_scale, float flux_lo,
+                            float flux_hi, float offset, float scale, float y_scale,
+                            const float* d_ref, int32_t slope_order, float* d_slopes,
+                            float* d_intensities, int32_t* d_fallback, void* stream);
+
+/* ---- phase window (synthetic inputs) ---------------------------------------------
+ * d_dst[x + y*n] = gain * d_src[(ox + x) + (oy + y)*src_side] for x, y < n:
+ * a translated, rescaled window of a larger screen (frozen-flow shift and
+ * r0 scaling of a von Karman screen).  No counterpart in the reference. */
+int32_t shwfs_phase_window(const float* d_src, int32_t src_side, int32_t ox, int32_t oy,
+                           float gain, float* d_dst, int32_t n, void* stream);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* One 16x32 by 32x16 product on the matrix cores with the operand layouts the
  * DFT kernels use: d_A (16x32) and d_B (32x16) row-major float, rounded to the
