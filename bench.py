@@ -65,6 +65,54 @@ def measure(method, cfg_name, steps, warmup, phase, thr, wg=0):
     return out, s
 
 
+def measure_batched(method, k, phase, thr, graphs=10, frames_per_graph=20):
+    """k independent 80x80 frames (own screens and outputs) per launch through
+    sutra.WfsBatch, replayed from a captured hipGraph."""
+    import copy
+    from shwfs import configs, system
+    cfg0 = system.geometry(configs.get("large"))
+    systems = []
+    for i in range(k):
+        s = system.ShSystem(copy.deepcopy(cfg0), method=method, fp64_moments=True, threshold=thr)
+        s.set_phase(np.roll(phase, 37 * i, axis=0))
+        systems.append(s)
+    import sutra
+    b = sutra.WfsBatch(systems[0].ctx, [s.wfs for s in systems], [s.centro for s in systems], False)
+    b.set_tiles([method] * k, [0] * k)
+    b.capture(frames_per_graph)
+    b.replay(2)
+    sync()
+    t0 = time.perf_counter()
+    ms = b.replay(graphs)
+    sync()
+    wall = time.perf_counter() - t0
+    frames = graphs * frames_per_graph * k
+    return dict(frames_per_s=frames / wall, gpu_frames_per_s=frames / (ms * 1e-3),
+                us_per_frame=1e3 * ms / frames, frames_per_launch=k, launches=graphs * frames_per_graph)
+
+
+def measure_uneven(graphs=10, frames_per_graph=100):
+    """The eight uneven SH WFS of the batching workload, one launch per frame,
+    tiles from the autotuner (FP32-accurate methods), 1000 hipGraph frames."""
+    from shwfs import configs, system
+    cfgs = configs.batch()
+    bs = system.BatchSystem(cfgs)
+    bs.set_phases([system.make_screens(c, 1)[0] for c in cfgs])
+    us = bs.batch.autotune(method_mask=3, reps=20)
+    bs.batch.capture(frames_per_graph)
+    bs.batch.replay(1)
+    sync()
+    t0 = time.perf_counter()
+    ms = bs.batch.replay(graphs)
+    sync()
+    wall = time.perf_counter() - t0
+    frames = graphs * frames_per_graph
+    return dict(frames_per_s=frames / wall, gpu_frames_per_s=frames / (ms * 1e-3),
+                us_per_frame=1e3 * ms / frames, autotune_us=us, methods=bs.batch.methods,
+                splits=bs.batch.splits, grid=bs.batch.grid, remap=bs.batch.remap,
+                nvalid=[int(s.wfs.nvalid) for s in bs.systems])
+
+
 def oracle_frame(s, phase, thr):
     from oracle import centroid as oc
     from oracle import wfs as ow
@@ -99,6 +147,9 @@ def main():
     ap.add_argument("--methods", default="0,1,2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wg", type=int, default=0, help="spot-kernel workgroup size (0 = default)")
+    ap.add_argument("--batched", type=int, default=8,
+                    help="frames of the 80x80 WFS per launch in the batched form (0 = skip)")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the eight-WFS batch")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (HBM counter calibration)")
     a = ap.parse_args()
@@ -163,6 +214,25 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                 kernel="spot+TCoG fused kernel (%s)" % METHOD_NAMES[best],
                 bytes_per_launch=nbytes, avg_launch_us=round(rb["kernel_us"], 3))
+    batched = {}
+    if a.batched > 0:
+        for m in (0, 2, 1):
+            try:
+                r = measure_batched(m, a.batched, phase, thr)
+            except RuntimeError as e:
+                print("batched %s unavailable: %s" % (METHOD_NAMES[m], e), file=sys.stderr)
+                continue
+            batched[METHOD_NAMES[m]] = {k: (round(v, 3) if isinstance(v, float) else v)
+                                        for k, v in r.items()}
+            print("[bench] batched x%d %s: %.0f frames/s (%.2f us/frame)"
+                  % (a.batched, METHOD_NAMES[m], r["frames_per_s"], r["us_per_frame"]),
+                  file=sys.stderr, flush=True)
+    uneven = None
+    if not a.no_uneven:
+        r = measure_uneven()
+        uneven = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+        print("[bench] uneven 8-WFS batch: %.0f frames/s (%.2f us/frame), methods %s"
+              % (r["frames_per_s"], r["us_per_frame"], r["methods"]), file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(rb["system"], phase, thr)
@@ -185,6 +255,15 @@ def main():
         speedup_vs_plain_fp32=(round(rb["frames_per_s"] / plain["frames_per_s"], 3)
                                if plain else None),
         roofline=roof, cpu_baseline=cpu,
+        batched_80x80=dict(
+            workload="%d independent 80x80 frames (own screen and outputs) per launch, "
+                     "hipGraph replay" % a.batched, methods=batched,
+            speedup_best_vs_plain_fp32_single=(
+                round(max(v["frames_per_s"] for v in batched.values()) / plain["frames_per_s"], 3)
+                if batched and plain else None)) if batched else None,
+        uneven_batch=dict(workload="6x 74x74 (Nfft 16) + 40x40 (Nfft 32) + 8x8 (Nfft 64) SH, "
+                                   "one launch per frame, autotuned tiles, 1000 hipGraph frames",
+                          **uneven) if uneven else None,
     )
     print(json.dumps(line), flush=True)
 
