@@ -140,6 +140,8 @@ static void declare_wfs_sh(py::module& m) {
       .def_property_readonly("wg_size", &SutraWfsSH::get_wg_size)
       .def_property_readonly("bin_block", [](SutraWfsSH& w) { return w.bin_block; })
       .def_property_readonly("bin_origin", [](SutraWfsSH& w) { return w.bin_origin; })
+      .def_property_readonly("d_pupil_unit", [](SutraWfsSH& w) { return w.d_pupil_unit; },
+                             py::return_value_policy::reference_internal)
       .def("set_wg_size", &SutraWfsSH::set_wg_size, py::arg("wg_size"),
            "threads per workgroup of the spot kernel: 0 (default), 64, 128, 256")
       .def_property_readonly("noise_call", [](SutraWfsSH& w) { return w.noise_call; })

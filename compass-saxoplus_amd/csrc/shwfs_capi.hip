@@ -114,6 +114,21 @@ __global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype,
   }
 }
 
+// one wave per sub-aperture: is every pupil value of its window exactly 1?
+__global__ __launch_bounds__(64) void unit_pupil_kernel(const float* pupil, int npup,
+                                                        const int* istart, const int* jstart,
+                                                        int pdiam, int nvalid, int* flags) {
+  const int s = blockIdx.x;
+  if (s >= nvalid) return;
+  bool all = true;
+  for (int i = threadIdx.x; i < pdiam * pdiam; i += 64) {
+    const int x = i % pdiam, y = i / pdiam;
+    all = all && pupil[(size_t)(istart[s] + x) + (size_t)(jstart[s] + y) * npup] == 1.0f;
+  }
+  const bool wave_all = __all(all);
+  if (threadIdx.x == 0) flags[s] = wave_all ? 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void fill_weights_kernel(float* out, const float* in, int npix2,
                                                            long n) {
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L)
@@ -209,6 +224,7 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
   p.validsubsy = g->d_validsubsy;
   p.flux = g->d_fluxPerSub;
   p.binmap = g->d_binmap;
+  p.pupil_unit = f->d_pupil_unit;
   p.npup = g->npup;
   p.nvalid = g->nvalid;
   p.pdiam = g->nphase;
@@ -277,6 +293,17 @@ int32_t shwfs_mfma_probe(const float* d_A, const float* d_B, float* d_D, int32_t
 }
 
 int32_t shwfs_sh_check_geom(const shwfs_sh_geom* g, int32_t method) { return check_geom(g, method); }
+
+int32_t shwfs_sh_unit_pupil(const shwfs_sh_geom* g, const float* d_pupil, int32_t* d_flags,
+                            void* stream) {
+  int32_t rc = check_geom(g, SHWFS_METHOD_FFT_FP32);
+  if (rc) return rc;
+  if (!d_pupil || !d_flags) return fail(SHWFS_ERR_ARG, "unit_pupil: NULL argument");
+  hipLaunchKernelGGL(unit_pupil_kernel, dim3(g->nvalid), dim3(64), 0, (hipStream_t)stream, d_pupil,
+                     g->npup, g->d_istart, g->d_jstart, g->nphase, g->nvalid, d_flags);
+  SHWFS_LAUNCHED("unit_pupil_kernel");
+  return SHWFS_OK;
+}
 
 int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t npix,
                                   int32_t nrebin, int32_t* origin) {

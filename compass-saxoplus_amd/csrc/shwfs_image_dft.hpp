@@ -333,11 +333,20 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   __syncthreads();
 }
 
-// Noise-free variants are held to 96 VGPRs (5 waves per SIMD, so a frame of
-// ~5000 sub-apertures is resident at once); the noise samplers keep their
-// registers.
+// Register budgets (waves per SIMD), the largest at which the code still
+// fits without spilling: 7 (72 VGPRs) for the noise-free FP16 full-window
+// 4x4-binning variant, 5 (96 VGPRs) for the other noise-free variants, so a
+// frame of ~5000 sub-apertures is resident at once; 4 for the noise samplers
+// and the N = 64 tiles.
+template <int PREC, bool NOISE, int N, bool FULL, int NR>
+constexpr int dft_waves_per_eu() {
+  if (NOISE || N == 64) return 4;
+  return (PREC == SHWFS_METHOD_DFT_FP16 && FULL && NR == 4) ? 7 : 5;
+}
+
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((NOISE || N == 64) ? 4 : 5)))
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
 void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
   constexpr int nr = NR;
   using L = DftLayout<N, Q>;

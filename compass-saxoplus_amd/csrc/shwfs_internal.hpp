@@ -60,6 +60,7 @@ struct ImageParams {
   const int* validsubsy;
   const float* flux;
   const int* binmap;
+  const int* pupil_unit;  // optional [nvalid]: 1 = pupil window all ones (not read)
   int npup, nvalid, pdiam, npix, nrebin, nsub;
   int bin_block;   // 1: binmap is the (shifted) block form, indices computed from bin_origin
   int bin_origin;

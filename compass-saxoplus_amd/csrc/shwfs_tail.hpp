@@ -56,12 +56,13 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
     const int npup = P.npup;
     const float* phase = P.phase + P.istart[s] + (size_t)P.jstart[s] * npup;
     const float* pupil = P.pupil + P.istart[s] + (size_t)P.jstart[s] * npup;
+    const bool unit = P.pupil_unit != nullptr && P.pupil_unit[s] != 0;  // window all ones
     const float turn_scale = P.scale * 0.15915494309189533577f;
     int x = lane % pdiam, y = lane / pdiam;
     const int dx = LANES % pdiam, dy = LANES / pdiam;
     for (int c0 = 0; c0 < EPL; c0 += CH) {
       float ph[CH], pu[CH], of[CH];
-      int xs[CH], ys[CH];
+      int xs[CH], ys[CH], gs[CH];
 #pragma unroll
       for (int e = 0; e < CH; ++e) {
         const int idx = lane + (c0 + e) * LANES;
@@ -70,9 +71,8 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
         ys[e] = y;
         const int yc = in ? y : pdiam - 1;
         const int xc = in ? x : pdiam - 1;
-        const int g = xc + yc * npup;
-        ph[e] = phase[g];
-        pu[e] = pupil[g];
+        gs[e] = xc + yc * npup;
+        ph[e] = phase[gs[e]];
         of[e] = P.offsets[in ? idx : np2 - 1];
         x += dx;
         y += dy;
@@ -80,6 +80,13 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
           x -= pdiam;
           y += 1;
         }
+      }
+      if (unit) {
+#pragma unroll
+        for (int e = 0; e < CH; ++e) pu[e] = 1.0f;
+      } else {
+#pragma unroll
+        for (int e = 0; e < CH; ++e) pu[e] = pupil[gs[e]];
       }
 #pragma unroll
       for (int e = 0; e < CH; ++e) {

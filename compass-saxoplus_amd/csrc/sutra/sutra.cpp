@@ -185,7 +185,9 @@ int32_t SutraWfs::set_noise(float noise_, int64_t seed) {
 
 int32_t SutraWfs::set_pupil(float* pupil) {
   current_context->set_active_device(device, 1);
-  return d_pupil->host2device(pupil);
+  int32_t rc = d_pupil->host2device(pupil);
+  pupil_changed();
+  return rc;
 }
 
 // reference: libsutra/src.cpp/sutra_wfs.cpp:247-261 (wrong size only prints)
@@ -267,6 +269,7 @@ SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
       own_plan(false),
       bin_block(0),
       bin_origin(0),
+      d_pupil_unit(nullptr),
       noise_this_call(true) {
   if (fakecam_) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
 }
@@ -335,6 +338,7 @@ int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_pla
 
 SutraWfsSH::~SutraWfsSH() {
   current_context->set_active_device(device, 1);
+  del(d_pupil_unit);
   del(d_ftkernel);
   del(d_bincube);
   del(d_binimg);
@@ -404,7 +408,21 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
   bin_block = rc;
   bin_origin = rc ? origin : 0;
   arrays_loaded = true;
+  pupil_changed();
   return EXIT_SUCCESS;
+}
+
+// fully illuminated sub-apertures skip the pupil read in the spot kernels
+void SutraWfsSH::pupil_changed() {
+  if (!arrays_loaded) return;
+  current_context->set_active_device(device, 1);
+  if (d_pupil_unit == nullptr) {
+    int64_t d[2] = {1, nvalid};
+    d_pupil_unit = new CarmaObj<int32_t>(current_context, d);
+  }
+  shwfs_sh_geom g = geom();
+  shwfs_throw_if(shwfs_sh_unit_pupil(&g, d_pupil->get_data(), d_pupil_unit->get_data(), stream),
+                 "pupil analysis");
 }
 
 shwfs_sh_geom SutraWfsSH::geom() const {
@@ -447,6 +465,7 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
   f.d_binimg = d_binimg ? d_binimg->get_data() : nullptr;
   f.d_intensities = d_intensities->get_data();
   f.centroid = nullptr;
+  f.d_pupil_unit = d_pupil_unit ? d_pupil_unit->get_data() : nullptr;
   return f;
 }
 

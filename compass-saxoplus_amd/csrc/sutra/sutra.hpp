@@ -129,6 +129,8 @@ class SutraWfs {
   int32_t wfs_initgs(float xpos, float ypos, float lambda, float mag, float zerop, int64_t size,
                      float noise, int64_t seed, float G, float thetaML, float dx, float dy);
   int32_t set_pupil(float* pupil);
+  // called after the pupil changed (derived classes refresh what depends on it)
+  virtual void pupil_changed() {}
   int32_t set_binimg(float* binimg, int32_t nElem);
   int32_t set_dark(float* dark, int32_t nElem);
   int32_t set_flat(float* flat, int32_t nElem);
@@ -169,6 +171,7 @@ class SutraWfsSH : public SutraWfs {
   bool arrays_loaded;
   bool own_plan;  // campli_plan was cloned by set_method and belongs to this WFS
   int32_t bin_block, bin_origin;  // binmap analysis done at load_arrays
+  CarmaObj<int32_t>* d_pupil_unit;  // [nvalid] 1 where the pupil window is all ones
 
   SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel, CarmaObj<hipFloatComplex>* d_camplipup,
              CarmaObj<hipFloatComplex>* d_camplifoc, CarmaObj<hipFloatComplex>* d_fttotim,
@@ -199,6 +202,7 @@ class SutraWfsSH : public SutraWfs {
   shwfs_sh_geom geom() const;
   // Frame arguments of the next comp_image (advances nothing).
   shwfs_frame_args frame_args(bool with_noise) const;
+  void pupil_changed() override;
 
  private:
   int32_t comp_generic();

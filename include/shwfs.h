@@ -106,6 +106,9 @@ typedef struct shwfs_frame_args {
   float* d_binimg;           /* [(npix*nxsub)^2] or NULL to skip the image */
   float* d_intensities;      /* [nvalid] raw per-subap flux before normalisation */
   const shwfs_centroid_args* centroid; /* NULL or fused centroider (validx/size ignored) */
+  const int32_t* d_pupil_unit; /* NULL, or [nvalid] 1 where the pupil window of the
+                                  sub-aperture is all ones (shwfs_sh_unit_pupil of
+                                  this d_pupil): the kernels skip reading it */
 } shwfs_frame_args;
 
 /* ---- spot image (SutraWfsSH::comp_generic + fill_binimage, fused) ---------
@@ -118,6 +121,11 @@ typedef struct shwfs_frame_args {
  * plus an optional fused get_centroids. */
 int32_t shwfs_sh_comp_image(const shwfs_sh_geom* geom, const shwfs_frame_args* frame, void* stream);
 
+/* d_flags[s] = 1 if every pupil value of the window of sub-aperture s is
+ * exactly 1, else 0 (for shwfs_frame_args.d_pupil_unit).  No counterpart in
+ * the reference: fully illuminated sub-apertures then skip the pupil read. */
+int32_t shwfs_sh_unit_pupil(const shwfs_sh_geom* geom, const float* d_pupil, int32_t* d_flags,
+                            void* stream);
 /* Validates a geometry against what the kernels implement (no launch). */
 int32_t shwfs_sh_check_geom(const shwfs_sh_geom* geom, int32_t method);
 

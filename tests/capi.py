@@ -38,7 +38,7 @@ class FrameArgs(C.Structure):
     _fields_ = [("d_phase", vp), ("d_pupil", vp), ("scale", f32), ("nphot", f32), ("noise", f32),
                 ("noise_background", f32), ("seed", u64), ("call", u32), ("method", i32),
                 ("wg_size", i32), ("d_bincube", vp), ("d_binimg", vp), ("d_intensities", vp),
-                ("centroid", C.POINTER(CentroidArgs))]
+                ("centroid", C.POINTER(CentroidArgs)), ("d_pupil_unit", vp)]
 
 
 class BatchTile(C.Structure):
@@ -55,6 +55,7 @@ def load():
     lib = C.CDLL(LIB_PATH)
     lib.shwfs_sh_comp_image.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), vp]
     lib.shwfs_sh_check_geom.argtypes = [C.POINTER(ShGeom), i32]
+    lib.shwfs_sh_unit_pupil.argtypes = [C.POINTER(ShGeom), vp, vp, vp]
     lib.shwfs_binmap_block_origin.argtypes = [vp, i32, i32, i32, C.POINTER(i32)]
     lib.shwfs_centroids.argtypes = [vp, vp, C.POINTER(CentroidArgs), vp]
     lib.shwfs_fill_weights.argtypes = [vp, vp, i32, i32, vp]

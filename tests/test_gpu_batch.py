@@ -55,7 +55,9 @@ def test_batch_mixed_methods_and_persistent_grid():
             s.wfs.set_method(methods[i])
             ref = _single(s)[0]
             err = relative_array_error(got[i], ref)
-            assert err <= (0.0 if methods[i] == FFT else 1e-6), (grid, remap, i, err)
+            # WFS 0 runs the FFT with one row transform per row (split 1); the
+            # per-WFS path splits it in two (pdiam <= nfft / 2): float rounding only
+            assert err <= 1e-6, (grid, remap, i, err)
 
 
 def test_graph_replay_advances_the_noise_frame():

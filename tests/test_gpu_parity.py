@@ -78,3 +78,16 @@ def test_large_tcog_fp64_matches_oracle():
     assert err < TOL
     # the reference's side effect: thresholded pixels written back into the image
     assert relative_array_error(s.binimg(), img_out) < TOL
+
+
+def test_unit_pupil_flags_match_the_pupil():
+    c = configs.get("large")
+    s = system.ShSystem(c)
+    t = s.tables()
+    pup, pd = t["pupil"], t["pdiam"]
+    expect = np.array([np.all(pup[i:i + pd, j:j + pd] == 1.0)
+                       for i, j in zip(t["istart"], t["jstart"])], dtype=np.int32)
+    got = np.array(s.wfs.d_pupil_unit).ravel()
+    print("unit-pupil sub-apertures: %d of %d" % (got.sum(), got.size), flush=True)
+    np.testing.assert_array_equal(got, expect)
+    assert 0 < got.sum() < got.size
