@@ -395,6 +395,8 @@ static void declare_rtc(py::module& m) {
                 l.append(py::cast(t, py::return_value_policy::reference));
               else if (auto* w = dynamic_cast<SutraCentroiderWcog<float, float>*>(c))
                 l.append(py::cast(w, py::return_value_policy::reference));
+              else if (auto* m = dynamic_cast<SutraCentroiderMlp<float, float>*>(c))
+                l.append(py::cast(m, py::return_value_policy::reference));
               else
                 l.append(py::cast(dynamic_cast<SutraCentroiderCog<float, float>*>(c),
                                   py::return_value_policy::reference));

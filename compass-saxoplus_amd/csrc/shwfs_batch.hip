@@ -24,6 +24,7 @@ struct shwfs_batch {
   // device task table
   BatchTask* d_tasks = nullptr;
   int ntasks = 0, total_tiles = 0;
+  int kind_set = 0;  // BatchKindSet covering the tasks
   size_t lds = 0;
   uint32_t* d_frame = nullptr;
   hipStream_t own_stream = nullptr;
@@ -145,6 +146,9 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   b->ntasks = n;
   b->total_tiles = first;
   b->lds = lds;
+  int set = 0;
+  for (const auto& t : sorted) set |= batch_kind_set(t.kind);
+  b->kind_set = (set == BKS_FFT || set == BKS_BF16 || set == BKS_FP16) ? set : BKS_ALL;
   return SHWFS_OK;
 }
 
@@ -173,10 +177,10 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
   }
   const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
   if (b->noisy)
-    return launch_batch_noisy(b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds, b->d_frame,
-                              frame_off, b->remap, st);
-  return launch_batch_quiet(b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds, b->d_frame,
-                            frame_off, b->remap, st);
+    return launch_batch_noisy(b->kind_set, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+                              b->d_frame, frame_off, b->remap, st);
+  return launch_batch_quiet(b->kind_set, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+                            b->d_frame, frame_off, b->remap, st);
 }
 
 int32_t advance(shwfs_batch* b, uint32_t n, hipStream_t st) {
@@ -188,7 +192,8 @@ int32_t advance(shwfs_batch* b, uint32_t n, hipStream_t st) {
 int resident_blocks(shwfs_batch* b) {
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-  const int per = b->noisy ? batch_blocks_per_cu_noisy(b->lds) : batch_blocks_per_cu_quiet(b->lds);
+  const int per = b->noisy ? batch_blocks_per_cu_noisy(b->kind_set, b->lds)
+                           : batch_blocks_per_cu_quiet(b->kind_set, b->lds);
   return cus * (per > 0 ? per : 1);
 }
 

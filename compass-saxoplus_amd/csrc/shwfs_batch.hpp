@@ -35,14 +35,26 @@ struct BatchTask {
 };
 
 template <int N, int Q, int PREC, bool NOISE>
-__device__ __forceinline__ void batch_dft(const BatchTask& T, const ImageParams& P, int s,
-                                          float* smem) {
+__device__ __forceinline__ void batch_dft(const BatchTask& T, int s, float* smem, unsigned call) {
   const AwOperand<N, Q, PREC> aw(static_cast<const Frag<PREC>*>(T.wtab), (int)threadIdx.x);
-  dft_subap<N, Q, PREC, NOISE, false>(P, s, (int)threadIdx.x, smem, P.binmap, T.nr, aw);
+  if (T.P.pdiam == DftLayout<N, Q>::P16)
+    dft_subap<N, Q, PREC, NOISE, true>(T.P, s, (int)threadIdx.x, smem, T.P.binmap, T.nr, aw, call);
+  else
+    dft_subap<N, Q, PREC, NOISE, false>(T.P, s, (int)threadIdx.x, smem, T.P.binmap, T.nr, aw, call);
 }
 
-template <bool NOISE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+// Kind sets compiled into one batch kernel: the host launches the smallest
+// instantiation that covers the kinds of its tasks (a kernel holding every
+// kind pays for the union of their registers and code size).
+enum BatchKindSet : int { BKS_FFT = 1, BKS_BF16 = 2, BKS_FP16 = 4, BKS_ALL = 7 };
+
+__host__ __device__ constexpr int batch_kind_set(int kind) {
+  return kind < BK_BF16_16 ? BKS_FFT : (kind < BK_FP16_16 ? BKS_BF16 : BKS_FP16);
+}
+
+template <bool NOISE, int SET>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(SET == BKS_FP16 && !NOISE ? 5 : 4)))
 void sh_batch_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total_tiles,
                      const uint32_t* __restrict__ frame_base, uint32_t frame_off, int remap) {
   extern __shared__ float smem[];
@@ -52,42 +64,76 @@ void sh_batch_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total_
   for (int t = b0; t < total_tiles; t += G) {
     int i = 0;
     while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
+    // the task's parameters are read in place (uniform addresses: scalar
+    // loads); a private copy would live in scratch memory
     const BatchTask& T = tasks[i];
-    ImageParams P = T.P;
-    P.call += fcall;
+    const ImageParams& P = T.P;
     const int s0 = (t - T.first_tile) * T.per_tile;
     switch (T.kind) {
-      case BK_FFT16_1: fft_tile<16, 1, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_FFT16_2: fft_tile<16, 2, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_FFT32_1: fft_tile<32, 1, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_FFT32_2: fft_tile<32, 2, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_FFT64_1: fft_tile<64, 1, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_FFT64_2: fft_tile<64, 2, 64, NOISE>(P, s0, smem, P.binmap); break;
-      case BK_BF16_16: batch_dft<16, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, P, s0, smem); break;
-      case BK_BF16_32: batch_dft<32, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, P, s0, smem); break;
-      case BK_BF16_64: batch_dft<64, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, P, s0, smem); break;
-      case BK_BF16_32Q2: batch_dft<32, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, P, s0, smem); break;
-      case BK_BF16_64Q2: batch_dft<64, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, P, s0, smem); break;
-      case BK_FP16_16: batch_dft<16, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, P, s0, smem); break;
-      case BK_FP16_32: batch_dft<32, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, P, s0, smem); break;
-      case BK_FP16_64: batch_dft<64, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, P, s0, smem); break;
-      case BK_FP16_32Q2: batch_dft<32, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, P, s0, smem); break;
-      case BK_FP16_64Q2: batch_dft<64, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, P, s0, smem); break;
+      case BK_FFT16_1:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<16, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_FFT16_2:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<16, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_FFT32_1:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<32, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_FFT32_2:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<32, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_FFT64_1:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<64, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_FFT64_2:
+        if constexpr ((SET & BKS_FFT) != 0) fft_tile<64, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        break;
+      case BK_BF16_16:
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_BF16_32:
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_BF16_64:
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_BF16_32Q2:
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_BF16_64Q2:
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_FP16_16:
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_FP16_32:
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_FP16_64:
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_FP16_32Q2:
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        break;
+      case BK_FP16_64Q2:
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        break;
       default: break;
     }
   }
 }
 
-// Launchers of the two instantiations (each in a translation unit of its own).
-int32_t launch_batch_quiet(const BatchTask* tasks, int ntasks, int total_tiles, int grid,
+// Launchers of the instantiations (noise-free / noisy in translation units of
+// their own); `set` is a BatchKindSet.
+int32_t launch_batch_quiet(int set, const BatchTask* tasks, int ntasks, int total_tiles, int grid,
                            size_t lds, const uint32_t* frame_base, uint32_t frame_off, int remap,
                            hipStream_t st);
-int32_t launch_batch_noisy(const BatchTask* tasks, int ntasks, int total_tiles, int grid,
+int32_t launch_batch_noisy(int set, const BatchTask* tasks, int ntasks, int total_tiles, int grid,
                            size_t lds, const uint32_t* frame_base, uint32_t frame_off, int remap,
                            hipStream_t st);
 
 // Resident single-wave workgroups per CU for a given dynamic LDS size.
-int batch_blocks_per_cu_quiet(size_t lds);
-int batch_blocks_per_cu_noisy(size_t lds);
+int batch_blocks_per_cu_quiet(int set, size_t lds);
+int batch_blocks_per_cu_noisy(int set, size_t lds);
 
 }  // namespace shwfs

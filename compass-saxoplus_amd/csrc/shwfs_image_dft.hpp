@@ -186,7 +186,8 @@ __device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1>
 __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
                                           float* smem, const int* bm, const int nr_rt,
-                                          const AwOperand<N, Q, PREC>& aw) {
+                                          const AwOperand<N, Q, PREC>& aw,
+                                          unsigned call_add = 0u) {
   const int nr = NRC >= 0 ? NRC : nr_rt;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
@@ -326,9 +327,9 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   }
   __syncthreads();
   if (nr == 0) {
-    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale);
+    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add);
   } else {
-    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale);
+    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add);
   }
   __syncthreads();
 }

@@ -43,9 +43,10 @@ struct FftLayout {
 // lanes (group `slot` of N lanes owns s0 + slot).  smem: the workgroup's LDS
 // (S slots of slot_floats, then the binmap if bm is not the global table).
 // Contains workgroup barriers: every lane of the workgroup calls it.
+// call_add: added to the Philox call counter of the noise.
 template <int N, int R, int TPB, bool NOISE>
 __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* smem,
-                                         const int* bm) {
+                                         const int* bm, unsigned call_add = 0u) {
   using L = FftLayout<N, R>;
   constexpr int M = L::M;
   constexpr int PS = L::PS;
@@ -117,7 +118,7 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
   __syncthreads();
 
   // ---- 3. tail
-  spot_tail<N, N, NOISE>(P, s, valid, lane, hr, bin, bm, 1.0f);
+  spot_tail<N, N, NOISE>(P, s, valid, lane, hr, bin, bm, 1.0f, call_add);
   __syncthreads();
 }
 

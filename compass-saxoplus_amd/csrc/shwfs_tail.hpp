@@ -129,7 +129,8 @@ __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, boo
 // fastest), in units of 1/inten_scale; I: their sum (the same units).
 template <int LANES, bool NOISE>
 __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool valid, int lane,
-                                            const float* bin, float I, float inten_scale) {
+                                            const float* bin, float I, float inten_scale,
+                                            unsigned call_add = 0u) {
   const int npix = P.npix;
   const int npix2 = npix * npix;
   float cI = 0.0f, cX = 0.0f, cY = 0.0f;
@@ -145,8 +146,9 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
       float b = bin[p] * fnorm;
       const unsigned e = (unsigned)(p + s * npix2);
       if constexpr (NOISE) {
-        if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, P.call, 0u, P.seed);
-        if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, P.call, 1u, P.seed);
+        const unsigned call = P.call + call_add;
+        if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
+        if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
       }
       P.bincube[e] = b;
       const int py = p / npix, px = p - py * npix;
@@ -184,7 +186,7 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
 template <int LANES, int NF, bool NOISE>
 __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
                                           const float* hr, float* bin, const int* s_binmap,
-                                          float inten_scale) {
+                                          float inten_scale, unsigned call_add = 0u) {
   const int npix2 = P.npix * P.npix;
   const int nreb2 = P.nrebin * P.nrebin;
   float part = 0.0f;
@@ -215,7 +217,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
     }
   }
   const float I = lane_group_sum<LANES>(part);
-  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale);
+  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add);
 }
 
 }  // namespace shwfs

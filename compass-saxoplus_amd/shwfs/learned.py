@@ -71,8 +71,14 @@ class LearnedCentroider:
         off = 3.5
         return np.stack([(sx / (I + 1e-6) - off), (sy / (I + 1e-6) - off)], 1) * self.pixsize
 
-    def errors(self, X, T, n):
+    def errors(self, X, T, n, min_flux=None):
+        """Slope RMS errors (arcsec) of the network and of CoG against the
+        geometric slopes; with min_flux, only over spots whose flux is at least
+        that (CoG's error blows up as the flux nears zero)."""
         t = np.array(T).reshape(2, n, order="F").T
-        e_mlp = float(np.sqrt(np.mean((self.predict(X, n) - t) ** 2)))
-        e_cog = float(np.sqrt(np.mean((self.cog(X, n) - t) ** 2)))
+        keep = np.ones(n, bool)
+        if min_flux is not None:
+            keep = np.array(X).reshape(64, n, order="F").sum(0) >= min_flux
+        e_mlp = float(np.sqrt(np.mean((self.predict(X, n) - t)[keep] ** 2)))
+        e_cog = float(np.sqrt(np.mean((self.cog(X, n) - t)[keep] ** 2)))
         return e_mlp, e_cog
