@@ -34,6 +34,20 @@ def algorithmic_bytes(w, nvalid):
             + (npix * nx) ** 2 * 4 + nvalid * 32)
 
 
+def profiled_traffic(method_name):
+    """HBM bytes per launch of the spot kernel of this method on the 80x80
+    frame, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", "spot_kernels_80x80_traffic.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 4 and f[0] == method_name:
+                return int(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def sync():
     import carma
     carma.deviceSync()
@@ -211,7 +225,10 @@ def main():
     nbytes = algorithmic_bytes(w, w._nvalid)
     achieved = nbytes / (rb["kernel_us"] * 1e-6) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=profiled_traffic(METHOD_NAMES[best]),
+                traffic_source="profiles/spot_kernels_80x80_traffic.txt (rocprofv3 FETCH_SIZE x 2.0 "
+                               "+ WRITE_SIZE x 1.0 per launch)",
                 kernel="spot+TCoG fused kernel (%s)" % METHOD_NAMES[best],
                 bytes_per_launch=nbytes, avg_launch_us=round(rb["kernel_us"], 3))
     batched = {}
