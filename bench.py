@@ -93,6 +93,7 @@ def measure_batched(method, k, phase, thr, graphs=10, frames_per_graph=20):
     import sutra
     b = sutra.WfsBatch(systems[0].ctx, [s.wfs for s in systems], [s.centro for s in systems], False)
     b.set_tiles([method] * k, [0] * k)
+    b.autotune(method_mask=1 << method, reps=10)  # launch grid, XCD mapping (and FFT split)
     b.capture(frames_per_graph)
     b.replay(2)
     sync()
@@ -102,7 +103,8 @@ def measure_batched(method, k, phase, thr, graphs=10, frames_per_graph=20):
     wall = time.perf_counter() - t0
     frames = graphs * frames_per_graph * k
     return dict(frames_per_s=frames / wall, gpu_frames_per_s=frames / (ms * 1e-3),
-                us_per_frame=1e3 * ms / frames, frames_per_launch=k, launches=graphs * frames_per_graph)
+                us_per_frame=1e3 * ms / frames, frames_per_launch=k, launches=graphs * frames_per_graph,
+                grid=b.grid, remap=b.remap, splits=b.splits[:1])
 
 
 def measure_uneven(graphs=10, frames_per_graph=100):

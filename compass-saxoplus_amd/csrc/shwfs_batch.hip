@@ -25,6 +25,10 @@ struct shwfs_batch {
   BatchTask* d_tasks = nullptr;
   int ntasks = 0, total_tiles = 0;
   int kind_set = 0;  // BatchKindSet covering the tasks
+  // every task the same MFMA variant: launched through the specialised
+  // same-kind kernel (method >= 0) instead of the per-kind dispatch kernel
+  int uni_method = -1, uni_nfft = 0, uni_pdiam = 0, uni_nr = 0;
+  bool uni_full = false;
   size_t lds = 0;
   uint32_t* d_frame = nullptr;
   hipStream_t own_stream = nullptr;
@@ -149,6 +153,21 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   int set = 0;
   for (const auto& t : sorted) set |= batch_kind_set(t.kind);
   b->kind_set = (set == BKS_FFT || set == BKS_BF16 || set == BKS_FP16) ? set : BKS_ALL;
+  // same-kind MFMA batch: one kind, rebin factor and window class for all
+  b->uni_method = -1;
+  const int m0 = b->tiles[0].method;
+  bool uni = m0 != SHWFS_METHOD_FFT_FP32;
+  for (int i = 0; i < n && uni; ++i) {
+    uni = tasks[i].kind == tasks[0].kind && tasks[i].nr == tasks[0].nr &&
+          b->geoms[i].nphase == b->geoms[0].nphase;
+  }
+  if (uni) {
+    b->uni_method = m0;
+    b->uni_nfft = b->geoms[0].nfft;
+    b->uni_pdiam = b->geoms[0].nphase;
+    b->uni_nr = tasks[0].nr;
+    b->uni_full = b->uni_pdiam == 16 || b->uni_pdiam == 32;
+  }
   return SHWFS_OK;
 }
 
@@ -176,6 +195,10 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
     }
   }
   const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
+  if (b->uni_method >= 0)
+    return launch_dft_tasks(b->uni_method, b->uni_nfft, b->uni_pdiam, b->uni_full, b->uni_nr,
+                            b->noisy, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+                            b->d_frame, frame_off, b->remap, st);
   if (b->noisy)
     return launch_batch_noisy(b->kind_set, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
                               b->d_frame, frame_off, b->remap, st);

@@ -23,24 +23,40 @@ enum BatchKind : int32_t {
   BK_COUNT
 };
 
-struct BatchTask {
-  ImageParams P;
-  int32_t kind;        // BatchKind
-  int32_t first_tile;  // global index of the task's first tile
-  int32_t ntiles;
-  int32_t per_tile;    // sub-apertures per tile
-  int32_t nr;          // DFT register rebin factor (0: gather)
-  int32_t pad_;
-  const void* wtab;    // DFT operand table
-};
+
+// One device function per stage kind, not inlined into the batch kernel:
+// each is register-allocated on its own (inlined into one switch, the kinds
+// share one allocation and spill).  Arguments arrive in VGPRs; readfirstlane
+// restores their wave uniformity so the task is read with scalar loads.
+__device__ __forceinline__ const BatchTask* uniform_task(const BatchTask* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<const BatchTask*>(((uint64_t)hi << 32) | lo);
+}
+
+template <int N, int R, bool NOISE>
+__device__ __noinline__ void batch_fft(const BatchTask* task, int s0, unsigned call) {
+  extern __shared__ float smem[];
+  const BatchTask* T = uniform_task(task);
+  s0 = __builtin_amdgcn_readfirstlane(s0);
+  call = __builtin_amdgcn_readfirstlane(call);
+  fft_tile<N, R, 64, NOISE>(T->P, s0, smem, T->P.binmap, call);
+}
 
 template <int N, int Q, int PREC, bool NOISE>
-__device__ __forceinline__ void batch_dft(const BatchTask& T, int s, float* smem, unsigned call) {
-  const AwOperand<N, Q, PREC> aw(static_cast<const Frag<PREC>*>(T.wtab), (int)threadIdx.x);
-  if (T.P.pdiam == DftLayout<N, Q>::P16)
-    dft_subap<N, Q, PREC, NOISE, true>(T.P, s, (int)threadIdx.x, smem, T.P.binmap, T.nr, aw, call);
+__device__ __noinline__ void batch_dft(const BatchTask* task, int s, unsigned call) {
+  extern __shared__ float smem[];
+  const BatchTask* T = uniform_task(task);
+  s = __builtin_amdgcn_readfirstlane(s);
+  call = __builtin_amdgcn_readfirstlane(call);
+  const AwOperand<N, Q, PREC> aw(static_cast<const Frag<PREC>*>(T->wtab), (int)threadIdx.x);
+  if (T->P.pdiam == DftLayout<N, Q>::P16)
+    dft_subap<N, Q, PREC, NOISE, true>(T->P, s, (int)threadIdx.x, smem, T->P.binmap, T->nr, aw,
+                                       call);
   else
-    dft_subap<N, Q, PREC, NOISE, false>(T.P, s, (int)threadIdx.x, smem, T.P.binmap, T.nr, aw, call);
+    dft_subap<N, Q, PREC, NOISE, false>(T->P, s, (int)threadIdx.x, smem, T->P.binmap, T->nr, aw,
+                                        call);
 }
 
 // Kind sets compiled into one batch kernel: the host launches the smallest
@@ -67,56 +83,55 @@ void sh_batch_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total_
     // the task's parameters are read in place (uniform addresses: scalar
     // loads); a private copy would live in scratch memory
     const BatchTask& T = tasks[i];
-    const ImageParams& P = T.P;
     const int s0 = (t - T.first_tile) * T.per_tile;
     switch (T.kind) {
       case BK_FFT16_1:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<16, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<16, 1, NOISE>(&T, s0, fcall);
         break;
       case BK_FFT16_2:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<16, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<16, 2, NOISE>(&T, s0, fcall);
         break;
       case BK_FFT32_1:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<32, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<32, 1, NOISE>(&T, s0, fcall);
         break;
       case BK_FFT32_2:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<32, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<32, 2, NOISE>(&T, s0, fcall);
         break;
       case BK_FFT64_1:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<64, 1, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<64, 1, NOISE>(&T, s0, fcall);
         break;
       case BK_FFT64_2:
-        if constexpr ((SET & BKS_FFT) != 0) fft_tile<64, 2, 64, NOISE>(P, s0, smem, P.binmap, fcall);
+        if constexpr ((SET & BKS_FFT) != 0) batch_fft<64, 2, NOISE>(&T, s0, fcall);
         break;
       case BK_BF16_16:
-        if constexpr ((SET & BKS_BF16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(&T, s0, fcall);
         break;
       case BK_BF16_32:
-        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(&T, s0, fcall);
         break;
       case BK_BF16_64:
-        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_BF16X3, NOISE>(&T, s0, fcall);
         break;
       case BK_BF16_32Q2:
-        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(&T, s0, fcall);
         break;
       case BK_BF16_64Q2:
-        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_BF16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_BF16X3, NOISE>(&T, s0, fcall);
         break;
       case BK_FP16_16:
-        if constexpr ((SET & BKS_FP16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<16, 1, SHWFS_METHOD_DFT_FP16, NOISE>(&T, s0, fcall);
         break;
       case BK_FP16_32:
-        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 1, SHWFS_METHOD_DFT_FP16, NOISE>(&T, s0, fcall);
         break;
       case BK_FP16_64:
-        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 1, SHWFS_METHOD_DFT_FP16, NOISE>(&T, s0, fcall);
         break;
       case BK_FP16_32Q2:
-        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<32, 2, SHWFS_METHOD_DFT_FP16, NOISE>(&T, s0, fcall);
         break;
       case BK_FP16_64Q2:
-        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_FP16, NOISE>(T, s0, smem, fcall);
+        if constexpr ((SET & BKS_FP16) != 0) batch_dft<64, 2, SHWFS_METHOD_DFT_FP16, NOISE>(&T, s0, fcall);
         break;
       default: break;
     }

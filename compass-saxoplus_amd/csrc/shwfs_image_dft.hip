@@ -27,6 +27,24 @@ const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t st) {
   return t;
 }
 
+int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, bool noisy,
+                         const BatchTask* tasks, int ntasks, int total, int grid, size_t lds,
+                         const uint32_t* frame_base, uint32_t frame_off, int remap,
+                         hipStream_t st) {
+  const bool q1 = pdiam <= 16;
+  if (pdiam > 32)
+    return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT batch: pdiam must be <= 32");
+#define SHWFS_ARGS nfft, full, nr, noisy, tasks, ntasks, total, grid, lds, frame_base, frame_off, remap, st
+  if (method == SHWFS_METHOD_DFT_FP16)
+    return q1 ? launch_dft_tasks_pq<SHWFS_METHOD_DFT_FP16, 1>(SHWFS_ARGS)
+              : launch_dft_tasks_pq<SHWFS_METHOD_DFT_FP16, 2>(SHWFS_ARGS);
+  if (method == SHWFS_METHOD_DFT_BF16X3)
+    return q1 ? launch_dft_tasks_pq<SHWFS_METHOD_DFT_BF16X3, 1>(SHWFS_ARGS)
+              : launch_dft_tasks_pq<SHWFS_METHOD_DFT_BF16X3, 2>(SHWFS_ARGS);
+#undef SHWFS_ARGS
+  return fail(SHWFS_ERR_ARG, "unknown DFT method");
+}
+
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t st) {
   if (method == SHWFS_METHOD_DFT_FP16) return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st);
   if (method == SHWFS_METHOD_DFT_BF16X3) return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st);

@@ -18,7 +18,8 @@
 //     product = hi*hi + hi*lo + lo*hi (lo*lo dropped), 3 MFMA per tile;
 //   * SHWFS_METHOD_DFT_FP16: fp16 operands, 1 MFMA per tile; the field of a
 //     sub-aperture is scaled by a power of two s such that the bound
-//     pdiam * max|E| >= max|G| lands in [2^14, 2^15), and the flux by 1/s^2.
+//     pdiam * max|E| >= max|G| lands in [2^14, 2^15), and the flux by 1/s^2
+//     (max|E| <= 1 for a window of unit pupil, measured otherwise).
 // The field uses the hardware sine and cosine on the angle in turns (about
 // 1e-6 absolute error): these methods trade accuracy for speed and state their
 // error against the float64 oracle.  Binning, noise, detector image and
@@ -219,14 +220,19 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
     }
   float escale = 1.0f;
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
-    float amax = 0.0f;
+    // bound of max |G|: pdiam * max |E|, with max |E| <= 1 known for a
+    // window of unit pupil, measured over the wave otherwise
+    float amax = (float)pdiam;
+    if (P.pupil_unit == nullptr || P.pupil_unit[s] == 0) {
+      float m = 0.0f;
 #pragma unroll
-    for (int ma = 0; ma < Q; ++ma)
+      for (int ma = 0; ma < Q; ++ma)
 #pragma unroll
-      for (int ks = 0; ks < Q; ++ks)
+        for (int ks = 0; ks < Q; ++ks)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) amax = fmaxf(amax, fabsf(ev[ma][ks][e]));
-    amax = wave_max(amax) * (float)pdiam;  // >= max |G|
+          for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[ma][ks][e]));
+      amax = wave_max(m) * (float)pdiam;
+    }
     if (amax > 0.0f) escale = ldexpf(1.0f, 14 - ilogbf(amax));
   }
   F ae[Q][Q];
@@ -369,6 +375,58 @@ void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
   }
 }
 
+// Same-kind batches: every task runs this (N, Q, PREC, FULL, NR) variant.
+// Workgroup b handles global sub-apertures b, b + grid, ... of the flattened
+// task range (task i owns [first_tile, first_tile + ntiles)); task
+// parameters are read in place (uniform addresses, scalar loads).
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
+void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total,
+                         const Frag<PREC>* wtab, const uint32_t* __restrict__ frame_base,
+                         uint32_t frame_off, int remap) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x;
+  const uint32_t call = frame_base[0] + frame_off;
+  const AwOperand<N, Q, PREC> aw(wtab, lane);
+  const int G = (int)gridDim.x;
+  const int b0 = remap ? xcd_remap((int)blockIdx.x, G) : (int)blockIdx.x;
+  for (int t = b0; t < total; t += G) {
+    int i = 0;
+    while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
+    const ImageParams& P = tasks[i].P;
+    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane, smem, P.binmap, NR,
+                                           aw, call);
+  }
+}
+
+template <int N, int Q, int PREC>
+static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask* tasks,
+                                  int ntasks, int total, int grid, size_t lds,
+                                  const uint32_t* frame_base, uint32_t frame_off, int remap,
+                                  hipStream_t st) {
+  const Frag<PREC>* wtab = wfrag_table<N, Q, PREC>(st);
+  if (!wtab) return fail(SHWFS_ERR_HIP, "DFT batch: building the DFT operand table failed");
+  decltype(&sh_dft_tasks_kernel<N, Q, PREC, false, true, 4>) k = nullptr;
+#define SHWFS_PICK(NZ, FL)                                                             \
+  switch (nr) {                                                                        \
+    case 4: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 4>; break;                     \
+    case 2: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 2>; break;                     \
+    case 1: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 1>; break;                     \
+    default: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 0>; break;                    \
+  }
+  if (noisy) {
+    if (full) { SHWFS_PICK(true, true) } else { SHWFS_PICK(true, false) }
+  } else {
+    if (full) { SHWFS_PICK(false, true) } else { SHWFS_PICK(false, false) }
+  }
+#undef SHWFS_PICK
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64), lds, st, tasks, ntasks, total, wtab, frame_base,
+                     frame_off, remap);
+  SHWFS_LAUNCHED("sh_dft_tasks_kernel");
+  return SHWFS_OK;
+}
+
 // Largest grid of the DFT spot kernel (workgroups).
 constexpr int kDftMaxGrid = 65536;
 
@@ -438,6 +496,30 @@ int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
 }
 
 template <int PREC, int Q>
+int32_t launch_dft_tasks_pq(int nfft, bool full, int nr, bool noisy, const BatchTask* tasks,
+                            int ntasks, int total, int grid, size_t lds,
+                            const uint32_t* frame_base, uint32_t frame_off, int remap,
+                            hipStream_t st) {
+#define SHWFS_ARGS full, nr, noisy, tasks, ntasks, total, grid, lds, frame_base, frame_off, remap, st
+  if constexpr (Q == 1) {
+    switch (nfft) {
+      case 16: return launch_dft_tasks_n<16, 1, PREC>(SHWFS_ARGS);
+      case 32: return launch_dft_tasks_n<32, 1, PREC>(SHWFS_ARGS);
+      case 64: return launch_dft_tasks_n<64, 1, PREC>(SHWFS_ARGS);
+      default: break;
+    }
+  } else {
+    switch (nfft) {
+      case 32: return launch_dft_tasks_n<32, 2, PREC>(SHWFS_ARGS);
+      case 64: return launch_dft_tasks_n<64, 2, PREC>(SHWFS_ARGS);
+      default: break;
+    }
+  }
+#undef SHWFS_ARGS
+  return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT batch: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
+}
+
+template <int PREC, int Q>
 const void* dft_table_pq(int nfft, hipStream_t st) {
   if constexpr (Q == 1) {
     switch (nfft) {
@@ -457,7 +539,10 @@ const void* dft_table_pq(int nfft, hipStream_t st) {
 
 #define SHWFS_DFT_EXTERN(PREC, Q)                                                         \
   extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t); \
-  extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);
+  extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);                \
+  extern template int32_t launch_dft_tasks_pq<PREC, Q>(                                 \
+      int, bool, int, bool, const BatchTask*, int, int, int, size_t, const uint32_t*,   \
+      uint32_t, int, hipStream_t);
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 1)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 2)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 1)

@@ -80,6 +80,19 @@ struct ImageParams {
   float* cintens;
 };
 
+// One WFS of a batch (device task table): its parameters, stage kind and
+// range of global tiles.
+struct BatchTask {
+  ImageParams P;
+  int32_t kind;        // BatchKind
+  int32_t first_tile;  // global index of the task's first tile
+  int32_t ntiles;
+  int32_t per_tile;    // sub-apertures per tile
+  int32_t nr;          // DFT register rebin factor (0: gather)
+  int32_t pad_;
+  const void* wtab;    // DFT operand table
+};
+
 // Validated kernel parameters of one frame of one WFS (no launch).
 int32_t check_geom(const shwfs_sh_geom* g, int method);
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
@@ -91,6 +104,13 @@ int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, 
 // call_base is not NULL (device frame counter of a batch)
 int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
                           hipStream_t s, const unsigned* call_base = nullptr);
+// Same-kind batches of the MFMA stage: every task uses one (nfft, pdiam
+// class, method, full window, rebin) variant; one single-wave workgroup per
+// sub-aperture of the flattened task range.
+int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, bool noisy,
+                         const BatchTask* tasks, int ntasks, int total, int grid, size_t lds,
+                         const uint32_t* frame_base, uint32_t frame_off, int remap,
+                         hipStream_t st);
 // Device table of the DFT operand fragments for (nfft, pdiam, method), built
 // on first use on the current device; NULL (and the error set) otherwise.
 const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t s);

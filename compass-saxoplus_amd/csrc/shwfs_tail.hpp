@@ -51,6 +51,7 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
                                             float* E, int pe, int plane) {
   const int pdiam = PD > 0 ? PD : P.pdiam;  // PD: pdiam known at compile time
   const int np2 = pdiam * pdiam;
+  if (lane >= LANES) __builtin_unreachable();
   constexpr int CH = EPL < 16 ? EPL : 16;  // elements in flight per lane (register bound)
   if (valid) {
     const int npup = P.npup;
@@ -62,7 +63,8 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
     const int dx = LANES % pdiam, dy = LANES / pdiam;
     for (int c0 = 0; c0 < EPL; c0 += CH) {
       float ph[CH], pu[CH], of[CH];
-      int xs[CH], ys[CH], gs[CH];
+      int xs[CH], ys[CH];
+      unsigned gs[CH];  // non-negative window offsets: 32-bit vector offsets on a scalar base
 #pragma unroll
       for (int e = 0; e < CH; ++e) {
         const int idx = lane + (c0 + e) * LANES;
@@ -71,9 +73,9 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
         ys[e] = y;
         const int yc = in ? y : pdiam - 1;
         const int xc = in ? x : pdiam - 1;
-        gs[e] = xc + yc * npup;
+        gs[e] = (unsigned)(xc + yc * npup);
         ph[e] = phase[gs[e]];
-        of[e] = P.offsets[in ? idx : np2 - 1];
+        of[e] = P.offsets[(unsigned)(in ? idx : np2 - 1)];
         x += dx;
         y += dy;
         if (x >= pdiam) {
