@@ -319,6 +319,17 @@ __device__ __forceinline__ int dpp_int(int v, int ctrl_sel) {
   }
 }
 
+// In-row permutation moves without an `old` operand: with full row and bank
+// masks every lane receives a value (ctrl_sel as dpp_int, 0..3).
+__device__ __forceinline__ int dpp_row_perm(int v, int ctrl_sel) {
+  switch (ctrl_sel) {
+    case 0: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);
+    default: return __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true);
+  }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) v += __int_as_float(dpp_int(__float_as_int(v), c));
@@ -329,14 +340,64 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const long long b = __double_as_longlong(v);
-    const int lo = dpp_int((int)(b & 0xffffffffLL), c);
-    const int hi = dpp_int((int)(b >> 32), c);
+    const int lo = c < 4 ? dpp_row_perm((int)(b & 0xffffffffLL), c)
+                         : dpp_int((int)(b & 0xffffffffLL), c);
+    const int hi = c < 4 ? dpp_row_perm((int)(b >> 32), c) : dpp_int((int)(b >> 32), c);
     v += __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
   }
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 63);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
   return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each 16-lane row, held by every lane of the row.
+__device__ __forceinline__ float row_sum(float v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v += __int_as_float(dpp_int(__float_as_int(v), c));
+  return v;
+}
+__device__ __forceinline__ double row_sum(double v) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_row_perm((int)(b & 0xffffffffLL), c);
+    const int hi = dpp_row_perm((int)(b >> 32), c);
+    v += __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  return v;
+}
+
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Whole-wave totals of three per-lane partial sums, through 192 entries of
+// LDS private to the (single-wave) workgroup: every lane stores its three
+// partials, lane 16 q + r (q < 3) adds entries 4r .. 4r+3 of quantity q, the
+// rows reduce in registers and the totals are read from lanes 0, 16 and 32.
+// Contains a workgroup barrier: every lane calls it.
+template <typename T>
+__device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
+  red[lane] = a;
+  red[64 + lane] = b;
+  red[128 + lane] = c;
+  __syncthreads();
+  T v = T(0);
+  if (lane < 48) {
+    const T* p = red + (lane >> 4) * 64 + 4 * (lane & 15);
+    v = (p[0] + p[1]) + (p[2] + p[3]);
+  }
+  v = row_sum(v);
+  a = lane_value(v, 0);
+  b = lane_value(v, 16);
+  c = lane_value(v, 32);
 }
 
 __device__ __forceinline__ float wave_max(float v) {

@@ -45,11 +45,15 @@ struct DftLayout {
   static constexpr int EPLANE = P16 * PE;        // Er plane, then Ei plane
   static constexpr int EPL = (P16 * P16 + 63) / 64;  // field elements per lane
   // LDS per wave: the field planes; the general-binmap tail adds the |F|^2
-  // image (over the planes) and the spot
+  // image (over the planes) and the spot.  The moment totals use 192 doubles
+  // (kRedFloats): after the spot (binned in registers) or at the start (over
+  // the dead |F|^2 image).
+  static constexpr int kRedFloats = 384;
   static __host__ __device__ int wave_floats(int npix, bool gather) {
     const int spot = (npix * npix + 3) & ~3;
-    if (!gather) return 2 * EPLANE > spot ? 2 * EPLANE : spot;
-    return (2 * EPLANE > N * N ? 2 * EPLANE : N * N) + spot;
+    if (!gather) return 2 * EPLANE > spot + kRedFloats ? 2 * EPLANE : spot + kRedFloats;
+    const int img = 2 * EPLANE > N * N ? 2 * EPLANE : N * N;
+    return (img > kRedFloats ? img : kRedFloats) + spot;
   }
 };
 
@@ -219,6 +223,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         ev[ma][ks][e] = (FULL || (x0 + e < pdiam && y < pdiam)) ? v[e] : 0.0f;
     }
   float escale = 1.0f;
+  int escale_exp = 0;  // escale = 2^escale_exp
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
     // bound of max |G|: pdiam * max |E|, with max |E| <= 1 known for a
     // window of unit pupil, measured over the wave otherwise
@@ -233,7 +238,8 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
           for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[ma][ks][e]));
       amax = wave_max(m) * (float)pdiam;
     }
-    if (amax > 0.0f) escale = ldexpf(1.0f, 14 - ilogbf(amax));
+    if (amax > 0.0f) escale_exp = 14 - ilogbf(amax);
+    escale = ldexpf(1.0f, escale_exp);
   }
   F ae[Q][Q];
 #pragma unroll
@@ -254,7 +260,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
       ga[ma][nt] = acc;
     }
-  const float inten_scale = 1.0f / (escale * escale);
+  const float inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2
 
   // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
   // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
@@ -332,10 +338,12 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
     }
   }
   __syncthreads();
+  // moment totals: after the spot, or over the |F|^2 image once it is binned
+  void* red = nr > 0 ? smem + ((npix * npix + 3) & ~3) : smem;
   if (nr == 0) {
-    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add);
+    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add, red);
   } else {
-    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add);
+    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add, red);
   }
   __syncthreads();
 }
@@ -354,7 +362,7 @@ constexpr int dft_waves_per_eu() {
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
-void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
+void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
   constexpr int nr = NR;
   using L = DftLayout<N, Q>;
   extern __shared__ float smem[];
@@ -365,14 +373,13 @@ void sh_image_dft_kernel(ImageParams P, int iters, const Frag<PREC>* wtab) {
     for (int i = lane; i < nbm; i += 64) s_binmap[i] = P.binmap[i];
   }
   const AwOperand<N, Q, PREC> aw(wtab, lane);
-  // workgroup g handles sub-apertures g, g + G, ...; workgroups of one XCD
-  // hold contiguous g
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  for (int it = 0; it < iters; ++it) {
-    const int s = g + it * (int)gridDim.x;
-    if (s >= P.nvalid) break;  // uniform over the (single-wave) workgroup
-    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem, s_binmap, NR, aw);
-  }
+  // workgroup g handles sub-aperture g; workgroups of one XCD hold contiguous
+  // g.  One sub-aperture per workgroup, with no loop: values derived from the
+  // lane index would otherwise be hoisted out of the loop and held in
+  // registers across the whole body.
+  const int s = xcd_remap(blockIdx.x, gridDim.x);
+  if (s >= P.nvalid) return;  // uniform over the (single-wave) workgroup
+  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem, s_binmap, NR, aw);
 }
 
 // Same-kind batches: every task runs this (N, Q, PREC, FULL, NR) variant.
@@ -395,8 +402,12 @@ void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, int ntasks, int to
     int i = 0;
     while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
     const ImageParams& P = tasks[i].P;
-    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane, smem, P.binmap, NR,
-                                           aw, call);
+    // a fresh (opaque) lane index per iteration keeps lane-derived values
+    // inside the loop instead of holding them in registers across the body
+    int lane_t = lane;
+    asm volatile("" : "+v"(lane_t));
+    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane_t, smem, P.binmap,
+                                           NR, aw, call);
   }
 }
 
@@ -427,9 +438,6 @@ static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask
   return SHWFS_OK;
 }
 
-// Largest grid of the DFT spot kernel (workgroups).
-constexpr int kDftMaxGrid = 65536;
-
 // Register binning applies when the binmap is the NGS block form with a block
 // origin aligned on the rebin factor (nrebin in {1, 2, 4}); otherwise 0, the
 // general gather over |F|^2 in LDS.
@@ -446,20 +454,17 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "DFT spot kernel: LDS footprint exceeds the per-workgroup limit");
   // grid: one single-wave workgroup per sub-aperture (the dispatcher balances
-  // them over the CUs as they drain), up to kDftMaxGrid; beyond that each
-  // workgroup loops over sub-apertures g, g + grid, ...
+  // them over the CUs as they drain)
   const bool noisy = p.noise > -1.0f;
   const bool full = p.pdiam == L::P16;
   auto kern = noisy ? (full ? sh_image_dft_kernel<N, Q, PREC, true, true, NR>
                             : sh_image_dft_kernel<N, Q, PREC, true, false, NR>)
                     : (full ? sh_image_dft_kernel<N, Q, PREC, false, true, NR>
                             : sh_image_dft_kernel<N, Q, PREC, false, false, NR>);
-  int grid = p.nvalid < kDftMaxGrid ? p.nvalid : kDftMaxGrid;
-  if (grid < 1) grid = 1;
-  const int iters = (p.nvalid + grid - 1) / grid;
+  const int grid = p.nvalid > 1 ? p.nvalid : 1;
   const Frag<PREC>* wtab = wfrag_table<N, Q, PREC>(st);
   if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, p, iters, wtab);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, p, wtab);
   SHWFS_LAUNCHED("sh_image_dft_kernel");
   return SHWFS_OK;
 }

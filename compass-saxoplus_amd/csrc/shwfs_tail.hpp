@@ -126,15 +126,42 @@ __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, boo
   }
 }
 
+// One wave (64 lanes) per sub-aperture: the three moment sums reach their
+// totals through LDS (wave_sum3); lanes 0 and 1 divide and write the x and y
+// slopes.  red: 192 entries of Acc of LDS private to the workgroup.
+template <typename Acc>
+__device__ __forceinline__ void centroid_finish64(const ImageParams& P, int s, bool valid,
+                                                  int lane, Acc I, Acc Sx, Acc Sy, Acc* red) {
+  wave_sum3(red, lane, I, Sx, Sy);
+  if (valid && lane < 2) {
+    const int iy = lane;  // lane 0: x slope, lane 1: y slope
+    const int ix = P.slope_order == 0 ? s + iy * P.nvalid : 2 * s + iy;
+    const float r = P.ref ? P.ref[ix] : 0.0f;
+    const Acc num = iy ? Sy : Sx;
+    P.slopes[ix] = ((float)((double)num / ((double)I + 1.e-6)) - P.coffset) * P.cscale - r;
+    if (iy == 0) P.cintens[s] = (float)I;
+  }
+}
+
+// Row of pixel p in a spot of npix columns: p / npix through the float
+// reciprocal of npix (exact for p < 2^20: (p + 1/2) / npix stays at least
+// 1/(2 npix) away from an integer, far beyond the few-ulp product error).
+__device__ __forceinline__ int pixel_row(int p, float inv_npix) {
+  return (int)(((float)p + 0.5f) * inv_npix);
+}
+
 // Binned spot -> flux normalisation, noise, bincube / binimg, fused
 // centroid.  bin: the npix^2 binned pixels of sub-aperture s in LDS (x
 // fastest), in units of 1/inten_scale; I: their sum (the same units).
+// red: for LANES == 64, LDS scratch of 192 doubles for the moment totals
+// (wave_sum3; must not overlap bin), or nullptr for the DPP reductions.
 template <int LANES, bool NOISE>
 __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool valid, int lane,
                                             const float* bin, float I, float inten_scale,
-                                            unsigned call_add = 0u) {
+                                            unsigned call_add = 0u, void* red = nullptr) {
   const int npix = P.npix;
   const int npix2 = npix * npix;
+  const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
   float cI = 0.0f, cX = 0.0f, cY = 0.0f;
   double dI = 0.0, dX = 0.0, dY = 0.0;
   if (valid) {
@@ -143,7 +170,7 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     const int vy = P.validsubsy[s];
     if (lane == 0) P.intensities[s] = I * inten_scale;
     // photometric normalisation b * flux / I * nphot as one factor
-    const float fnorm = (I != 0.0f) ? fl / I * P.nphot : 1.0f;
+    const float fnorm = (I != 0.0f) ? fl * __builtin_amdgcn_rcpf(I) * P.nphot : 1.0f;
     for (int p = lane; p < npix2; p += LANES) {
       float b = bin[p] * fnorm;
       const unsigned e = (unsigned)(p + s * npix2);
@@ -153,7 +180,7 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
         if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
       }
       P.bincube[e] = b;
-      const int py = p / npix, px = p - py * npix;
+      const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
       float out = b;
       if (P.ckind != SHWFS_CENTROID_NONE) {
         float d = b;
@@ -175,6 +202,15 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     }
   }
   if (P.ckind != SHWFS_CENTROID_NONE) {
+    if constexpr (LANES == 64) {
+      if (red != nullptr) {
+        if (P.fp64)
+          centroid_finish64<double>(P, s, valid, lane, dI, dX, dY, static_cast<double*>(red));
+        else
+          centroid_finish64<float>(P, s, valid, lane, cI, cX, cY, static_cast<float*>(red));
+        return;
+      }
+    }
     if (P.fp64)
       centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY);
     else
@@ -188,7 +224,8 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
 template <int LANES, int NF, bool NOISE>
 __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
                                           const float* hr, float* bin, const int* s_binmap,
-                                          float inten_scale, unsigned call_add = 0u) {
+                                          float inten_scale, unsigned call_add = 0u,
+                                          void* red = nullptr) {
   const int npix2 = P.npix * P.npix;
   const int nreb2 = P.nrebin * P.nrebin;
   float part = 0.0f;
@@ -219,7 +256,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
     }
   }
   const float I = lane_group_sum<LANES>(part);
-  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add);
+  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add, red);
 }
 
 }  // namespace shwfs
