@@ -23,6 +23,7 @@ struct shwfs_batch {
   bool noisy = false;
   // device task table
   BatchTask* d_tasks = nullptr;
+  TaskStarts starts{};  // first tiles of the first kTaskStarts tasks
   int ntasks = 0, total_tiles = 0;
   int kind_set = 0;  // BatchKindSet covering the tasks
   // every task the same MFMA variant: launched through the specialised
@@ -147,6 +148,8 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   }
   if (b->d_tasks == nullptr) SHWFS_HIP(hipMalloc((void**)&b->d_tasks, sizeof(BatchTask) * n));
   SHWFS_HIP(hipMemcpy(b->d_tasks, sorted.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice));
+  b->starts = TaskStarts{};
+  for (int k = 0; k < n && k < kTaskStarts; ++k) b->starts.first[k] = sorted[k].first_tile;
   b->ntasks = n;
   b->total_tiles = first;
   b->lds = lds;
@@ -161,6 +164,8 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
     uni = tasks[i].kind == tasks[0].kind && tasks[i].nr == tasks[0].nr &&
           b->geoms[i].nphase == b->geoms[0].nphase;
   }
+  // the same-kind kernel holds kDftWaves waves' slots per workgroup
+  if (uni && lds * kDftWaves > max_lds_per_block()) uni = false;
   if (uni) {
     b->uni_method = m0;
     b->uni_nfft = b->geoms[0].nfft;
@@ -197,12 +202,14 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
   const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
   if (b->uni_method >= 0)
     return launch_dft_tasks(b->uni_method, b->uni_nfft, b->uni_pdiam, b->uni_full, b->uni_nr,
-                            b->noisy, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+                            b->noisy, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid, b->lds,
                             b->d_frame, frame_off, b->remap, st);
   if (b->noisy)
-    return launch_batch_noisy(b->kind_set, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+    return launch_batch_noisy(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
+                              b->lds,
                               b->d_frame, frame_off, b->remap, st);
-  return launch_batch_quiet(b->kind_set, b->d_tasks, b->ntasks, b->total_tiles, grid, b->lds,
+  return launch_batch_quiet(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
+                            b->lds,
                             b->d_frame, frame_off, b->remap, st);
 }
 
@@ -448,6 +455,7 @@ int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps,
   for (int gi = 0; gi < 3; ++gi)
     for (int rm = 0; rm < 2; ++rm) {
       if (grids[gi] == b->grid && rm == b->remap) continue;
+      if (b->uni_method >= 0 && grids[gi] != 0) continue;  // same-kind kernel: no persistent grid
       const int sg = b->grid, sr = b->remap;
       b->grid = grids[gi];
       b->remap = rm;

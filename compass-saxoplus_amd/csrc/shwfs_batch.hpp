@@ -71,15 +71,15 @@ __host__ __device__ constexpr int batch_kind_set(int kind) {
 template <bool NOISE, int SET>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(SET == BKS_FP16 && !NOISE ? 5 : 4)))
-void sh_batch_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total_tiles,
-                     const uint32_t* __restrict__ frame_base, uint32_t frame_off, int remap) {
+void sh_batch_kernel(const BatchTask* __restrict__ tasks, const TaskStarts starts, int ntasks,
+                     int total_tiles, const uint32_t* __restrict__ frame_base, uint32_t frame_off,
+                     int remap) {
   extern __shared__ float smem[];
   const uint32_t fcall = frame_base[0] + frame_off;
   const int G = (int)gridDim.x;
   const int b0 = remap ? xcd_remap((int)blockIdx.x, G) : (int)blockIdx.x;
   for (int t = b0; t < total_tiles; t += G) {
-    int i = 0;
-    while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
+    const int i = find_task(starts, tasks, ntasks, t);
     // the task's parameters are read in place (uniform addresses: scalar
     // loads); a private copy would live in scratch memory
     const BatchTask& T = tasks[i];
@@ -140,10 +140,12 @@ void sh_batch_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total_
 
 // Launchers of the instantiations (noise-free / noisy in translation units of
 // their own); `set` is a BatchKindSet.
-int32_t launch_batch_quiet(int set, const BatchTask* tasks, int ntasks, int total_tiles, int grid,
+int32_t launch_batch_quiet(int set, const BatchTask* tasks, const TaskStarts& starts, int ntasks,
+                           int total_tiles, int grid,
                            size_t lds, const uint32_t* frame_base, uint32_t frame_off, int remap,
                            hipStream_t st);
-int32_t launch_batch_noisy(int set, const BatchTask* tasks, int ntasks, int total_tiles, int grid,
+int32_t launch_batch_noisy(int set, const BatchTask* tasks, const TaskStarts& starts, int ntasks,
+                           int total_tiles, int grid,
                            size_t lds, const uint32_t* frame_base, uint32_t frame_off, int remap,
                            hipStream_t st);
 

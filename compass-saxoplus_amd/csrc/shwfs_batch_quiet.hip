@@ -4,23 +4,26 @@
 namespace shwfs {
 
 template <int SET>
-static int32_t launch_set(const BatchTask* tasks, int ntasks, int total_tiles, int grid, size_t lds,
+static int32_t launch_set(const BatchTask* tasks, const TaskStarts& starts, int ntasks,
+                          int total_tiles, int grid, size_t lds,
                           const uint32_t* frame_base, uint32_t frame_off, int remap,
                           hipStream_t st) {
-  hipLaunchKernelGGL((sh_batch_kernel<false, SET>), dim3(grid), dim3(64), lds, st, tasks, ntasks,
+  hipLaunchKernelGGL((sh_batch_kernel<false, SET>), dim3(grid), dim3(64), lds, st, tasks, starts,
+                     ntasks,
                      total_tiles, frame_base, frame_off, remap);
   SHWFS_LAUNCHED("sh_batch_kernel");
   return SHWFS_OK;
 }
 
-int32_t launch_batch_quiet(int set, const BatchTask* tasks, int ntasks, int total_tiles, int grid,
+int32_t launch_batch_quiet(int set, const BatchTask* tasks, const TaskStarts& starts, int ntasks,
+                           int total_tiles, int grid,
                            size_t lds, const uint32_t* frame_base, uint32_t frame_off, int remap,
                            hipStream_t st) {
   switch (set) {
-    case BKS_FFT: return launch_set<BKS_FFT>(tasks, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
-    case BKS_BF16: return launch_set<BKS_BF16>(tasks, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
-    case BKS_FP16: return launch_set<BKS_FP16>(tasks, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
-    default: return launch_set<BKS_ALL>(tasks, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
+    case BKS_FFT: return launch_set<BKS_FFT>(tasks, starts, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
+    case BKS_BF16: return launch_set<BKS_BF16>(tasks, starts, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
+    case BKS_FP16: return launch_set<BKS_FP16>(tasks, starts, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
+    default: return launch_set<BKS_ALL>(tasks, starts, ntasks, total_tiles, grid, lds, frame_base, frame_off, remap, st);
   }
 }
 

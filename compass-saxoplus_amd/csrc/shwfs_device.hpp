@@ -378,17 +378,26 @@ __device__ __forceinline__ double lane_value(double v, int l) {
   return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// Orders the LDS accesses of one wave around a hand-over between its own
+// lanes: LDS operations of a wave complete in order, and the fences keep the
+// compiler from moving accesses across.  No workgroup barrier: other waves of
+// the workgroup are not waited for.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Whole-wave totals of three per-lane partial sums, through 192 entries of
-// LDS private to the (single-wave) workgroup: every lane stores its three
-// partials, lane 16 q + r (q < 3) adds entries 4r .. 4r+3 of quantity q, the
-// rows reduce in registers and the totals are read from lanes 0, 16 and 32.
-// Contains a workgroup barrier: every lane calls it.
+// LDS private to the wave: every lane stores its three partials, lane
+// 16 q + r (q < 3) adds entries 4r .. 4r+3 of quantity q, the rows reduce in
+// registers and the totals are read from lanes 0, 16 and 32.
 template <typename T>
 __device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
   red[lane] = a;
   red[64 + lane] = b;
   red[128 + lane] = c;
-  __syncthreads();
+  wave_lds_sync();
   T v = T(0);
   if (lane < 48) {
     const T* p = red + (lane >> 4) * 64 + 4 * (lane & 15);

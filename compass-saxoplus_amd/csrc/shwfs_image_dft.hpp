@@ -197,6 +197,8 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
+  SubapScalars pre;
+  pre.load(P, s);
 
   // ---- field: coalesced window loads (x fastest over lanes) -> Er, Ei planes
   // in LDS -> stage A operand, A_E[y][k] with y = 16 ma + (lane & 15) and
@@ -204,7 +206,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   // Ei at x = k - P16 (0 outside the pdiam window)
   load_fields<64, L::EPL, true, true, FULL ? L::P16 : 0>(P, s, true, lane, smem, L::PE,
                                                           L::EPLANE);
-  __syncthreads();
+  wave_lds_sync();
   float ev[Q][Q][8];
 #pragma unroll
   for (int ma = 0; ma < Q; ++ma)
@@ -270,7 +272,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   float* hr = smem;   // nr == 0: hr[kx + ky*N]
   float* bin = nr > 0 ? smem : smem + (L::wave_floats(npix, true) - ((npix * npix + 3) & ~3));
   float part = 0.0f;  // this lane's share of the binned flux (nr > 0)
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int nt = 0; nt < L::NTB; ++nt) {
     F bg[L::KS];
@@ -337,15 +339,16 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       }
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // moment totals: after the spot, or over the |F|^2 image once it is binned
   void* red = nr > 0 ? smem + ((npix * npix + 3) & ~3) : smem;
   if (nr == 0) {
-    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add, red);
+    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add, red, &pre);
   } else {
-    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add, red);
+    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add, red,
+                           &pre);
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 // Register budgets (waves per SIMD), the largest at which the code still
@@ -359,60 +362,72 @@ constexpr int dft_waves_per_eu() {
   return (PREC == SHWFS_METHOD_DFT_FP16 && FULL && NR == 4) ? 7 : 5;
 }
 
+// Waves per workgroup of the spot kernels.  The waves are independent (one
+// sub-aperture and one LDS slot each, no workgroup barrier after the start):
+// grouping them lifts the per-CU workgroup limit off the number of resident
+// waves.
+constexpr int kDftWaves = 4;
+
+// One frame: wave w of workgroup g handles sub-aperture kDftWaves g' + w with
+// g' = xcd_remap(g) (workgroups of one XCD hold contiguous sub-apertures).
+// One sub-aperture per wave, with no loop: values derived from the lane index
+// would otherwise be hoisted out of the loop and held in registers across the
+// whole body.  Spare waves of the last workgroup recompute the last
+// sub-aperture (same values to the same addresses) rather than exit early: an
+// exit test would hold back the parameter loads behind a load of nvalid.
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
 void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
   constexpr int nr = NR;
   using L = DftLayout<N, Q>;
   extern __shared__ float smem[];
-  const int lane = threadIdx.x;
-  int* s_binmap = (int*)(smem + L::wave_floats(P.npix, nr == 0));
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int WF = L::wave_floats(P.npix, nr == 0);
+  int* s_binmap = (int*)(smem + kDftWaves * WF);
   if (nr == 0 && !P.bin_block) {
     const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
-    for (int i = lane; i < nbm; i += 64) s_binmap[i] = P.binmap[i];
+    for (int i = threadIdx.x; i < nbm; i += 64 * kDftWaves) s_binmap[i] = P.binmap[i];
+    __syncthreads();
   }
   const AwOperand<N, Q, PREC> aw(wtab, lane);
-  // workgroup g handles sub-aperture g; workgroups of one XCD hold contiguous
-  // g.  One sub-aperture per workgroup, with no loop: values derived from the
-  // lane index would otherwise be hoisted out of the loop and held in
-  // registers across the whole body.
-  const int s = xcd_remap(blockIdx.x, gridDim.x);
-  if (s >= P.nvalid) return;  // uniform over the (single-wave) workgroup
-  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem, s_binmap, NR, aw);
+  const int s = min(xcd_remap(blockIdx.x, gridDim.x) * kDftWaves + wave, P.nvalid - 1);
+  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, NR, aw);
 }
 
 // Same-kind batches: every task runs this (N, Q, PREC, FULL, NR) variant.
-// Workgroup b handles global sub-apertures b, b + grid, ... of the flattened
-// task range (task i owns [first_tile, first_tile + ntiles)); task
-// parameters are read in place (uniform addresses, scalar loads).
+// Wave w of workgroup b handles global sub-aperture kDftWaves b' + w of the
+// flattened task range (b' = b, or xcd_remap(b) with remap; task i owns
+// [first_tile, first_tile + ntiles)); spare waves of the last workgroup
+// recompute the last sub-aperture.  slot: LDS floats per wave (the largest
+// slot of the tasks).  The task's parameters are copied whole at the start
+// (scalar loads issued together) rather than read field by field where they
+// are used.
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
-void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, int ntasks, int total,
-                         const Frag<PREC>* wtab, const uint32_t* __restrict__ frame_base,
-                         uint32_t frame_off, int remap) {
+void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts starts, int ntasks,
+                         int total, int slot, const Frag<PREC>* wtab,
+                         const uint32_t* __restrict__ frame_base, uint32_t frame_off, int remap) {
   extern __shared__ float smem[];
-  const int lane = threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const uint32_t call = frame_base[0] + frame_off;
   const AwOperand<N, Q, PREC> aw(wtab, lane);
   const int G = (int)gridDim.x;
-  const int b0 = remap ? xcd_remap((int)blockIdx.x, G) : (int)blockIdx.x;
-  for (int t = b0; t < total; t += G) {
-    int i = 0;
-    while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
-    const ImageParams& P = tasks[i].P;
-    // a fresh (opaque) lane index per iteration keeps lane-derived values
-    // inside the loop instead of holding them in registers across the body
-    int lane_t = lane;
-    asm volatile("" : "+v"(lane_t));
-    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane_t, smem, P.binmap,
-                                           NR, aw, call);
-  }
+  const int b = (int)blockIdx.x;
+  const int bb = b + (remap != 0) * (xcd_remap(b, G) - b);  // no branch ahead of the loads
+  const int t = min(bb * kDftWaves + wave, total - 1);
+  const int i = find_task(starts, tasks, ntasks, t);
+  const ImageParams P = tasks[i].P;
+  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane, smem + wave * slot,
+                                         P.binmap, NR, aw, call);
 }
 
 template <int N, int Q, int PREC>
 static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask* tasks,
+                                  const TaskStarts& starts,
                                   int ntasks, int total, int grid, size_t lds,
                                   const uint32_t* frame_base, uint32_t frame_off, int remap,
                                   hipStream_t st) {
@@ -432,7 +447,13 @@ static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask
     if (full) { SHWFS_PICK(false, true) } else { SHWFS_PICK(false, false) }
   }
 #undef SHWFS_PICK
-  hipLaunchKernelGGL(k, dim3(grid), dim3(64), lds, st, tasks, ntasks, total, wtab, frame_base,
+  (void)grid;  // one wave per sub-aperture
+  if (total <= 0) return SHWFS_OK;
+  if (lds * kDftWaves > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: LDS footprint exceeds the per-workgroup limit");
+  const int slot = (int)(lds / sizeof(float));
+  hipLaunchKernelGGL(k, dim3((total + kDftWaves - 1) / kDftWaves), dim3(64 * kDftWaves),
+                     lds * kDftWaves, st, tasks, starts, ntasks, total, slot, wtab, frame_base,
                      frame_off, remap);
   SHWFS_LAUNCHED("sh_dft_tasks_kernel");
   return SHWFS_OK;
@@ -450,7 +471,7 @@ template <int N, int Q, int PREC, int NR>
 static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
   using L = DftLayout<N, Q>;
   const int nbm = (NR == 0 && !p.bin_block) ? p.nrebin * p.nrebin * p.npix * p.npix : 0;
-  const size_t lds = sizeof(float) * ((size_t)L::wave_floats(p.npix, NR == 0) + nbm);
+  const size_t lds = sizeof(float) * ((size_t)kDftWaves * L::wave_floats(p.npix, NR == 0) + nbm);
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "DFT spot kernel: LDS footprint exceeds the per-workgroup limit");
   // grid: one single-wave workgroup per sub-aperture (the dispatcher balances
@@ -461,10 +482,11 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
                             : sh_image_dft_kernel<N, Q, PREC, true, false, NR>)
                     : (full ? sh_image_dft_kernel<N, Q, PREC, false, true, NR>
                             : sh_image_dft_kernel<N, Q, PREC, false, false, NR>);
-  const int grid = p.nvalid > 1 ? p.nvalid : 1;
   const Frag<PREC>* wtab = wfrag_table<N, Q, PREC>(st);
   if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, st, p, wtab);
+  if (p.nvalid <= 0) return SHWFS_OK;
+  const int grid = (p.nvalid + kDftWaves - 1) / kDftWaves;  // one wave per sub-aperture
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kDftWaves), lds, st, p, wtab);
   SHWFS_LAUNCHED("sh_image_dft_kernel");
   return SHWFS_OK;
 }
@@ -502,10 +524,12 @@ int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
 
 template <int PREC, int Q>
 int32_t launch_dft_tasks_pq(int nfft, bool full, int nr, bool noisy, const BatchTask* tasks,
+                            const TaskStarts& starts,
                             int ntasks, int total, int grid, size_t lds,
                             const uint32_t* frame_base, uint32_t frame_off, int remap,
                             hipStream_t st) {
-#define SHWFS_ARGS full, nr, noisy, tasks, ntasks, total, grid, lds, frame_base, frame_off, remap, st
+#define SHWFS_ARGS \
+  full, nr, noisy, tasks, starts, ntasks, total, grid, lds, frame_base, frame_off, remap, st
   if constexpr (Q == 1) {
     switch (nfft) {
       case 16: return launch_dft_tasks_n<16, 1, PREC>(SHWFS_ARGS);
@@ -546,8 +570,8 @@ const void* dft_table_pq(int nfft, hipStream_t st) {
   extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t); \
   extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);                \
   extern template int32_t launch_dft_tasks_pq<PREC, Q>(                                 \
-      int, bool, int, bool, const BatchTask*, int, int, int, size_t, const uint32_t*,   \
-      uint32_t, int, hipStream_t);
+      int, bool, int, bool, const BatchTask*, const TaskStarts&, int, int, int, size_t, \
+      const uint32_t*, uint32_t, int, hipStream_t);
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 1)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 2)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 1)

@@ -93,6 +93,28 @@ struct BatchTask {
   const void* wtab;    // DFT operand table
 };
 
+// First tile of each task (ascending), passed by value as a kernel argument:
+// a workgroup finds its task with scalar compares instead of a chain of
+// dependent loads from the task table; tasks past kTaskStarts are found in
+// the table.
+constexpr int kTaskStarts = 16;
+struct TaskStarts {
+  int32_t first[kTaskStarts];
+};
+
+#if defined(__HIPCC__)
+__device__ __forceinline__ int find_task(const TaskStarts& ts, const BatchTask* tasks, int ntasks,
+                                         int t) {
+  int i = 0;
+#pragma unroll
+  for (int k = 1; k < kTaskStarts; ++k) i += (k < ntasks && ts.first[k] <= t) ? 1 : 0;
+  if (ntasks > kTaskStarts && i == kTaskStarts - 1) {  // past the argument table
+    while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
+  }
+  return i;
+}
+#endif
+
 // Validated kernel parameters of one frame of one WFS (no launch).
 int32_t check_geom(const shwfs_sh_geom* g, int method);
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
@@ -108,7 +130,8 @@ int32_t launch_background(float* img, long n, float sigma, unsigned long long se
 // class, method, full window, rebin) variant; one single-wave workgroup per
 // sub-aperture of the flattened task range.
 int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, bool noisy,
-                         const BatchTask* tasks, int ntasks, int total, int grid, size_t lds,
+                         const BatchTask* tasks, const TaskStarts& starts, int ntasks, int total,
+                         int grid, size_t lds,
                          const uint32_t* frame_base, uint32_t frame_off, int remap,
                          hipStream_t st);
 // Device table of the DFT operand fragments for (nfft, pdiam, method), built

@@ -126,17 +126,38 @@ __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, boo
   }
 }
 
+// Per-sub-aperture scalars of the tail (flux, detector offsets, reference
+// slopes), loaded when a wave starts so that their latency overlaps the spot
+// computation instead of adding to the end of it.
+struct SubapScalars {
+  float fl, rx, ry;
+  int vx, vy;
+  __device__ __forceinline__ void load(const ImageParams& P, int s) {
+    fl = P.flux[s];
+    vx = P.validsubsx[s];
+    vy = P.validsubsy[s];
+    rx = 0.0f;
+    ry = 0.0f;
+    if (P.ref) {
+      rx = P.ref[P.slope_order == 0 ? s : 2 * s];
+      ry = P.ref[P.slope_order == 0 ? s + P.nvalid : 2 * s + 1];
+    }
+  }
+};
+
 // One wave (64 lanes) per sub-aperture: the three moment sums reach their
 // totals through LDS (wave_sum3); lanes 0 and 1 divide and write the x and y
-// slopes.  red: 192 entries of Acc of LDS private to the workgroup.
+// slopes.  red: 192 entries of Acc of LDS private to the workgroup; pre: the
+// prefetched reference slopes, or nullptr.
 template <typename Acc>
 __device__ __forceinline__ void centroid_finish64(const ImageParams& P, int s, bool valid,
-                                                  int lane, Acc I, Acc Sx, Acc Sy, Acc* red) {
+                                                  int lane, Acc I, Acc Sx, Acc Sy, Acc* red,
+                                                  const SubapScalars* pre = nullptr) {
   wave_sum3(red, lane, I, Sx, Sy);
   if (valid && lane < 2) {
     const int iy = lane;  // lane 0: x slope, lane 1: y slope
     const int ix = P.slope_order == 0 ? s + iy * P.nvalid : 2 * s + iy;
-    const float r = P.ref ? P.ref[ix] : 0.0f;
+    const float r = pre ? (iy ? pre->ry : pre->rx) : (P.ref ? P.ref[ix] : 0.0f);
     const Acc num = iy ? Sy : Sx;
     P.slopes[ix] = ((float)((double)num / ((double)I + 1.e-6)) - P.coffset) * P.cscale - r;
     if (iy == 0) P.cintens[s] = (float)I;
@@ -155,19 +176,21 @@ __device__ __forceinline__ int pixel_row(int p, float inv_npix) {
 // fastest), in units of 1/inten_scale; I: their sum (the same units).
 // red: for LANES == 64, LDS scratch of 192 doubles for the moment totals
 // (wave_sum3; must not overlap bin), or nullptr for the DPP reductions.
+// pre: the sub-aperture's scalars loaded beforehand, or nullptr.
 template <int LANES, bool NOISE>
 __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool valid, int lane,
                                             const float* bin, float I, float inten_scale,
-                                            unsigned call_add = 0u, void* red = nullptr) {
+                                            unsigned call_add = 0u, void* red = nullptr,
+                                            const SubapScalars* pre = nullptr) {
   const int npix = P.npix;
   const int npix2 = npix * npix;
   const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
   float cI = 0.0f, cX = 0.0f, cY = 0.0f;
   double dI = 0.0, dX = 0.0, dY = 0.0;
   if (valid) {
-    const float fl = P.flux[s];
-    const int vx = P.validsubsx[s];
-    const int vy = P.validsubsy[s];
+    const float fl = pre ? pre->fl : P.flux[s];
+    const int vx = pre ? pre->vx : P.validsubsx[s];
+    const int vy = pre ? pre->vy : P.validsubsy[s];
     if (lane == 0) P.intensities[s] = I * inten_scale;
     // photometric normalisation b * flux / I * nphot as one factor
     const float fnorm = (I != 0.0f) ? fl * __builtin_amdgcn_rcpf(I) * P.nphot : 1.0f;
@@ -205,9 +228,10 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     if constexpr (LANES == 64) {
       if (red != nullptr) {
         if (P.fp64)
-          centroid_finish64<double>(P, s, valid, lane, dI, dX, dY, static_cast<double*>(red));
+          centroid_finish64<double>(P, s, valid, lane, dI, dX, dY, static_cast<double*>(red),
+                                    pre);
         else
-          centroid_finish64<float>(P, s, valid, lane, cI, cX, cY, static_cast<float*>(red));
+          centroid_finish64<float>(P, s, valid, lane, cI, cX, cY, static_cast<float*>(red), pre);
         return;
       }
     }
@@ -225,7 +249,8 @@ template <int LANES, int NF, bool NOISE>
 __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
                                           const float* hr, float* bin, const int* s_binmap,
                                           float inten_scale, unsigned call_add = 0u,
-                                          void* red = nullptr) {
+                                          void* red = nullptr,
+                                          const SubapScalars* pre = nullptr) {
   const int npix2 = P.npix * P.npix;
   const int nreb2 = P.nrebin * P.nrebin;
   float part = 0.0f;
@@ -256,7 +281,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
     }
   }
   const float I = lane_group_sum<LANES>(part);
-  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add, red);
+  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add, red, pre);
 }
 
 }  // namespace shwfs
