@@ -24,6 +24,7 @@ struct shwfs_batch {
   // device task table
   BatchTask* d_tasks = nullptr;
   TaskStarts starts{};  // first tiles of the first kTaskStarts tasks
+  TaskArgs args{};      // parameters of the first kTaskArgs tasks
   int ntasks = 0, total_tiles = 0;
   int kind_set = 0;  // BatchKindSet covering the tasks
   // every task the same MFMA variant: launched through the specialised
@@ -150,6 +151,8 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   SHWFS_HIP(hipMemcpy(b->d_tasks, sorted.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice));
   b->starts = TaskStarts{};
   for (int k = 0; k < n && k < kTaskStarts; ++k) b->starts.first[k] = sorted[k].first_tile;
+  b->args = TaskArgs{};
+  for (int k = 0; k < n && k < kTaskArgs; ++k) b->args.P[k] = sorted[k].P;
   b->ntasks = n;
   b->total_tiles = first;
   b->lds = lds;
@@ -166,6 +169,7 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   }
   // the same-kind kernel holds kDftWaves waves' slots per workgroup
   if (uni && lds * kDftWaves > max_lds_per_block()) uni = false;
+  if (n > kTaskArgs) uni = false;  // task parameters travel as kernel arguments
   if (uni) {
     b->uni_method = m0;
     b->uni_nfft = b->geoms[0].nfft;
@@ -202,7 +206,8 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
   const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
   if (b->uni_method >= 0)
     return launch_dft_tasks(b->uni_method, b->uni_nfft, b->uni_pdiam, b->uni_full, b->uni_nr,
-                            b->noisy, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid, b->lds,
+                            b->noisy, b->d_tasks, b->starts, b->args, b->ntasks, b->total_tiles, grid,
+                            b->lds,
                             b->d_frame, frame_off, b->remap, st);
   if (b->noisy)
     return launch_batch_noisy(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,

@@ -5,7 +5,7 @@ namespace shwfs {
 template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 2>(const ImageParams&, int, hipStream_t);
 template const void* dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 2>(int, hipStream_t);
 template int32_t launch_dft_tasks_pq<SHWFS_METHOD_DFT_BF16X3, 2>(int, bool, int, bool, const BatchTask*,
-                                                   const TaskStarts&, int, int,
+                                                   const TaskStarts&, const TaskArgs&, int, int,
                                                    int, size_t, const uint32_t*, uint32_t, int,
                                                    hipStream_t);
 }  // namespace shwfs

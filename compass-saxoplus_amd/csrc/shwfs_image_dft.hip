@@ -28,15 +28,15 @@ const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t st) {
 }
 
 int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, bool noisy,
-                         const BatchTask* tasks, const TaskStarts& starts, int ntasks, int total,
-                         int grid, size_t lds,
+                         const BatchTask* tasks, const TaskStarts& starts, const TaskArgs& args,
+                         int ntasks, int total, int grid, size_t lds,
                          const uint32_t* frame_base, uint32_t frame_off, int remap,
                          hipStream_t st) {
   const bool q1 = pdiam <= 16;
   if (pdiam > 32)
     return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT batch: pdiam must be <= 32");
 #define SHWFS_ARGS \
-  nfft, full, nr, noisy, tasks, starts, ntasks, total, grid, lds, frame_base, frame_off, remap, st
+  nfft, full, nr, noisy, tasks, starts, args, ntasks, total, grid, lds, frame_base, frame_off, remap, st
   if (method == SHWFS_METHOD_DFT_FP16)
     return q1 ? launch_dft_tasks_pq<SHWFS_METHOD_DFT_FP16, 1>(SHWFS_ARGS)
               : launch_dft_tasks_pq<SHWFS_METHOD_DFT_FP16, 2>(SHWFS_ARGS);

@@ -401,15 +401,15 @@ void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
 // flattened task range (b' = b, or xcd_remap(b) with remap; task i owns
 // [first_tile, first_tile + ntiles)); spare waves of the last workgroup
 // recompute the last sub-aperture.  slot: LDS floats per wave (the largest
-// slot of the tasks).  The task's parameters are copied whole at the start
-// (scalar loads issued together) rather than read field by field where they
-// are used.
+// slot of the tasks).  At most kTaskArgs tasks: their parameters come with the
+// kernel arguments.
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
-void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts starts, int ntasks,
-                         int total, int slot, const Frag<PREC>* wtab,
-                         const uint32_t* __restrict__ frame_base, uint32_t frame_off, int remap) {
+void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts starts,
+                         const TaskArgs args, int ntasks, int total, int slot,
+                         const Frag<PREC>* wtab, const uint32_t* __restrict__ frame_base,
+                         uint32_t frame_off, int remap) {
   extern __shared__ float smem[];
   const int lane = (int)threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -419,15 +419,15 @@ void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts s
   const int b = (int)blockIdx.x;
   const int bb = b + (remap != 0) * (xcd_remap(b, G) - b);  // no branch ahead of the loads
   const int t = min(bb * kDftWaves + wave, total - 1);
-  const int i = find_task(starts, tasks, ntasks, t);
-  const ImageParams P = tasks[i].P;
-  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - tasks[i].first_tile, lane, smem + wave * slot,
+  const int i = find_task_args(starts, ntasks, t);
+  const ImageParams& P = args.P[i];
+  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - starts.first[i], lane, smem + wave * slot,
                                          P.binmap, NR, aw, call);
 }
 
 template <int N, int Q, int PREC>
 static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask* tasks,
-                                  const TaskStarts& starts,
+                                  const TaskStarts& starts, const TaskArgs& args,
                                   int ntasks, int total, int grid, size_t lds,
                                   const uint32_t* frame_base, uint32_t frame_off, int remap,
                                   hipStream_t st) {
@@ -451,10 +451,11 @@ static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask
   if (total <= 0) return SHWFS_OK;
   if (lds * kDftWaves > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: LDS footprint exceeds the per-workgroup limit");
+  if (ntasks > kTaskArgs) return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: more than 8 tasks");
   const int slot = (int)(lds / sizeof(float));
   hipLaunchKernelGGL(k, dim3((total + kDftWaves - 1) / kDftWaves), dim3(64 * kDftWaves),
-                     lds * kDftWaves, st, tasks, starts, ntasks, total, slot, wtab, frame_base,
-                     frame_off, remap);
+                     lds * kDftWaves, st, tasks, starts, args, ntasks, total, slot, wtab,
+                     frame_base, frame_off, remap);
   SHWFS_LAUNCHED("sh_dft_tasks_kernel");
   return SHWFS_OK;
 }
@@ -524,12 +525,12 @@ int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
 
 template <int PREC, int Q>
 int32_t launch_dft_tasks_pq(int nfft, bool full, int nr, bool noisy, const BatchTask* tasks,
-                            const TaskStarts& starts,
+                            const TaskStarts& starts, const TaskArgs& args,
                             int ntasks, int total, int grid, size_t lds,
                             const uint32_t* frame_base, uint32_t frame_off, int remap,
                             hipStream_t st) {
 #define SHWFS_ARGS \
-  full, nr, noisy, tasks, starts, ntasks, total, grid, lds, frame_base, frame_off, remap, st
+  full, nr, noisy, tasks, starts, args, ntasks, total, grid, lds, frame_base, frame_off, remap, st
   if constexpr (Q == 1) {
     switch (nfft) {
       case 16: return launch_dft_tasks_n<16, 1, PREC>(SHWFS_ARGS);
@@ -570,8 +571,8 @@ const void* dft_table_pq(int nfft, hipStream_t st) {
   extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t); \
   extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);                \
   extern template int32_t launch_dft_tasks_pq<PREC, Q>(                                 \
-      int, bool, int, bool, const BatchTask*, const TaskStarts&, int, int, int, size_t, \
-      const uint32_t*, uint32_t, int, hipStream_t);
+      int, bool, int, bool, const BatchTask*, const TaskStarts&, const TaskArgs&, int, int,   \
+      int, size_t, const uint32_t*, uint32_t, int, hipStream_t);
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 1)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_BF16X3, 2)
 SHWFS_DFT_EXTERN(SHWFS_METHOD_DFT_FP16, 1)

@@ -102,6 +102,14 @@ struct TaskStarts {
   int32_t first[kTaskStarts];
 };
 
+// Parameters of the first kTaskArgs tasks as a kernel argument (read through
+// the constant cache with the other arguments instead of from the task table
+// in global memory).  Same-kind batches of at most kTaskArgs tasks use it.
+constexpr int kTaskArgs = 8;
+struct TaskArgs {
+  ImageParams P[kTaskArgs];
+};
+
 #if defined(__HIPCC__)
 __device__ __forceinline__ int find_task(const TaskStarts& ts, const BatchTask* tasks, int ntasks,
                                          int t) {
@@ -111,6 +119,14 @@ __device__ __forceinline__ int find_task(const TaskStarts& ts, const BatchTask* 
   if (ntasks > kTaskStarts && i == kTaskStarts - 1) {  // past the argument table
     while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
   }
+  return i;
+}
+
+// The same for at most kTaskArgs tasks (no table lookup).
+__device__ __forceinline__ int find_task_args(const TaskStarts& ts, int ntasks, int t) {
+  int i = 0;
+#pragma unroll
+  for (int k = 1; k < kTaskArgs; ++k) i += (k < ntasks && ts.first[k] <= t) ? 1 : 0;
   return i;
 }
 #endif
@@ -130,8 +146,8 @@ int32_t launch_background(float* img, long n, float sigma, unsigned long long se
 // class, method, full window, rebin) variant; one single-wave workgroup per
 // sub-aperture of the flattened task range.
 int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, bool noisy,
-                         const BatchTask* tasks, const TaskStarts& starts, int ntasks, int total,
-                         int grid, size_t lds,
+                         const BatchTask* tasks, const TaskStarts& starts, const TaskArgs& args,
+                         int ntasks, int total, int grid, size_t lds,
                          const uint32_t* frame_base, uint32_t frame_off, int remap,
                          hipStream_t st);
 // Device table of the DFT operand fragments for (nfft, pdiam, method), built
