@@ -196,6 +196,12 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
     return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft (enlarged HR field of view) is not implemented");
   if (g->nphase > g->nfft) return fail(SHWFS_ERR_ARG, "nphase must be <= nfft");
   if (g->npix * g->nrebin > g->ntot) return fail(SHWFS_ERR_ARG, "npix*nrebin must be <= ntot");
+  // the kernels index the spot cube, the detector image and the screen with
+  // 32-bit offsets (and 24-bit products)
+  const long long nsub = (long long)g->npix * g->nxsub;
+  if ((long long)g->nvalid * g->npix * g->npix >= (1LL << 30) || nsub * nsub >= (1LL << 30) ||
+      (long long)g->npup * g->npup >= (1LL << 30))
+    return fail(SHWFS_ERR_UNSUPPORTED, "geometry too large (spot cube, image or screen >= 2^30 elements)");
   if (!g->d_istart || !g->d_jstart || !g->d_validsubsx || !g->d_validsubsy || !g->d_offsets ||
       !g->d_fluxPerSub || !g->d_binmap)
     return fail(SHWFS_ERR_STATE, "geometry tables not loaded (load_arrays not called?)");

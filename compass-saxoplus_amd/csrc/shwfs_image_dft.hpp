@@ -313,7 +313,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         b += __int_as_float(dpp_int(__float_as_int(b), 1));  // lane ^ 2
         const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
         if ((lane & 3) == 0 && px < npix && py < npix) {
-          bin[px + py * npix] = b;
+          bin[px + (int)__umul24(py, npix)] = b;
           part += b;
         }
       } else if (nr == 2) {
@@ -323,7 +323,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
           b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
           const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
           if ((lane & 1) == 0 && px < npix && py < npix) {
-            bin[px + py * npix] = b;
+            bin[px + (int)__umul24(py, npix)] = b;
             part += b;
           }
         }
@@ -332,7 +332,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         for (int v = 0; v < 4; ++v) {
           const int px = u, py = (ky0 + v - o) & (N - 1);
           if (px < npix && py < npix) {
-            bin[px + py * npix] = iv[v];
+            bin[px + (int)__umul24(py, npix)] = iv[v];
             part += iv[v];
           }
         }

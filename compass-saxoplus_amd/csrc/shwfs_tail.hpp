@@ -164,11 +164,66 @@ __device__ __forceinline__ void centroid_finish64(const ImageParams& P, int s, b
   }
 }
 
+// a[i] = v through a 32-bit byte offset from a base held in scalar registers
+// (the arrays written here are under 4 GiB: i < 2^30).
+__device__ __forceinline__ void store_at(float* a, unsigned i, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(a) + (i << 2)) = v;
+}
+
 // Row of pixel p in a spot of npix columns: p / npix through the float
 // reciprocal of npix (exact for p < 2^20: (p + 1/2) / npix stays at least
 // 1/(2 npix) away from an integer, far beyond the few-ulp product error).
 __device__ __forceinline__ int pixel_row(int p, float inv_npix) {
   return (int)(((float)p + 0.5f) * inv_npix);
+}
+
+// Moment accumulators of the fused centroider (float or float64 sums).
+struct Moments {
+  float cI = 0.0f, cX = 0.0f, cY = 0.0f;
+  double dI = 0.0, dX = 0.0, dY = 0.0;
+};
+
+// The pixels p = lane, lane + LANES, ... of sub-aperture s: flux
+// normalisation, noise, bincube, moments and binimg.  KIND (centroider) and
+// FP64 (float64 moments) are fixed per instantiation so the loop carries no
+// per-pixel tests of them.
+template <int LANES, bool NOISE, int KIND, bool FP64>
+__device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lane, const float* bin,
+                                            float fnorm, int vx, int vy, unsigned call_add,
+                                            Moments& m) {
+  const int npix = P.npix;
+  const int npix2 = npix * npix;
+  const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
+  for (int p = lane; p < npix2; p += LANES) {
+    float b = bin[p] * fnorm;
+    const unsigned e = (unsigned)(p + s * npix2);
+    if constexpr (NOISE) {
+      const unsigned call = P.call + call_add;
+      if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
+      if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
+    }
+    store_at(P.bincube, e, b);
+    const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
+    float out = b;
+    if constexpr (KIND != SHWFS_CENTROID_NONE) {
+      float d = b;
+      if constexpr (KIND != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
+      if constexpr (KIND == SHWFS_CENTROID_TCOG) out = d;
+      const float fx = (float)px, fy = (float)py;
+      float w = 1.0f;
+      if constexpr (KIND == SHWFS_CENTROID_WCOG) w = P.weights[e];
+      if constexpr (FP64) {
+        m.dI += (double)d * w;
+        m.dX += (double)d * fx * w;
+        m.dY += (double)d * fy * w;
+      } else {
+        m.cI += d * w;
+        m.cX += d * fx * w;
+        m.cY += d * fy * w;
+      }
+    }
+    if (P.binimg) store_at(P.binimg, (unsigned)(vx + px) + __umul24(vy + py, P.nsub), out);
+  }
 }
 
 // Binned spot -> flux normalisation, noise, bincube / binimg, fused
@@ -182,11 +237,7 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
                                             const float* bin, float I, float inten_scale,
                                             unsigned call_add = 0u, void* red = nullptr,
                                             const SubapScalars* pre = nullptr) {
-  const int npix = P.npix;
-  const int npix2 = npix * npix;
-  const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
-  float cI = 0.0f, cX = 0.0f, cY = 0.0f;
-  double dI = 0.0, dX = 0.0, dY = 0.0;
+  Moments m;
   if (valid) {
     const float fl = pre ? pre->fl : P.flux[s];
     const int vx = pre ? pre->vx : P.validsubsx[s];
@@ -194,36 +245,23 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     if (lane == 0) P.intensities[s] = I * inten_scale;
     // photometric normalisation b * flux / I * nphot as one factor
     const float fnorm = (I != 0.0f) ? fl * __builtin_amdgcn_rcpf(I) * P.nphot : 1.0f;
-    for (int p = lane; p < npix2; p += LANES) {
-      float b = bin[p] * fnorm;
-      const unsigned e = (unsigned)(p + s * npix2);
-      if constexpr (NOISE) {
-        const unsigned call = P.call + call_add;
-        if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
-        if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
-      }
-      P.bincube[e] = b;
-      const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
-      float out = b;
-      if (P.ckind != SHWFS_CENTROID_NONE) {
-        float d = b;
-        if (P.ckind != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
-        if (P.ckind == SHWFS_CENTROID_TCOG) out = d;
-        const float fx = (float)px, fy = (float)py;
-        const float w = (P.ckind == SHWFS_CENTROID_WCOG) ? P.weights[e] : 1.0f;
-        if (P.fp64) {
-          dI += (double)d * w;
-          dX += (double)d * fx * w;
-          dY += (double)d * fy * w;
-        } else {
-          cI += d * w;
-          cX += d * fx * w;
-          cY += d * fy * w;
-        }
-      }
-      if (P.binimg) P.binimg[(vx + px) + (size_t)(vy + py) * P.nsub] = out;
+#define SHWFS_TAIL(K, F) tail_pixels<LANES, NOISE, K, F>(P, s, lane, bin, fnorm, vx, vy, call_add, m)
+    switch (P.ckind) {
+      case SHWFS_CENTROID_NONE: SHWFS_TAIL(SHWFS_CENTROID_NONE, false); break;
+      case SHWFS_CENTROID_COG:
+        if (P.fp64) SHWFS_TAIL(SHWFS_CENTROID_COG, true); else SHWFS_TAIL(SHWFS_CENTROID_COG, false);
+        break;
+      case SHWFS_CENTROID_WCOG:
+        if (P.fp64) SHWFS_TAIL(SHWFS_CENTROID_WCOG, true); else SHWFS_TAIL(SHWFS_CENTROID_WCOG, false);
+        break;
+      default:
+        if (P.fp64) SHWFS_TAIL(SHWFS_CENTROID_TCOG, true); else SHWFS_TAIL(SHWFS_CENTROID_TCOG, false);
+        break;
     }
+#undef SHWFS_TAIL
   }
+  double dI = m.dI, dX = m.dX, dY = m.dY;
+  float cI = m.cI, cX = m.cX, cY = m.cY;
   if (P.ckind != SHWFS_CENTROID_NONE) {
     if constexpr (LANES == 64) {
       if (red != nullptr) {
