@@ -183,6 +183,109 @@ __device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x
   return f;
 }
 
+// Stage A's field operand, evaluated in the lanes that hold it (no LDS
+// pass).  The operand is A_E[y][k] with y = 16 ma + (lane & 15) and
+// k = 32 ks + 8 (lane >> 4) + e: k < P16 holds Er at x = k, k >= P16 holds Ei
+// at x = k - P16 (0 outside the pdiam window).  ev[ma][ks][e] receives it.
+//  * Q == 2 (P16 = 32): lane l evaluates x = 8 (l >> 4) + e at both rows
+//    y = (l & 15) and 16 + (l & 15); Er is the ks = 0 operand, Ei ks = 1.
+//  * Q == 1 (P16 = 16): lane l evaluates the four points x = 8 h + 4 u + j
+//    (h = (l >> 4) & 1, u = l >> 5, j < 4) of row y = l & 15; one
+//    v_permlane32_swap per point pair hands lanes < 32 the Er and lanes >= 32
+//    the Ei of x = 8 h .. 8 h + 7.
+// Screen reads: x consecutive within a lane, 16 rows per instruction.  The
+// angle is taken in turns, reduced to [0, 1) by v_fract, and fed to the
+// hardware sine and cosine.
+typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+template <int N, int Q, bool FULL>
+__device__ __forceinline__ void field_operand(const ImageParams& P, int s, int lane,
+                                              float (&ev)[Q][Q][8]) {
+  using L = DftLayout<N, Q>;
+  constexpr float kInv2Pi = 0.15915494309189533577f;
+  constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
+  const int pdiam = FULL ? L::P16 : P.pdiam;
+  const int npup = P.npup;
+  const unsigned win = (unsigned)P.istart[s] + (unsigned)P.jstart[s] * (unsigned)npup;
+  const bool unit = P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
+  const float turn_scale = P.scale * kInv2Pi;
+  const int x0 = Q == 1 ? 8 * ((lane >> 4) & 1) + 4 * (lane >> 5) : 8 * (lane >> 4);
+  float er[Q][NP], ei[Q][NP];
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma) {
+    const int y = 16 * ma + (lane & 15);
+    float ph[NP], of[NP], pu[NP];
+    bool in[NP];
+    if constexpr (FULL) {
+      // the NP points of a lane are consecutive and inside the window: 16-byte
+      // loads (4-byte aligned; gfx950 global loads need dword alignment only)
+      const unsigned g = win + (unsigned)x0 + (unsigned)y * (unsigned)npup;
+#pragma unroll
+      for (int j = 0; j < NP; j += 4) {
+        const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
+            reinterpret_cast<const char*>(P.phase) + ((g + j) << 2));
+        ph[j] = v.x; ph[j + 1] = v.y; ph[j + 2] = v.z; ph[j + 3] = v.w;
+        const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x0 + j + y * pdiam));
+        of[j] = o.x; of[j + 1] = o.y; of[j + 2] = o.z; of[j + 3] = o.w;
+        in[j] = in[j + 1] = in[j + 2] = in[j + 3] = true;
+        pu[j] = pu[j + 1] = pu[j + 2] = pu[j + 3] = 1.0f;
+      }
+      if (!unit) {
+#pragma unroll
+        for (int j = 0; j < NP; j += 4) {
+          const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
+              reinterpret_cast<const char*>(P.pupil) + ((g + j) << 2));
+          pu[j] = v.x; pu[j + 1] = v.y; pu[j + 2] = v.z; pu[j + 3] = v.w;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const int x = x0 + j;
+        in[j] = x < pdiam && y < pdiam;
+        const int xc = in[j] ? x : 0, yc = in[j] ? y : 0;  // clamped into the window
+        const unsigned g = win + (unsigned)xc + (unsigned)yc * (unsigned)npup;
+        ph[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
+        of[j] = P.offsets[(unsigned)(xc + yc * pdiam)];
+        pu[j] = 1.0f;
+      }
+      if (!unit) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+          const int x = x0 + j;
+          const int xc = in[j] ? x : 0, yc = in[j] ? y : 0;
+          const unsigned g = win + (unsigned)xc + (unsigned)yc * (unsigned)npup;
+          pu[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const float r = __builtin_amdgcn_fractf(fmaf(ph[j], turn_scale, -of[j] * kInv2Pi));
+      const float a = in[j] ? pu[j] : 0.0f;
+      er[ma][j] = __builtin_amdgcn_cosf(r) * a;
+      ei[ma][j] = __builtin_amdgcn_sinf(r) * a;
+    }
+  }
+  if constexpr (Q == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(er[0][j]),
+                                                       __float_as_uint(ei[0][j]), false, false);
+      ev[0][0][j] = __uint_as_float(sw[0]);
+      ev[0][0][4 + j] = __uint_as_float(sw[1]);
+    }
+  } else {
+#pragma unroll
+    for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        ev[ma][0][e] = er[ma][e];
+        ev[ma][1][e] = ei[ma][e];
+      }
+  }
+}
+
 // Spot, detector pixels and centroid of sub-aperture s (< nvalid, uniform
 // over the wave).  smem: the wave's LDS slot; bm: the binmap table (LDS or
 // global) used when nr == 0; nr: register rebin factor (dft_register_rebin),
@@ -200,30 +303,9 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   SubapScalars pre;
   pre.load(P, s);
 
-  // ---- field: coalesced window loads (x fastest over lanes) -> Er, Ei planes
-  // in LDS -> stage A operand, A_E[y][k] with y = 16 ma + (lane & 15) and
-  // k = 32 ks + 8 (lane >> 4) + e; k < P16 holds Er at x = k, k >= P16 holds
-  // Ei at x = k - P16 (0 outside the pdiam window)
-  load_fields<64, L::EPL, true, true, FULL ? L::P16 : 0>(P, s, true, lane, smem, L::PE,
-                                                          L::EPLANE);
-  wave_lds_sync();
+  // ---- field: stage A's operand, computed in registers
   float ev[Q][Q][8];
-#pragma unroll
-  for (int ma = 0; ma < Q; ++ma)
-#pragma unroll
-    for (int ks = 0; ks < Q; ++ks) {
-      const int y = 16 * ma + (lane & 15);
-      const int k0 = 32 * ks + 8 * (lane >> 4);
-      const int im = k0 >= L::P16 ? 1 : 0;
-      const int x0 = k0 - im * L::P16;
-      const float* src = smem + im * L::EPLANE + y * L::PE + x0;
-      const float4 a = *reinterpret_cast<const float4*>(src);
-      const float4 b = *reinterpret_cast<const float4*>(src + 4);
-      const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        ev[ma][ks][e] = (FULL || (x0 + e < pdiam && y < pdiam)) ? v[e] : 0.0f;
-    }
+  field_operand<N, Q, FULL>(P, s, lane, ev);
   float escale = 1.0f;
   int escale_exp = 0;  // escale = 2^escale_exp
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
