@@ -310,9 +310,11 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   int escale_exp = 0;  // escale = 2^escale_exp
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
     // bound of max |G|: pdiam * max |E|, with max |E| <= 1 known for a
-    // window of unit pupil, measured over the wave otherwise
-    float amax = (float)pdiam;
-    if (P.pupil_unit == nullptr || P.pupil_unit[s] == 0) {
+    // window of unit pupil, measured over the wave otherwise; the exponent
+    // comes from the float's bits (scalar work: the bound is wave-uniform)
+    if (P.pupil_unit != nullptr && P.pupil_unit[s] != 0) {
+      escale_exp = 14 - (31 - __builtin_clz((unsigned)pdiam));  // 14 - ilogb(pdiam)
+    } else {
       float m = 0.0f;
 #pragma unroll
       for (int ma = 0; ma < Q; ++ma)
@@ -320,10 +322,12 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         for (int ks = 0; ks < Q; ++ks)
 #pragma unroll
           for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[ma][ks][e]));
-      amax = wave_max(m) * (float)pdiam;
+      const float amax = wave_max(m) * (float)pdiam;
+      const int ex = ((__float_as_int(amax) >> 23) & 0xff) - 127;  // ilogb for normal amax
+      // zero or subnormal bound: leave the field unscaled
+      escale_exp = amax >= 1.17549435e-38f ? 14 - ex : 0;
     }
-    if (amax > 0.0f) escale_exp = 14 - ilogbf(amax);
-    escale = ldexpf(1.0f, escale_exp);
+    escale = __int_as_float((127 + escale_exp) << 23);  // 2^escale_exp (|escale_exp| < 127)
   }
   F ae[Q][Q];
 #pragma unroll
@@ -344,7 +348,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
       ga[ma][nt] = acc;
     }
-  const float inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2
+  const float inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2 (may be subnormal)
 
   // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
   // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
