@@ -198,16 +198,18 @@ __device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x
 // hardware sine and cosine.
 typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
+// unit: the window's pupil is all ones (its reads are skipped); amp_unit:
+// the amplitude used for such a window (the operand scale), the pupil value
+// otherwise.
 template <int N, int Q, bool FULL>
-__device__ __forceinline__ void field_operand(const ImageParams& P, int s, int lane,
-                                              float (&ev)[Q][Q][8]) {
+__device__ __forceinline__ void field_operand(const ImageParams& P, int s, int lane, bool unit,
+                                              float amp_unit, float (&ev)[Q][Q][8]) {
   using L = DftLayout<N, Q>;
   constexpr float kInv2Pi = 0.15915494309189533577f;
   constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
   const int pdiam = FULL ? L::P16 : P.pdiam;
   const int npup = P.npup;
   const unsigned win = (unsigned)P.istart[s] + (unsigned)P.jstart[s] * (unsigned)npup;
-  const bool unit = P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
   const float turn_scale = P.scale * kInv2Pi;
   const int x0 = Q == 1 ? 8 * ((lane >> 4) & 1) + 4 * (lane >> 5) : 8 * (lane >> 4);
   float er[Q][NP], ei[Q][NP];
@@ -228,7 +230,7 @@ __device__ __forceinline__ void field_operand(const ImageParams& P, int s, int l
         const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x0 + j + y * pdiam));
         of[j] = o.x; of[j + 1] = o.y; of[j + 2] = o.z; of[j + 3] = o.w;
         in[j] = in[j + 1] = in[j + 2] = in[j + 3] = true;
-        pu[j] = pu[j + 1] = pu[j + 2] = pu[j + 3] = 1.0f;
+        pu[j] = pu[j + 1] = pu[j + 2] = pu[j + 3] = amp_unit;
       }
       if (!unit) {
 #pragma unroll
@@ -247,7 +249,7 @@ __device__ __forceinline__ void field_operand(const ImageParams& P, int s, int l
         const unsigned g = win + (unsigned)xc + (unsigned)yc * (unsigned)npup;
         ph[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
         of[j] = P.offsets[(unsigned)(xc + yc * pdiam)];
-        pu[j] = 1.0f;
+        pu[j] = amp_unit;
       }
       if (!unit) {
 #pragma unroll
@@ -303,17 +305,21 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   SubapScalars pre;
   pre.load(P, s);
 
-  // ---- field: stage A's operand, computed in registers
+  // ---- field: stage A's operand, computed in registers.  FP16: the field is
+  // scaled by escale = 2^escale_exp so that the bound pdiam * max |E| of
+  // max |G| lands in [2^14, 2^15): max |E| <= 1 for a window of unit pupil
+  // (the scale is then applied with the amplitude), measured over the wave
+  // otherwise; the exponents are scalar work (the bound is wave-uniform).
+  const bool unit = P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
+  const int unit_exp =
+      PREC == SHWFS_METHOD_DFT_FP16 ? 14 - (31 - __builtin_clz((unsigned)pdiam)) : 0;
   float ev[Q][Q][8];
-  field_operand<N, Q, FULL>(P, s, lane, ev);
+  field_operand<N, Q, FULL>(P, s, lane, unit, __int_as_float((127 + unit_exp) << 23), ev);
   float escale = 1.0f;
   int escale_exp = 0;  // escale = 2^escale_exp
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
-    // bound of max |G|: pdiam * max |E|, with max |E| <= 1 known for a
-    // window of unit pupil, measured over the wave otherwise; the exponent
-    // comes from the float's bits (scalar work: the bound is wave-uniform)
-    if (P.pupil_unit != nullptr && P.pupil_unit[s] != 0) {
-      escale_exp = 14 - (31 - __builtin_clz((unsigned)pdiam));  // 14 - ilogb(pdiam)
+    if (unit) {
+      escale_exp = unit_exp;
     } else {
       float m = 0.0f;
 #pragma unroll
@@ -326,8 +332,14 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       const int ex = ((__float_as_int(amax) >> 23) & 0xff) - 127;  // ilogb for normal amax
       // zero or subnormal bound: leave the field unscaled
       escale_exp = amax >= 1.17549435e-38f ? 14 - ex : 0;
+      escale = __int_as_float((127 + escale_exp) << 23);  // 2^escale_exp (|escale_exp| < 127)
+#pragma unroll
+      for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+        for (int ks = 0; ks < Q; ++ks)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) ev[ma][ks][e] *= escale;
     }
-    escale = __int_as_float((127 + escale_exp) << 23);  // 2^escale_exp (|escale_exp| < 127)
   }
   F ae[Q][Q];
 #pragma unroll
@@ -335,7 +347,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
 #pragma unroll
     for (int ks = 0; ks < Q; ++ks)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) ae[ma][ks].set(e, ev[ma][ks][e] * escale);
+      for (int e = 0; e < 8; ++e) ae[ma][ks].set(e, ev[ma][ks][e]);
 
   // ---- stage A: G (rows y, cols (re/im, kx)) = E A_W^T
   f32x4 ga[Q][L::MT];
@@ -394,9 +406,14 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         for (int v = 0; v < 4; ++v) hr[kx + (ky0 + v) * N] = iv[v];
       } else if (nr == 4) {
         // the 4 ky of the lane x lanes kx .. kx+3 form one binned pixel
-        float b = (iv[0] + iv[1]) + (iv[2] + iv[3]);
-        b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
-        b += __int_as_float(dpp_int(__float_as_int(b), 1));  // lane ^ 2
+        float b = fb[mt][0] * fb[mt][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (v > 0) b = fmaf(fb[mt][v], fb[mt][v], b);
+          b = fmaf(fb[mt + L::MT / 2][v], fb[mt + L::MT / 2][v], b);
+        }
+        b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
+        b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
         const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
         if ((lane & 3) == 0 && px < npix && py < npix) {
           bin[px + (int)__umul24(py, npix)] = b;
@@ -406,7 +423,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
           float b = iv[2 * g] + iv[2 * g + 1];
-          b += __int_as_float(dpp_int(__float_as_int(b), 0));  // lane ^ 1
+          b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
           const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
           if ((lane & 1) == 0 && px < npix && py < npix) {
             bin[px + (int)__umul24(py, npix)] = b;
