@@ -33,8 +33,26 @@ def test_mfma_probe_matches_matmul(method):
     assert np.array_equal(dD.cpu().numpy(), A @ B)
 
 
+# Shapes that take the other operand and binning variants of the MFMA
+# kernels: 32-point windows (two 16-row operand blocks), windows smaller than
+# the operand tile (masked), N = 64, one-pixel and 2x2 register binning, and
+# the general binmap gather (block origin not on the rebin grid).
+SHAPES = {
+    "q2_n64": dict(diam=8.0, nxsub=8, pdiam=32, npix=16, nfft=64, nrebin=4),
+    "q2_masked": dict(diam=8.0, nxsub=20, pdiam=24, npix=8, nfft=32, nrebin=4),
+    "rebin1": dict(diam=4.0, nxsub=10, pdiam=16, npix=12, nfft=16, nrebin=1),
+    "gather": dict(diam=8.0, nxsub=24, pdiam=16, npix=7, nfft=32, nrebin=4),
+}
+
+
+def _config(name):
+    if name in SHAPES:
+        return configs.sh_config(**SHAPES[name], r0=0.16, seeds=(3,), name=name)
+    return configs.get(name)
+
+
 def _run(name, method):
-    c = configs.get(name)
+    c = _config(name)
     c["p_centroider"].type = "cog"
     s = system.ShSystem(c, method=method)
     phase = system.make_screens(c, 1)[0]
@@ -66,3 +84,13 @@ def test_bf16x3_accuracy(name):
 def test_fp16_accuracy(name):
     spot, flux, rms = _run(name, capi.METHOD_DFT_FP16)
     assert rms < 1e-3 and flux < 1e-2
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+@pytest.mark.parametrize("method", [capi.METHOD_DFT_BF16X3, capi.METHOD_DFT_FP16])
+def test_mfma_variants_accuracy(name, method):
+    spot, flux, rms = _run(name, method)
+    if method == capi.METHOD_DFT_BF16X3:
+        assert spot < 1e-4 and flux < 1e-4 and rms < 1e-3
+    else:
+        assert rms < 1e-3 and flux < 1e-2

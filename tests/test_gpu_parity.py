@@ -47,6 +47,24 @@ def test_spots_match_oracle(name):
     _check_image(s, phase)
 
 
+# Plain-path FFT variants: N = 64, one-pixel binning, the binmap gather, and
+# windows smaller than the FFT (the shapes of the MFMA variant tests).
+FFT_SHAPES = {
+    "n64": dict(diam=8.0, nxsub=8, pdiam=32, npix=16, nfft=64, nrebin=4),
+    "rebin1": dict(diam=4.0, nxsub=10, pdiam=16, npix=12, nfft=16, nrebin=1),
+    "gather": dict(diam=8.0, nxsub=24, pdiam=16, npix=7, nfft=32, nrebin=4),
+    "masked32": dict(diam=8.0, nxsub=20, pdiam=24, npix=8, nfft=32, nrebin=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FFT_SHAPES))
+def test_fft_variants_match_oracle(name):
+    c = configs.sh_config(**FFT_SHAPES[name], r0=0.16, seeds=(3,), name=name)
+    s = system.ShSystem(c)
+    phase = system.make_screens(c, 1)[0]
+    _check_image(s, phase)
+
+
 def test_small_cog_slopes_match_oracle():
     c = configs.get("small")
     s = system.ShSystem(c)
