@@ -48,6 +48,39 @@ def profiled_traffic(method_name):
     return None
 
 
+MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 / fp16
+
+
+def mfma_utilisation(w, results, batched):
+    """Matrix-core use of the DFT methods on the 80x80 frame: useful flops
+    8 (pdiam^2 N + N^2 pdiam) per sub-aperture (the two complex products of the
+    pruned DFT) and issued flops (16x16x32 tiles: Q (N/8) Q for stage A,
+    (N/16) (N/8) Q for stage B, x3 for split-BF16x3), against the dense peak."""
+    n, p, nv = w._Nfft, w._pdiam, w._nvalid
+    q = 1 if p <= 16 else 2
+    tiles = q * (n // 8) * q + (n // 16) * (n // 8) * q
+    useful = 8.0 * (p * p * n + n * n * p) * nv
+    out = {}
+    for m, name, mult in ((2, "dft_fp16", 1), (1, "dft_bf16x3", 3)):
+        issued = tiles * 16384.0 * mult * nv
+        row = {}
+        if m in results:
+            t = results[m]["kernel_us"] * 1e-6
+            row["single_frame"] = dict(useful_tflops=round(useful / t / 1e12, 1),
+                                       issued_tflops=round(issued / t / 1e12, 1))
+        if name in batched:
+            t = batched[name]["us_per_frame"] * 1e-6
+            row["batched"] = dict(useful_tflops=round(useful / t / 1e12, 1),
+                                  issued_tflops=round(issued / t / 1e12, 1))
+        for v in row.values():
+            v["frac_useful"] = round(v["useful_tflops"] / MFMA_PEAK_TFLOPS, 4)
+            v["frac_issued"] = round(v["issued_tflops"] / MFMA_PEAK_TFLOPS, 4)
+        if row:
+            out[name] = row
+    return dict(peak_tflops=MFMA_PEAK_TFLOPS, useful_flops_per_frame=useful,
+                tiles_per_subap=tiles, methods=out) if out else None
+
+
 def sync():
     import carma
     carma.deviceSync()
@@ -252,6 +285,7 @@ def main():
         uneven = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
         print("[bench] uneven 8-WFS batch: %.0f frames/s (%.2f us/frame), methods %s"
               % (r["frames_per_s"], r["us_per_frame"], r["methods"]), file=sys.stderr, flush=True)
+    mfma = mfma_utilisation(w, results, batched)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(rb["system"], phase, thr)
@@ -273,7 +307,7 @@ def main():
                  for k, v in results.items()},
         speedup_vs_plain_fp32=(round(rb["frames_per_s"] / plain["frames_per_s"], 3)
                                if plain else None),
-        roofline=roof, cpu_baseline=cpu,
+        roofline=roof, cpu_baseline=cpu, mfma=mfma,
         batched_80x80=dict(
             workload="%d independent 80x80 frames (own screen and outputs) per launch, "
                      "hipGraph replay" % a.batched, methods=batched,

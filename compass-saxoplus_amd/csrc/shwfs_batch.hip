@@ -134,6 +134,41 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
     lds = std::max(lds, l);
     cost[i] = tile_cost(tasks[i], b->geoms[i].nfft);
   }
+  // same-kind MFMA batch (one kind, rebin factor and window class for all, at
+  // most kTaskArgs WFS: their parameters travel as kernel arguments): tiles of
+  // kDftTaskSubaps sub-apertures, one wave each, kDftWaves waves' LDS slots
+  // per workgroup
+  const int m0 = b->tiles[0].method;
+  bool uni = m0 != SHWFS_METHOD_FFT_FP32 && n <= kTaskArgs;
+  for (int i = 0; i < n && uni; ++i) {
+    uni = tasks[i].kind == tasks[0].kind && tasks[i].nr == tasks[0].nr &&
+          b->geoms[i].nphase == b->geoms[0].nphase;
+  }
+  if (uni) {
+    size_t wlds = 0;
+    for (int i = 0; i < n; ++i) {
+      const int npix = b->geoms[i].npix;
+      const bool gather = tasks[i].nr == 0;
+      int f = 0;
+      switch (tasks[i].kind - (m0 == SHWFS_METHOD_DFT_BF16X3 ? BK_BF16_16 : BK_FP16_16)) {
+        case 0: f = subaps_floats<16, 1>(npix, gather, kDftTaskSubaps); break;
+        case 1: f = subaps_floats<32, 1>(npix, gather, kDftTaskSubaps); break;
+        case 2: f = subaps_floats<64, 1>(npix, gather, kDftTaskSubaps); break;
+        case 3: f = subaps_floats<32, 2>(npix, gather, kDftTaskSubaps); break;
+        default: f = subaps_floats<64, 2>(npix, gather, kDftTaskSubaps); break;
+      }
+      wlds = std::max(wlds, sizeof(float) * (size_t)f);
+    }
+    if (wlds * kDftWaves > max_lds_per_block()) uni = false;
+    if (uni) {
+      lds = wlds;
+      for (int i = 0; i < n; ++i) {
+        tasks[i].per_tile = kDftTaskSubaps;
+        tasks[i].ntiles = (b->geoms[i].nvalid + kDftTaskSubaps - 1) / kDftTaskSubaps;
+        cost[i] = tile_cost(tasks[i], b->geoms[i].nfft);
+      }
+    }
+  }
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "batch: LDS footprint exceeds the per-workgroup limit");
   // heaviest tiles first (they start first and the light ones fill in)
@@ -159,17 +194,7 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   int set = 0;
   for (const auto& t : sorted) set |= batch_kind_set(t.kind);
   b->kind_set = (set == BKS_FFT || set == BKS_BF16 || set == BKS_FP16) ? set : BKS_ALL;
-  // same-kind MFMA batch: one kind, rebin factor and window class for all
   b->uni_method = -1;
-  const int m0 = b->tiles[0].method;
-  bool uni = m0 != SHWFS_METHOD_FFT_FP32;
-  for (int i = 0; i < n && uni; ++i) {
-    uni = tasks[i].kind == tasks[0].kind && tasks[i].nr == tasks[0].nr &&
-          b->geoms[i].nphase == b->geoms[0].nphase;
-  }
-  // the same-kind kernel holds kDftWaves waves' slots per workgroup
-  if (uni && lds * kDftWaves > max_lds_per_block()) uni = false;
-  if (n > kTaskArgs) uni = false;  // task parameters travel as kernel arguments
   if (uni) {
     b->uni_method = m0;
     b->uni_nfft = b->geoms[0].nfft;

@@ -390,8 +390,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Whole-wave totals of three per-lane partial sums, through 192 entries of
 // LDS private to the wave: every lane stores its three partials, lane
-// 16 q + r (q < 3) adds entries 4r .. 4r+3 of quantity q, the rows reduce in
-// registers and the totals are read from lanes 0, 16 and 32.
+// 16 q + r (q < 3) adds entries r, r + 16, r + 32, r + 48 of quantity q in
+// that order, the rows reduce in registers and the totals are read from
+// lanes 0, 16 and 32.  (The order is that of 16 lanes each accumulating the
+// values of lanes r, r + 16, ... in turn, then one row sum.)
 template <typename T>
 __device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
   red[lane] = a;
@@ -400,8 +402,8 @@ __device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
   wave_lds_sync();
   T v = T(0);
   if (lane < 48) {
-    const T* p = red + (lane >> 4) * 64 + 4 * (lane & 15);
-    v = (p[0] + p[1]) + (p[2] + p[3]);
+    const T* p = red + (lane >> 4) * 64 + (lane & 15);
+    v = ((p[0] + p[16]) + p[32]) + p[48];
   }
   v = row_sum(v);
   a = lane_value(v, 0);
@@ -416,10 +418,14 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // Sum over groups of G lanes; G == 64 uses the DPP wave sum.
+// Sum over groups of G lanes: G == 64 uses the DPP wave sum, G == 16 (one
+// row) the DPP row sum, other sizes lane shuffles.
 template <int G, typename T>
 __device__ __forceinline__ T lane_group_sum(T v) {
   if constexpr (G == 64)
     return wave_sum(v);
+  else if constexpr (G == 16)
+    return row_sum(v);
   else
     return group_sum<G>(v);
 }

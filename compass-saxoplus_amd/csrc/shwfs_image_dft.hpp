@@ -288,22 +288,21 @@ __device__ __forceinline__ void field_operand(const ImageParams& P, int s, int l
   }
 }
 
-// Spot, detector pixels and centroid of sub-aperture s (< nvalid, uniform
-// over the wave).  smem: the wave's LDS slot; bm: the binmap table (LDS or
-// global) used when nr == 0; nr: register rebin factor (dft_register_rebin),
-// fixed at compile time by NRC >= 0; FULL: pdiam == P16 (the window fills the
-// operand tiles; no masking).
-template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1>
-__device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
-                                          float* smem, const int* bm, const int nr_rt,
-                                          const AwOperand<N, Q, PREC>& aw,
-                                          unsigned call_add = 0u) {
+// Binned spot of sub-aperture s (< nvalid, uniform over the wave), before
+// the flux normalisation, into bin (npix^2 floats of LDS, x fastest), in units
+// of 1/inten_scale (returned through inten_scale).  hr: N^2 floats of LDS for
+// the |F|^2 image (nr == 0 only); bm: the binmap table (LDS or global) used
+// when nr == 0; nr: register rebin factor (dft_register_rebin), fixed at
+// compile time by NRC >= 0; FULL: pdiam == P16 (the window fills the operand
+// tiles; no masking).
+template <int N, int Q, int PREC, bool FULL, int NRC = -1>
+__device__ __forceinline__ void dft_spot(const ImageParams& P, const int s, const int lane,
+                                         float* hr, float* bin, const int* bm, const int nr_rt,
+                                         const AwOperand<N, Q, PREC>& aw, float& inten_scale) {
   const int nr = NRC >= 0 ? NRC : nr_rt;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
-  SubapScalars pre;
-  pre.load(P, s);
 
   // ---- field: stage A's operand, computed in registers.  FP16: the field is
   // scaled by escale = 2^escale_exp so that the bound pdiam * max |E| of
@@ -360,16 +359,13 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
       ga[ma][nt] = acc;
     }
-  const float inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2 (may be subnormal)
+  inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2 (may be subnormal)
 
   // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
   // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
   // |F|^2 goes to LDS: binned in registers (nr > 0) or as the whole image for
   // the general binmap gather (nr == 0).  The field planes are dead by now.
   const int o = P.bin_origin, npix = P.npix;
-  float* hr = smem;   // nr == 0: hr[kx + ky*N]
-  float* bin = nr > 0 ? smem : smem + (L::wave_floats(npix, true) - ((npix * npix + 3) & ~3));
-  float part = 0.0f;  // this lane's share of the binned flux (nr > 0)
   wave_lds_sync();
 #pragma unroll
   for (int nt = 0; nt < L::NTB; ++nt) {
@@ -415,42 +411,98 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
         const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
-        if ((lane & 3) == 0 && px < npix && py < npix) {
-          bin[px + (int)__umul24(py, npix)] = b;
-          part += b;
-        }
+        if ((lane & 3) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
       } else if (nr == 2) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
           float b = iv[2 * g] + iv[2 * g + 1];
           b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
           const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
-          if ((lane & 1) == 0 && px < npix && py < npix) {
-            bin[px + (int)__umul24(py, npix)] = b;
-            part += b;
-          }
+          if ((lane & 1) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
         }
       } else {  // nr == 1
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int px = u, py = (ky0 + v - o) & (N - 1);
-          if (px < npix && py < npix) {
-            bin[px + (int)__umul24(py, npix)] = iv[v];
-            part += iv[v];
-          }
+          if (px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = iv[v];
         }
       }
     }
   }
-  wave_lds_sync();
-  // moment totals: after the spot, or over the |F|^2 image once it is binned
-  void* red = nr > 0 ? smem + ((npix * npix + 3) & ~3) : smem;
   if (nr == 0) {
-    spot_tail<64, N, NOISE>(P, s, true, lane, hr, bin, bm, inten_scale, call_add, red, &pre);
-  } else {
-    finish_spot<64, NOISE>(P, s, true, lane, bin, wave_sum(part), inten_scale, call_add, red,
-                           &pre);
+    wave_lds_sync();
+    (void)bin_gather<64, N>(P, true, lane, hr, bin, bm);
   }
+}
+
+// Spot, detector pixels and centroid of sub-aperture s by one wave.  smem: the
+// wave's LDS slot (wave_floats): the spot sits at its start (nr > 0) or end
+// (nr == 0), the moment totals use the 192 doubles after the spot (nr > 0)
+// or at the start, over the dead |F|^2 image (nr == 0).
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1>
+__device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
+                                          float* smem, const int* bm, const int nr_rt,
+                                          const AwOperand<N, Q, PREC>& aw,
+                                          unsigned call_add = 0u) {
+  using L = DftLayout<N, Q>;
+  const int nr = NRC >= 0 ? NRC : nr_rt;
+  SubapScalars pre;
+  pre.load(P, s);
+  const int npix = P.npix;
+  const int spot_f = (npix * npix + 3) & ~3;
+  float* bin = nr > 0 ? smem : smem + (L::wave_floats(npix, true) - spot_f);
+  float inten_scale;
+  dft_spot<N, Q, PREC, FULL, NRC>(P, s, lane, smem, bin, bm, nr_rt, aw, inten_scale);
+  wave_lds_sync();
+  const float I = spot_sum<64>(bin, npix * npix, true, lane);
+  void* red = nr > 0 ? smem + spot_f : smem;
+  finish_spot<64, NOISE>(P, s, true, lane, bin, I, inten_scale, call_add, red, &pre);
+  wave_lds_sync();
+}
+
+// LDS floats per wave of dft_subaps: the |F|^2 image (nr == 0), then SPW
+// spots (16-byte aligned), then SPW spot scales.
+template <int N, int Q>
+__host__ __device__ inline int subaps_floats(int npix, bool gather, int spw) {
+  const int spot = (npix * npix + 3) & ~3;
+  return (gather ? N * N : 0) + spw * spot + ((spw + 3) & ~3);
+}
+
+// Sub-apertures s0 .. s0 + SPW - 1 (those < nvalid) by one wave: their spots
+// one after the other, then the tails of all of them at once with 16 lanes per
+// sub-aperture (SPW == 4), so the per-sub-aperture sums, divisions and stores
+// are shared.  smem: the wave's LDS slot (subaps_floats).
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC, int SPW>
+__device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, const int lane,
+                                           float* smem, const int* bm, const int nr_rt,
+                                           const AwOperand<N, Q, PREC>& aw,
+                                           unsigned call_add = 0u) {
+  static_assert(SPW == 4, "dft_subaps: four sub-apertures per wave");
+  const int nr = NRC >= 0 ? NRC : nr_rt;
+  const int npix = P.npix, npix2 = npix * npix;
+  const int spot_f = (npix2 + 3) & ~3;
+  float* hr = smem;
+  float* spots = smem + (nr == 0 ? N * N : 0);
+  float* scales = spots + SPW * spot_f;
+  // this lane's sub-aperture in the tail: q = lane / 16
+  const int q = lane >> 4, r = lane & 15;
+  const int s = s0 + q;
+  const bool valid = s < P.nvalid;
+  SubapScalars pre;
+  pre.load(P, valid ? s : s0);  // per-lane loads, issued ahead of the spots
+#pragma unroll 1
+  for (int k = 0; k < SPW; ++k) {
+    const int sk = s0 + k;
+    if (sk >= P.nvalid) break;  // uniform
+    float inten_scale;
+    dft_spot<N, Q, PREC, FULL, NRC>(P, sk, lane, hr, spots + k * spot_f, bm, nr_rt, aw,
+                                    inten_scale);
+    if (lane == 0) scales[k] = inten_scale;
+  }
+  wave_lds_sync();
+  const float* sp = spots + q * spot_f;
+  const float I = spot_sum<16>(sp, npix2, valid, r);
+  finish_spot<16, NOISE>(P, s, valid, r, sp, I, scales[q], call_add, nullptr, &pre);
   wave_lds_sync();
 }
 
@@ -499,13 +551,16 @@ void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
   dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, NR, aw);
 }
 
+// Sub-apertures per wave of the same-kind batch kernel (its tiles).
+constexpr int kDftTaskSubaps = 4;
+
 // Same-kind batches: every task runs this (N, Q, PREC, FULL, NR) variant.
-// Wave w of workgroup b handles global sub-aperture kDftWaves b' + w of the
-// flattened task range (b' = b, or xcd_remap(b) with remap; task i owns
-// [first_tile, first_tile + ntiles)); spare waves of the last workgroup
-// recompute the last sub-aperture.  slot: LDS floats per wave (the largest
-// slot of the tasks).  At most kTaskArgs tasks: their parameters come with the
-// kernel arguments.
+// Wave w of workgroup b handles global tile kDftWaves b' + w of the flattened
+// task range (b' = b, or xcd_remap(b) with remap; task i owns tiles
+// [first_tile, first_tile + ntiles), kDftTaskSubaps sub-apertures each);
+// spare waves of the last workgroup recompute the last tile.  slot: LDS
+// floats per wave (the largest slot of the tasks).  At most kTaskArgs tasks:
+// their parameters come with the kernel arguments.
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
@@ -524,8 +579,8 @@ void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts s
   const int t = min(bb * kDftWaves + wave, total - 1);
   const int i = find_task_args(starts, ntasks, t);
   const ImageParams& P = args.P[i];
-  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, t - starts.first[i], lane, smem + wave * slot,
-                                         P.binmap, NR, aw, call);
+  dft_subaps<N, Q, PREC, NOISE, FULL, NR, kDftTaskSubaps>(
+      P, (t - starts.first[i]) * kDftTaskSubaps, lane, smem + wave * slot, P.binmap, NR, aw, call);
 }
 
 template <int N, int Q, int PREC>

@@ -212,18 +212,38 @@ __device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lan
       const float fx = (float)px, fy = (float)py;
       float w = 1.0f;
       if constexpr (KIND == SHWFS_CENTROID_WCOG) w = P.weights[e];
-      if constexpr (FP64) {
-        m.dI += (double)d * w;
-        m.dX += (double)d * fx * w;
-        m.dY += (double)d * fy * w;
-      } else {
-        m.cI += d * w;
-        m.cX += d * fx * w;
-        m.cY += d * fy * w;
+      {
+        // each term rounded before it is added (no fused multiply-add): the
+        // sums then depend only on the order of the terms
+#pragma clang fp contract(off)
+        if constexpr (FP64) {
+          m.dI += (double)d * w;
+          m.dX += (double)d * fx * w;
+          m.dY += (double)d * fy * w;
+        } else {
+          m.cI += d * w;
+          m.cX += d * fx * w;
+          m.cY += d * fy * w;
+        }
       }
     }
     if (P.binimg) store_at(P.binimg, (unsigned)(vx + px) + __umul24(vy + py, P.nsub), out);
   }
+}
+
+// Sum of a spot's npix2 pixels in LDS, for one group of G lanes (G = 16 or
+// 64; lane = the lane's index within the group), held by every lane of the
+// group.  Lanes r < 16 of the group add pixels r, r + 16, ... in turn, then
+// one 16-lane row sum: the same float sum for either group size.
+template <int G>
+__device__ __forceinline__ float spot_sum(const float* sp, int npix2, bool valid, int lane) {
+  static_assert(G == 16 || G == 64, "spot_sum: groups of 16 or 64 lanes");
+  float part = 0.0f;
+  if (valid && lane < 16)
+    for (int p = lane; p < npix2; p += 16) part += sp[p];
+  part = row_sum(part);
+  if constexpr (G == 64) part = lane_value(part, 0);
+  return part;
 }
 
 // Binned spot -> flux normalisation, noise, bincube / binimg, fused
@@ -280,15 +300,12 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
   }
 }
 
-// hr: the NF x NF |E|^2 image of this sub-aperture in LDS, flat index u + v*NF,
-// holding inten_scale^-1 times the true intensities (a power of two).
-// bin: npix^2 floats of LDS private to this sub-aperture.
-template <int LANES, int NF, bool NOISE>
-__device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
-                                          const float* hr, float* bin, const int* s_binmap,
-                                          float inten_scale, unsigned call_add = 0u,
-                                          void* red = nullptr,
-                                          const SubapScalars* pre = nullptr) {
+// Binned spot from the NF x NF |E|^2 image hr in LDS (flat index u + v*NF):
+// block form or the general binmap gather; returns this lane's share of the
+// spot's sum.  bin: npix^2 floats of LDS.
+template <int LANES, int NF>
+__device__ __forceinline__ float bin_gather(const ImageParams& P, bool valid, int lane,
+                                            const float* hr, float* bin, const int* s_binmap) {
   const int npix2 = P.npix * P.npix;
   const int nreb2 = P.nrebin * P.nrebin;
   float part = 0.0f;
@@ -318,6 +335,19 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
       }
     }
   }
+  return part;
+}
+
+// hr: the NF x NF |E|^2 image of this sub-aperture in LDS, flat index u + v*NF,
+// holding inten_scale^-1 times the true intensities (a power of two).
+// bin: npix^2 floats of LDS private to this sub-aperture.
+template <int LANES, int NF, bool NOISE>
+__device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
+                                          const float* hr, float* bin, const int* s_binmap,
+                                          float inten_scale, unsigned call_add = 0u,
+                                          void* red = nullptr,
+                                          const SubapScalars* pre = nullptr) {
+  const float part = bin_gather<LANES, NF>(P, valid, lane, hr, bin, s_binmap);
   const float I = lane_group_sum<LANES>(part);
   finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add, red, pre);
 }

@@ -41,6 +41,32 @@ def test_batch_matches_per_wfs_frames(method):
         assert max(errs) <= tol
 
 
+@pytest.mark.parametrize("method", [BF16X3, FP16])
+@pytest.mark.parametrize("centroider", ["cog", "tcog"])
+def test_same_kind_batch_matches_per_wfs_frames(method, centroider):
+    """Three WFS of one geometry (own screens and outputs) run the same-kind
+    kernel (four sub-apertures per wave, one tail for four); the per-WFS path
+    runs one sub-aperture per wave.  Both add the spot and the moments in the
+    same order, so the results are equal bit for bit."""
+    cfgs = [configs.get("noisy") for _ in range(3)]
+    for k, c in enumerate(cfgs):
+        c["p_wfs"].noise = -1.0
+        c["seeds"] = [11 + k]
+        c["p_centroider"].type = centroider
+        c["p_centroider"].thresh = 0.05
+    bs = system.BatchSystem(cfgs, method=method)
+    bs.set_phases([system.make_screens(c, 1)[0] for c in cfgs])
+    bs.batch.set_tiles([method] * len(cfgs), [0] * len(cfgs))
+    bs.batch.launch()
+    got = [(s.bincube(), s.binimg(), bs.slopes(i)) for i, s in enumerate(bs.systems)]
+    for i, s in enumerate(bs.systems):
+        ref = _single(s)
+        errs = [relative_array_error(g, r) for g, r in zip(got[i], ref)]
+        print("wfs %d method %d %s: spot %.1e image %.1e slopes %.1e"
+              % (i, method, centroider, *errs), flush=True)
+        assert errs == [0.0, 0.0, 0.0]
+
+
 def test_batch_mixed_methods_and_persistent_grid():
     cfgs = _batch_cfgs()
     bs = system.BatchSystem(cfgs)
