@@ -198,77 +198,127 @@ __device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x
 // hardware sine and cosine.
 typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// unit: the window's pupil is all ones (its reads are skipped); amp_unit:
-// the amplitude used for such a window (the operand scale), the pupil value
-// otherwise.
+// Screen and pupil values of one window as the lanes of stage A's field
+// operand need them (field_load), and the operand itself (field_eval).
+//  * Q == 2 (P16 = 32): lane l takes x = 8 (l >> 4) + e at the rows
+//    y = (l & 15) and 16 + (l & 15); Er is the ks = 0 operand, Ei ks = 1.
+//  * Q == 1 (P16 = 16): lane l takes the four points x = 8 h + 4 u + j
+//    (h = (l >> 4) & 1, u = l >> 5, j < 4) of row y = l & 15; one
+//    v_permlane32_swap per point pair hands lanes < 32 the Er and lanes >= 32
+//    the Ei of x = 8 h .. 8 h + 7.
+// The operand is A_E[y][k] with y = 16 ma + (lane & 15) and
+// k = 32 ks + 8 (lane >> 4) + e: k < P16 holds Er at x = k, k >= P16 holds Ei
+// at x = k - P16 (0 outside the pdiam window).  Screen reads: x consecutive
+// within a lane, 16 rows per instruction (16-byte loads for full windows).
+template <int Q>
+struct FieldIn {
+  static constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
+  float ph[Q][NP], pu[Q][NP];
+};
+
+template <int N, int Q>
+__device__ __forceinline__ int field_x0(int lane) {
+  return Q == 1 ? 8 * ((lane >> 4) & 1) + 4 * (lane >> 5) : 8 * (lane >> 4);
+}
+
+typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// The offsets (the same for every window) at the lane's points.
 template <int N, int Q, bool FULL>
-__device__ __forceinline__ void field_operand(const ImageParams& P, int s, int lane, bool unit,
-                                              float amp_unit, float (&ev)[Q][Q][8]) {
-  using L = DftLayout<N, Q>;
-  constexpr float kInv2Pi = 0.15915494309189533577f;
-  constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
-  const int pdiam = FULL ? L::P16 : P.pdiam;
-  const int npup = P.npup;
-  const unsigned win = (unsigned)P.istart[s] + (unsigned)P.jstart[s] * (unsigned)npup;
-  const float turn_scale = P.scale * kInv2Pi;
-  const int x0 = Q == 1 ? 8 * ((lane >> 4) & 1) + 4 * (lane >> 5) : 8 * (lane >> 4);
-  float er[Q][NP], ei[Q][NP];
+__device__ __forceinline__ void field_offsets(const ImageParams& P, int lane,
+                                              float (&of)[Q][FieldIn<Q>::NP]) {
+  constexpr int NP = FieldIn<Q>::NP;
+  const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
+  const int x0 = field_x0<N, Q>(lane);
 #pragma unroll
   for (int ma = 0; ma < Q; ++ma) {
     const int y = 16 * ma + (lane & 15);
-    float ph[NP], of[NP], pu[NP];
-    bool in[NP];
+    if constexpr (FULL) {
+#pragma unroll
+      for (int j = 0; j < NP; j += 4) {
+        const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x0 + j + y * pdiam));
+        of[ma][j] = o.x; of[ma][j + 1] = o.y; of[ma][j + 2] = o.z; of[ma][j + 3] = o.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const int x = x0 + j;
+        const bool in = x < pdiam && y < pdiam;
+        of[ma][j] = P.offsets[(unsigned)(in ? x + y * pdiam : 0)];
+      }
+    }
+  }
+}
+
+// Window win (flat screen index of its corner); unit: its pupil is all ones
+// (the pupil reads are skipped); amp_unit: the amplitude then used (the
+// operand scale), the pupil value otherwise.
+template <int N, int Q, bool FULL>
+__device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, bool unit,
+                                           float amp_unit, int lane, FieldIn<Q>& f) {
+  constexpr int NP = FieldIn<Q>::NP;
+  const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
+  const unsigned npup = (unsigned)P.npup;
+  const int x0 = field_x0<N, Q>(lane);
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma) {
+    const int y = 16 * ma + (lane & 15);
     if constexpr (FULL) {
       // the NP points of a lane are consecutive and inside the window: 16-byte
       // loads (4-byte aligned; gfx950 global loads need dword alignment only)
-      const unsigned g = win + (unsigned)x0 + (unsigned)y * (unsigned)npup;
+      const unsigned g = win + (unsigned)x0 + (unsigned)y * npup;
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
         const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
             reinterpret_cast<const char*>(P.phase) + ((g + j) << 2));
-        ph[j] = v.x; ph[j + 1] = v.y; ph[j + 2] = v.z; ph[j + 3] = v.w;
-        const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x0 + j + y * pdiam));
-        of[j] = o.x; of[j + 1] = o.y; of[j + 2] = o.z; of[j + 3] = o.w;
-        in[j] = in[j + 1] = in[j + 2] = in[j + 3] = true;
-        pu[j] = pu[j + 1] = pu[j + 2] = pu[j + 3] = amp_unit;
+        f.ph[ma][j] = v.x; f.ph[ma][j + 1] = v.y; f.ph[ma][j + 2] = v.z; f.ph[ma][j + 3] = v.w;
+        f.pu[ma][j] = f.pu[ma][j + 1] = f.pu[ma][j + 2] = f.pu[ma][j + 3] = amp_unit;
       }
       if (!unit) {
 #pragma unroll
         for (int j = 0; j < NP; j += 4) {
           const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
               reinterpret_cast<const char*>(P.pupil) + ((g + j) << 2));
-          pu[j] = v.x; pu[j + 1] = v.y; pu[j + 2] = v.z; pu[j + 3] = v.w;
+          f.pu[ma][j] = v.x; f.pu[ma][j + 1] = v.y; f.pu[ma][j + 2] = v.z; f.pu[ma][j + 3] = v.w;
         }
       }
     } else {
 #pragma unroll
       for (int j = 0; j < NP; ++j) {
         const int x = x0 + j;
-        in[j] = x < pdiam && y < pdiam;
-        const int xc = in[j] ? x : 0, yc = in[j] ? y : 0;  // clamped into the window
-        const unsigned g = win + (unsigned)xc + (unsigned)yc * (unsigned)npup;
-        ph[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
-        of[j] = P.offsets[(unsigned)(xc + yc * pdiam)];
-        pu[j] = amp_unit;
+        const bool in = x < pdiam && y < pdiam;
+        const unsigned g = win + (in ? (unsigned)x + (unsigned)y * npup : 0u);  // clamped
+        f.ph[ma][j] =
+            *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
+        f.pu[ma][j] = amp_unit;
+        if (!unit)
+          f.pu[ma][j] =
+              *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
+        if (!in) f.pu[ma][j] = 0.0f;
       }
-      if (!unit) {
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-          const int x = x0 + j;
-          const int xc = in[j] ? x : 0, yc = in[j] ? y : 0;
-          const unsigned g = win + (unsigned)xc + (unsigned)yc * (unsigned)npup;
-          pu[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      const float r = __builtin_amdgcn_fractf(fmaf(ph[j], turn_scale, -of[j] * kInv2Pi));
-      const float a = in[j] ? pu[j] : 0.0f;
-      er[ma][j] = __builtin_amdgcn_cosf(r) * a;
-      ei[ma][j] = __builtin_amdgcn_sinf(r) * a;
     }
   }
+}
+
+// Field E = amplitude * exp(i (phase scale - offset)) at the lane's points,
+// as stage A's operand.  The angle is taken in turns, reduced to [0, 1) by
+// v_fract, and fed to the hardware sine and cosine.
+template <int N, int Q>
+__device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const FieldIn<Q>& f,
+                                           const float (&of)[Q][FieldIn<Q>::NP],
+                                           float (&ev)[Q][Q][8]) {
+  constexpr float kInv2Pi = 0.15915494309189533577f;
+  constexpr int NP = FieldIn<Q>::NP;
+  const float turn_scale = P.scale * kInv2Pi;
+  float er[Q][NP], ei[Q][NP];
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const float r = __builtin_amdgcn_fractf(fmaf(f.ph[ma][j], turn_scale, -of[ma][j] * kInv2Pi));
+      er[ma][j] = __builtin_amdgcn_cosf(r) * f.pu[ma][j];
+      ei[ma][j] = __builtin_amdgcn_sinf(r) * f.pu[ma][j];
+    }
   if constexpr (Q == 1) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -288,6 +338,24 @@ __device__ __forceinline__ void field_operand(const ImageParams& P, int s, int l
   }
 }
 
+// Flat screen index of the corner of sub-aperture s's window.
+__device__ __forceinline__ unsigned window_corner(const ImageParams& P, int s) {
+  return (unsigned)P.istart[s] + (unsigned)P.jstart[s] * (unsigned)P.npup;
+}
+
+// Whether sub-aperture s's pupil window is all ones.
+__device__ __forceinline__ bool unit_window(const ImageParams& P, int s) {
+  return P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
+}
+
+// Amplitude of a unit-pupil window: 1, or for FP16 the operand scale
+// 2^(14 - ilogb(pdiam)) (the bound pdiam * max |E| = pdiam of max |G| then
+// lands in [2^14, 2^15)).
+template <int PREC>
+__device__ __forceinline__ int unit_scale_exp(int pdiam) {
+  return PREC == SHWFS_METHOD_DFT_FP16 ? 14 - (31 - __builtin_clz((unsigned)pdiam)) : 0;
+}
+
 // Binned spot of sub-aperture s (< nvalid, uniform over the wave), before
 // the flux normalisation, into bin (npix^2 floats of LDS, x fastest), in units
 // of 1/inten_scale (returned through inten_scale).  hr: N^2 floats of LDS for
@@ -296,9 +364,11 @@ __device__ __forceinline__ void field_operand(const ImageParams& P, int s, int l
 // compile time by NRC >= 0; FULL: pdiam == P16 (the window fills the operand
 // tiles; no masking).
 template <int N, int Q, int PREC, bool FULL, int NRC = -1>
-__device__ __forceinline__ void dft_spot(const ImageParams& P, const int s, const int lane,
-                                         float* hr, float* bin, const int* bm, const int nr_rt,
-                                         const AwOperand<N, Q, PREC>& aw, float& inten_scale) {
+__device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, float* hr,
+                                         float* bin, const int* bm, const int nr_rt,
+                                         const AwOperand<N, Q, PREC>& aw, const FieldIn<Q>& fin,
+                                         const float (&of)[Q][FieldIn<Q>::NP], const bool unit,
+                                         float& inten_scale) {
   const int nr = NRC >= 0 ? NRC : nr_rt;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
@@ -309,11 +379,9 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int s, cons
   // max |G| lands in [2^14, 2^15): max |E| <= 1 for a window of unit pupil
   // (the scale is then applied with the amplitude), measured over the wave
   // otherwise; the exponents are scalar work (the bound is wave-uniform).
-  const bool unit = P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
-  const int unit_exp =
-      PREC == SHWFS_METHOD_DFT_FP16 ? 14 - (31 - __builtin_clz((unsigned)pdiam)) : 0;
+  const int unit_exp = unit_scale_exp<PREC>(pdiam);
   float ev[Q][Q][8];
-  field_operand<N, Q, FULL>(P, s, lane, unit, __int_as_float((127 + unit_exp) << 23), ev);
+  field_eval<N, Q>(P, lane, fin, of, ev);
   float escale = 1.0f;
   int escale_exp = 0;  // escale = 2^escale_exp
   if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
@@ -451,8 +519,15 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   const int npix = P.npix;
   const int spot_f = (npix * npix + 3) & ~3;
   float* bin = nr > 0 ? smem : smem + (L::wave_floats(npix, true) - spot_f);
+  const int pdiam = FULL ? L::P16 : P.pdiam;
+  const bool unit = unit_window(P, s);
+  float of[Q][FieldIn<Q>::NP];
+  field_offsets<N, Q, FULL>(P, lane, of);
+  FieldIn<Q> fin;
+  field_load<N, Q, FULL>(P, window_corner(P, s), unit,
+                         __int_as_float((127 + unit_scale_exp<PREC>(pdiam)) << 23), lane, fin);
   float inten_scale;
-  dft_spot<N, Q, PREC, FULL, NRC>(P, s, lane, smem, bin, bm, nr_rt, aw, inten_scale);
+  dft_spot<N, Q, PREC, FULL, NRC>(P, lane, smem, bin, bm, nr_rt, aw, fin, of, unit, inten_scale);
   wave_lds_sync();
   const float I = spot_sum<64>(bin, npix * npix, true, lane);
   void* red = nr > 0 ? smem + spot_f : smem;
@@ -490,13 +565,31 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
   const bool valid = s < P.nvalid;
   SubapScalars pre;
   pre.load(P, valid ? s : s0);  // per-lane loads, issued ahead of the spots
-#pragma unroll 1
+  // windows and unit flags of the SPW sub-apertures (scalar loads, together);
+  // each window's screen values are loaded while the previous spot is computed
+  using L = DftLayout<N, Q>;
+  const int pdiam = FULL ? L::P16 : P.pdiam;
+  const float amp_unit = __int_as_float((127 + unit_scale_exp<PREC>(pdiam)) << 23);
+  const int last = P.nvalid - 1;
+  unsigned win[SPW];
+  bool unit[SPW];
+#pragma unroll
   for (int k = 0; k < SPW; ++k) {
-    const int sk = s0 + k;
-    if (sk >= P.nvalid) break;  // uniform
+    const int sk = min(s0 + k, last);
+    win[k] = window_corner(P, sk);
+    unit[k] = unit_window(P, sk);
+  }
+  float of[Q][FieldIn<Q>::NP];
+  field_offsets<N, Q, FULL>(P, lane, of);
+  FieldIn<Q> fin[2];
+  field_load<N, Q, FULL>(P, win[0], unit[0], amp_unit, lane, fin[0]);
+#pragma unroll
+  for (int k = 0; k < SPW; ++k) {
+    if (s0 + k > last) break;  // uniform
+    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], unit[k + 1], amp_unit, lane, fin[(k + 1) & 1]);
     float inten_scale;
-    dft_spot<N, Q, PREC, FULL, NRC>(P, sk, lane, hr, spots + k * spot_f, bm, nr_rt, aw,
-                                    inten_scale);
+    dft_spot<N, Q, PREC, FULL, NRC>(P, lane, hr, spots + k * spot_f, bm, nr_rt, aw, fin[k & 1],
+                                    of, unit[k], inten_scale);
     if (lane == 0) scales[k] = inten_scale;
   }
   wave_lds_sync();
