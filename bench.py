@@ -68,10 +68,11 @@ def mfma_utilisation(w, results, batched):
             t = results[m]["kernel_us"] * 1e-6
             row["single_frame"] = dict(useful_tflops=round(useful / t / 1e12, 1),
                                        issued_tflops=round(issued / t / 1e12, 1))
-        if name in batched:
-            t = batched[name]["us_per_frame"] * 1e-6
-            row["batched"] = dict(useful_tflops=round(useful / t / 1e12, 1),
-                                  issued_tflops=round(issued / t / 1e12, 1))
+        for k, per in batched.items():
+            if name in per:
+                t = per[name]["us_per_frame"] * 1e-6
+                row["batched_x%d" % k] = dict(useful_tflops=round(useful / t / 1e12, 1),
+                                             issued_tflops=round(issued / t / 1e12, 1))
         for v in row.values():
             v["frac_useful"] = round(v["useful_tflops"] / MFMA_PEAK_TFLOPS, 4)
             v["frac_issued"] = round(v["issued_tflops"] / MFMA_PEAK_TFLOPS, 4)
@@ -196,8 +197,9 @@ def main():
     ap.add_argument("--methods", default="0,1,2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wg", type=int, default=0, help="spot-kernel workgroup size (0 = default)")
-    ap.add_argument("--batched", type=int, default=8,
-                    help="frames of the 80x80 WFS per launch in the batched form (0 = skip)")
+    ap.add_argument("--batched", type=str, default="8,16",
+                    help="comma-separated frames of the 80x80 WFS per launch in the batched "
+                         "form (0 = skip)")
     ap.add_argument("--no-uneven", action="store_true", help="skip the eight-WFS batch")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (HBM counter calibration)")
@@ -266,18 +268,19 @@ def main():
                                "+ WRITE_SIZE x 1.0 per launch)",
                 kernel="spot+TCoG fused kernel (%s)" % METHOD_NAMES[best],
                 bytes_per_launch=nbytes, avg_launch_us=round(rb["kernel_us"], 3))
-    batched = {}
-    if a.batched > 0:
+    batched = {}  # frames per launch -> method -> result
+    for k in [int(x) for x in a.batched.split(",") if x.strip() and int(x) > 0]:
+        batched[k] = {}
         for m in (0, 2, 1):
             try:
-                r = measure_batched(m, a.batched, phase, thr)
+                r = measure_batched(m, k, phase, thr)
             except RuntimeError as e:
                 print("batched %s unavailable: %s" % (METHOD_NAMES[m], e), file=sys.stderr)
                 continue
-            batched[METHOD_NAMES[m]] = {k: (round(v, 3) if isinstance(v, float) else v)
-                                        for k, v in r.items()}
+            batched[k][METHOD_NAMES[m]] = {kk: (round(v, 3) if isinstance(v, float) else v)
+                                           for kk, v in r.items()}
             print("[bench] batched x%d %s: %.0f frames/s (%.2f us/frame)"
-                  % (a.batched, METHOD_NAMES[m], r["frames_per_s"], r["us_per_frame"]),
+                  % (k, METHOD_NAMES[m], r["frames_per_s"], r["us_per_frame"]),
                   file=sys.stderr, flush=True)
     uneven = None
     if not a.no_uneven:
@@ -309,11 +312,13 @@ def main():
                                if plain else None),
         roofline=roof, cpu_baseline=cpu, mfma=mfma,
         batched_80x80=dict(
-            workload="%d independent 80x80 frames (own screen and outputs) per launch, "
-                     "hipGraph replay" % a.batched, methods=batched,
+            workload="K independent 80x80 frames (own screen and outputs) per launch, "
+                     "hipGraph replay, for K in frames_per_launch",
+            frames_per_launch={str(k): v for k, v in batched.items()},
             speedup_best_vs_plain_fp32_single=(
-                round(max(v["frames_per_s"] for v in batched.values()) / plain["frames_per_s"], 3)
-                if batched and plain else None)) if batched else None,
+                round(max(x["frames_per_s"] for v in batched.values() for x in v.values())
+                      / plain["frames_per_s"], 3)
+                if batched and plain and any(batched.values()) else None)) if batched else None,
         uneven_batch=dict(workload="6x 74x74 (Nfft 16) + 40x40 (Nfft 32) + 8x8 (Nfft 64) SH, "
                                    "one launch per frame, autotuned tiles, 1000 hipGraph frames",
                           **uneven) if uneven else None,

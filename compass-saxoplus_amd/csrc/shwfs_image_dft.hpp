@@ -702,7 +702,7 @@ static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask
   if (total <= 0) return SHWFS_OK;
   if (lds * kDftWaves > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: LDS footprint exceeds the per-workgroup limit");
-  if (ntasks > kTaskArgs) return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: more than 8 tasks");
+  if (ntasks > kTaskArgs) return fail(SHWFS_ERR_UNSUPPORTED, "DFT batch: more than 16 tasks");
   const int slot = (int)(lds / sizeof(float));
   hipLaunchKernelGGL(k, dim3((total + kDftWaves - 1) / kDftWaves), dim3(64 * kDftWaves),
                      lds * kDftWaves, st, tasks, starts, args, ntasks, total, slot, wtab,

@@ -105,7 +105,7 @@ struct TaskStarts {
 // Parameters of the first kTaskArgs tasks as a kernel argument (read through
 // the constant cache with the other arguments instead of from the task table
 // in global memory).  Same-kind batches of at most kTaskArgs tasks use it.
-constexpr int kTaskArgs = 8;
+constexpr int kTaskArgs = 16;
 struct TaskArgs {
   ImageParams P[kTaskArgs];
 };
