@@ -102,17 +102,27 @@ __device__ __forceinline__ float aw_value(int mt, int ks, int lane, int e) {
   return col_im ? c : s;                // [Wi,  Wr]
 }
 
-// The A_W operand as per-lane fragments, built once per device and kept in a
-// small device table.
+// FP16: the A_W table holds 2^t A_W with t = 14 - log2(P16), the operand
+// scale of a unit-pupil field over a full window (bound P16 * max |E| of
+// max |G|, brought to [2^14, 2^15)); the field then enters unscaled.  BF16x3:
+// t = 0.
+template <int Q, int PREC>
+constexpr int table_exp() {
+  return PREC == SHWFS_METHOD_DFT_FP16 ? (Q == 1 ? 10 : 9) : 0;
+}
+
+// The A_W operand (times 2^table_exp) as per-lane fragments, built once per
+// device and kept in a small device table.
 template <int N, int Q, int PREC>
 __global__ void build_wfrag_kernel(Frag<PREC>* out) {
   using L = DftLayout<N, Q>;
+  constexpr float scale = (float)(1 << table_exp<Q, PREC>());
   const int lane = threadIdx.x;
   for (int mt = 0; mt < L::MT; ++mt)
     for (int ks = 0; ks < L::KS; ++ks) {
       Frag<PREC> f;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) f.set(e, aw_value<N, Q>(mt, ks, lane, e));
+      for (int e = 0; e < 8; ++e) f.set(e, aw_value<N, Q>(mt, ks, lane, e) * scale);
       out[(mt * L::KS + ks) * 64 + lane] = f;
     }
 }
@@ -223,7 +233,8 @@ __device__ __forceinline__ int field_x0(int lane) {
 
 typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// The offsets (the same for every window) at the lane's points.
+// The offsets (the same for every window) at the lane's points, as turns:
+// of = -offset / (2 pi).
 template <int N, int Q, bool FULL>
 __device__ __forceinline__ void field_offsets(const ImageParams& P, int lane,
                                               float (&of)[Q][FieldIn<Q>::NP]) {
@@ -247,15 +258,17 @@ __device__ __forceinline__ void field_offsets(const ImageParams& P, int lane,
         of[ma][j] = P.offsets[(unsigned)(in ? x + y * pdiam : 0)];
       }
     }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) of[ma][j] = -of[ma][j] * 0.15915494309189533577f;
   }
 }
 
 // Window win (flat screen index of its corner); unit: its pupil is all ones
-// (the pupil reads are skipped); amp_unit: the amplitude then used (the
-// operand scale), the pupil value otherwise.
+// (the pupil reads are skipped, amplitude 1).  Points outside a window
+// smaller than the operand tile get amplitude 0.
 template <int N, int Q, bool FULL>
 __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, bool unit,
-                                           float amp_unit, int lane, FieldIn<Q>& f) {
+                                           int lane, FieldIn<Q>& f) {
   constexpr int NP = FieldIn<Q>::NP;
   const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
   const unsigned npup = (unsigned)P.npup;
@@ -272,7 +285,7 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
         const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
             reinterpret_cast<const char*>(P.phase) + ((g + j) << 2));
         f.ph[ma][j] = v.x; f.ph[ma][j + 1] = v.y; f.ph[ma][j + 2] = v.z; f.ph[ma][j + 3] = v.w;
-        f.pu[ma][j] = f.pu[ma][j + 1] = f.pu[ma][j + 2] = f.pu[ma][j + 3] = amp_unit;
+        f.pu[ma][j] = f.pu[ma][j + 1] = f.pu[ma][j + 2] = f.pu[ma][j + 3] = 1.0f;
       }
       if (!unit) {
 #pragma unroll
@@ -290,7 +303,7 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
         const unsigned g = win + (in ? (unsigned)x + (unsigned)y * npup : 0u);  // clamped
         f.ph[ma][j] =
             *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
-        f.pu[ma][j] = amp_unit;
+        f.pu[ma][j] = 1.0f;
         if (!unit)
           f.pu[ma][j] =
               *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
@@ -303,7 +316,9 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
 // Field E = amplitude * exp(i (phase scale - offset)) at the lane's points,
 // as stage A's operand.  The angle is taken in turns, reduced to [0, 1) by
 // v_fract, and fed to the hardware sine and cosine.
-template <int N, int Q>
+// UNIT: a unit-pupil full window (amplitude 1 everywhere: no multiply).
+// of: the offsets in turns (field_offsets).
+template <int N, int Q, bool UNIT>
 __device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const FieldIn<Q>& f,
                                            const float (&of)[Q][FieldIn<Q>::NP],
                                            float (&ev)[Q][Q][8]) {
@@ -315,9 +330,13 @@ __device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const
   for (int ma = 0; ma < Q; ++ma)
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-      const float r = __builtin_amdgcn_fractf(fmaf(f.ph[ma][j], turn_scale, -of[ma][j] * kInv2Pi));
-      er[ma][j] = __builtin_amdgcn_cosf(r) * f.pu[ma][j];
-      ei[ma][j] = __builtin_amdgcn_sinf(r) * f.pu[ma][j];
+      const float r = __builtin_amdgcn_fractf(fmaf(f.ph[ma][j], turn_scale, of[ma][j]));
+      er[ma][j] = __builtin_amdgcn_cosf(r);
+      ei[ma][j] = __builtin_amdgcn_sinf(r);
+      if constexpr (!UNIT) {
+        er[ma][j] *= f.pu[ma][j];
+        ei[ma][j] *= f.pu[ma][j];
+      }
     }
   if constexpr (Q == 1) {
 #pragma unroll
@@ -348,13 +367,6 @@ __device__ __forceinline__ bool unit_window(const ImageParams& P, int s) {
   return P.pupil_unit != nullptr && P.pupil_unit[s] != 0;
 }
 
-// Amplitude of a unit-pupil window: 1, or for FP16 the operand scale
-// 2^(14 - ilogb(pdiam)) (the bound pdiam * max |E| = pdiam of max |G| then
-// lands in [2^14, 2^15)).
-template <int PREC>
-__device__ __forceinline__ int unit_scale_exp(int pdiam) {
-  return PREC == SHWFS_METHOD_DFT_FP16 ? 14 - (31 - __builtin_clz((unsigned)pdiam)) : 0;
-}
 
 // Binned spot of sub-aperture s (< nvalid, uniform over the wave), before
 // the flux normalisation, into bin (npix^2 floats of LDS, x fastest), in units
@@ -374,38 +386,39 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
 
-  // ---- field: stage A's operand, computed in registers.  FP16: the field is
-  // scaled by escale = 2^escale_exp so that the bound pdiam * max |E| of
-  // max |G| lands in [2^14, 2^15): max |E| <= 1 for a window of unit pupil
-  // (the scale is then applied with the amplitude), measured over the wave
-  // otherwise; the exponents are scalar work (the bound is wave-uniform).
-  const int unit_exp = unit_scale_exp<PREC>(pdiam);
+  // ---- field: stage A's operand, computed in registers.  FP16: G must land
+  // in [2^14, 2^15) of its bound pdiam * max |E|: the table carries 2^t
+  // (table_exp), which is the whole scale for a unit-pupil window (max |E| =
+  // 1); otherwise max |E| is measured over the wave and the field takes the
+  // rest, 2^(e_total - t) (scalar exponent work: the bound is wave-uniform).
+  constexpr int t = table_exp<Q, PREC>();
+  int e_total = t;  // G carries 2^e_total
   float ev[Q][Q][8];
-  field_eval<N, Q>(P, lane, fin, of, ev);
-  float escale = 1.0f;
-  int escale_exp = 0;  // escale = 2^escale_exp
-  if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
-    if (unit) {
-      escale_exp = unit_exp;
-    } else {
-      float m = 0.0f;
+  if (FULL && unit) {
+    field_eval<N, Q, FULL>(P, lane, fin, of, ev);
+  } else {
+    field_eval<N, Q, false>(P, lane, fin, of, ev);
+    if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
+      if (!unit) {
+        float m = 0.0f;
 #pragma unroll
-      for (int ma = 0; ma < Q; ++ma)
+        for (int ma = 0; ma < Q; ++ma)
 #pragma unroll
-        for (int ks = 0; ks < Q; ++ks)
+          for (int ks = 0; ks < Q; ++ks)
 #pragma unroll
-          for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[ma][ks][e]));
-      const float amax = wave_max(m) * (float)pdiam;
-      const int ex = ((__float_as_int(amax) >> 23) & 0xff) - 127;  // ilogb for normal amax
-      // zero or subnormal bound: leave the field unscaled
-      escale_exp = amax >= 1.17549435e-38f ? 14 - ex : 0;
-      escale = __int_as_float((127 + escale_exp) << 23);  // 2^escale_exp (|escale_exp| < 127)
+            for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[ma][ks][e]));
+        const float amax = wave_max(m) * (float)pdiam;
+        const int ex = ((__float_as_int(amax) >> 23) & 0xff) - 127;  // ilogb for normal amax
+        // zero or subnormal bound: the table's scale alone
+        e_total = amax >= 1.17549435e-38f ? 14 - ex : t;
+        const float fscale = __int_as_float((127 + e_total - t) << 23);  // |e_total - t| < 127
 #pragma unroll
-      for (int ma = 0; ma < Q; ++ma)
+        for (int ma = 0; ma < Q; ++ma)
 #pragma unroll
-        for (int ks = 0; ks < Q; ++ks)
+          for (int ks = 0; ks < Q; ++ks)
 #pragma unroll
-          for (int e = 0; e < 8; ++e) ev[ma][ks][e] *= escale;
+            for (int e = 0; e < 8; ++e) ev[ma][ks][e] *= fscale;
+      }
     }
   }
   F ae[Q][Q];
@@ -427,7 +440,9 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
       ga[ma][nt] = acc;
     }
-  inten_scale = ldexpf(1.0f, -2 * escale_exp);  // 1 / escale^2 (may be subnormal)
+  // F carries 2^(e_total + t) (stage B's table scale too): |F|^2 is scaled
+  // back by 2^(-2 (e_total + t)) (may be subnormal)
+  inten_scale = ldexpf(1.0f, -2 * (e_total + t));
 
   // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
   // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
@@ -524,8 +539,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   float of[Q][FieldIn<Q>::NP];
   field_offsets<N, Q, FULL>(P, lane, of);
   FieldIn<Q> fin;
-  field_load<N, Q, FULL>(P, window_corner(P, s), unit,
-                         __int_as_float((127 + unit_scale_exp<PREC>(pdiam)) << 23), lane, fin);
+  field_load<N, Q, FULL>(P, window_corner(P, s), unit, lane, fin);
   float inten_scale;
   dft_spot<N, Q, PREC, FULL, NRC>(P, lane, smem, bin, bm, nr_rt, aw, fin, of, unit, inten_scale);
   wave_lds_sync();
@@ -569,7 +583,6 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
   // each window's screen values are loaded while the previous spot is computed
   using L = DftLayout<N, Q>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
-  const float amp_unit = __int_as_float((127 + unit_scale_exp<PREC>(pdiam)) << 23);
   const int last = P.nvalid - 1;
   unsigned win[SPW];
   bool unit[SPW];
@@ -582,11 +595,11 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
   float of[Q][FieldIn<Q>::NP];
   field_offsets<N, Q, FULL>(P, lane, of);
   FieldIn<Q> fin[2];
-  field_load<N, Q, FULL>(P, win[0], unit[0], amp_unit, lane, fin[0]);
+  field_load<N, Q, FULL>(P, win[0], unit[0], lane, fin[0]);
 #pragma unroll
   for (int k = 0; k < SPW; ++k) {
     if (s0 + k > last) break;  // uniform
-    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], unit[k + 1], amp_unit, lane, fin[(k + 1) & 1]);
+    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], unit[k + 1], lane, fin[(k + 1) & 1]);
     float inten_scale;
     dft_spot<N, Q, PREC, FULL, NRC>(P, lane, hr, spots + k * spot_f, bm, nr_rt, aw, fin[k & 1],
                                     of, unit[k], inten_scale);
