@@ -109,20 +109,21 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
 
 template <int LANES, typename Acc>
 __device__ __forceinline__ void centroid_finish(const ImageParams& P, int s, bool valid, int lane,
-                                                Acc I, Acc Sx, Acc Sy) {
+                                                Acc I, Acc Sx, Acc Sy, float rx, float ry) {
   I = lane_group_sum<LANES>(I);
   Sx = lane_group_sum<LANES>(Sx);
   Sy = lane_group_sum<LANES>(Sy);
-  if (valid && lane == 0) {
-    const int ix = P.slope_order == 0 ? s : 2 * s;
-    const int iy = P.slope_order == 0 ? s + P.nvalid : 2 * s + 1;
-    const float rx = P.ref ? P.ref[ix] : 0.0f;
-    const float ry = P.ref ? P.ref[iy] : 0.0f;
+  // every lane of the group holds the totals: lane 0 writes the x slope,
+  // lane 1 the y slope (one division for both)
+  if (valid && lane < 2) {
+    const int iy = lane;
+    const int ix = P.slope_order == 0 ? s + iy * P.nvalid : 2 * s + iy;
+    const Acc num = iy ? Sy : Sx;
+    const float r = iy ? ry : rx;
     // As the reference: the division is done in double (1.e-6 is a double
     // literal), the rest in float.  With float64 moments the sums enter exact.
-    P.slopes[ix] = ((float)((double)Sx / ((double)I + 1.e-6)) - P.coffset) * P.cscale - rx;
-    P.slopes[iy] = ((float)((double)Sy / ((double)I + 1.e-6)) - P.coffset) * P.cscale - ry;
-    P.cintens[s] = (float)I;
+    P.slopes[ix] = ((float)((double)num / ((double)I + 1.e-6)) - P.coffset) * P.cscale - r;
+    if (iy == 0) P.cintens[s] = (float)I;
   }
 }
 
@@ -293,10 +294,18 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
         return;
       }
     }
+    float rx = 0.0f, ry = 0.0f;
+    if (pre) {
+      rx = pre->rx;
+      ry = pre->ry;
+    } else if (valid && P.ref && lane < 2) {
+      rx = P.ref[P.slope_order == 0 ? s : 2 * s];
+      ry = P.ref[P.slope_order == 0 ? s + P.nvalid : 2 * s + 1];
+    }
     if (P.fp64)
-      centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY);
+      centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY, rx, ry);
     else
-      centroid_finish<LANES, float>(P, s, valid, lane, cI, cX, cY);
+      centroid_finish<LANES, float>(P, s, valid, lane, cI, cX, cY, rx, ry);
   }
 }
 
