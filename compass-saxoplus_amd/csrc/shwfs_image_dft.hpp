@@ -9,10 +9,11 @@
 // on v_mfma_f32_16x16x32 tiles with FP32 accumulation, one wave per
 // sub-aperture.  A_W's A-operand fragments are, register for register, the
 // B-operand fragments of A_W^T, so one prebuilt table feeds both stages.
-// The field is evaluated with x fastest over the lanes (coalesced window
-// reads) and reaches stage A's operand layout through one LDS pass; stage A's
-// accumulators become stage B's operand in registers, through two cross-row
-// lane swaps (v_permlane32_swap, v_permlane16_swap).
+// The contraction index (re/im, x) of stage A and (re/im, y) of stage B is
+// stored in the order in which an accumulator tile lists its rows (k_slot):
+// stage A's accumulators are then stage B's operand as they stand, and each
+// lane evaluates the field at the points its operand slots hold; no lane
+// exchanges and no LDS pass.
 // Operand precisions:
 //   * SHWFS_METHOD_DFT_BF16X3: every operand x = hi + lo (bf16 each),
 //     product = hi*hi + hi*lo + lo*hi (lo*lo dropped), 3 MFMA per tile;
@@ -86,15 +87,31 @@ __device__ __forceinline__ f32x4 mma(const Frag<SHWFS_METHOD_DFT_BF16X3>& a,
   return mfma_bf16(a.hi, b.hi, c);
 }
 
+// Contraction slot e of K step ks in lane group g (= lane >> 4) of an
+// operand fragment holds the (re/im, coordinate) pair
+//     Q == 1:  im = e >= 4,  c = 4 g + (e & 3)
+//     Q == 2:  im = ks == 1, c = 16 (e >> 2) + 4 g + (e & 3)
+// (coordinate x in stage A, y in stage B).  An accumulator tile gives lane
+// j + 16 g rows 4 g .. 4 g + 3 of its column j, so the Gr and Gi tiles of
+// stage A (Q == 1), or its two row blocks of Gr (then of Gi; Q == 2), fill a
+// lane's slots in this order.  Both operands of a product take the same
+// order, which leaves the sums over K unchanged.
+template <int Q>
+__device__ __forceinline__ int slot_coord(int g, int e) {
+  return 4 * g + (e & 3) + (Q == 1 ? 0 : 16 * (e >> 2));
+}
+template <int Q>
+__device__ __forceinline__ bool slot_im(int ks, int e) {
+  return Q == 1 ? e >= 4 : ks == 1;
+}
+
 // Element e of lane l of the A_W fragment (mt, ks).
 template <int N, int Q>
 __device__ __forceinline__ float aw_value(int mt, int ks, int lane, int e) {
-  constexpr int P16 = 16 * Q;
   const int m = 16 * mt + (lane & 15);
-  const int k = 32 * ks + 8 * (lane >> 4) + e;
-  const bool row_im = m >= N, col_im = k >= P16;
+  const bool row_im = m >= N, col_im = slot_im<Q>(ks, e);
   const int kx = row_im ? m - N : m;
-  const int x = col_im ? k - P16 : k;
+  const int x = slot_coord<Q>(lane >> 4, e);
   const int idx = (kx * x) & (N - 1);
   const float c = kTw<N>.c[idx];   // cos(2 pi idx / N)
   const float s = kTw<N>.s[idx];   // -sin(2 pi idx / N)
@@ -172,63 +189,41 @@ struct AwOperand<N, Q, PREC, false> {
   }
 };
 
-// Two accumulator tiles X, Y of 16 rows each (D layout: lane j + 16 (i/4)
-// holds rows 4 (i/4) .. +3 of column j) -> one B operand of 32 rows
-// [X ; Y] (lane j + 16 (k/8) holds rows 8 (k/8) .. +7 of column j).  With
-// row groups Xg, Yg (g = lane >> 4): permlane32_swap gives
-// X' = [X0 X1 Y0 Y1], Y' = [X2 X3 Y2 Y3]; permlane16_swap then gives
-// X'' = [X0 X2 Y0 Y2], Y'' = [X1 X3 Y1 Y3], i.e. (elements 0-3, 4-7) of
-// group g = (X'', Y'') = rows (8g .. 8g+3, 8g+4 .. 8g+7) of [X ; Y].
+// Stage B's operand from two accumulator tiles of stage A: X's four rows
+// in slots 0-3, Y's in slots 4-7 (slot_coord / slot_im).
 template <int PREC>
-__device__ __forceinline__ Frag<PREC> rows_to_operand(const f32x4& X, const f32x4& Y, float scale) {
+__device__ __forceinline__ Frag<PREC> acc_to_operand(const f32x4& X, const f32x4& Y) {
   Frag<PREC> f;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(X[v]), __float_as_uint(Y[v]),
-                                                    false, false);
-    const auto b = __builtin_amdgcn_permlane16_swap(a[0], a[1], false, false);
-    f.set(v, __uint_as_float(b[0]) * scale);
-    f.set(4 + v, __uint_as_float(b[1]) * scale);
+    f.set(v, X[v]);
+    f.set(4 + v, Y[v]);
   }
   return f;
 }
 
-// Stage A's field operand, evaluated in the lanes that hold it (no LDS
-// pass).  The operand is A_E[y][k] with y = 16 ma + (lane & 15) and
-// k = 32 ks + 8 (lane >> 4) + e: k < P16 holds Er at x = k, k >= P16 holds Ei
-// at x = k - P16 (0 outside the pdiam window).  ev[ma][ks][e] receives it.
-//  * Q == 2 (P16 = 32): lane l evaluates x = 8 (l >> 4) + e at both rows
-//    y = (l & 15) and 16 + (l & 15); Er is the ks = 0 operand, Ei ks = 1.
-//  * Q == 1 (P16 = 16): lane l evaluates the four points x = 8 h + 4 u + j
-//    (h = (l >> 4) & 1, u = l >> 5, j < 4) of row y = l & 15; one
-//    v_permlane32_swap per point pair hands lanes < 32 the Er and lanes >= 32
-//    the Ei of x = 8 h .. 8 h + 7.
-// Screen reads: x consecutive within a lane, 16 rows per instruction.  The
-// angle is taken in turns, reduced to [0, 1) by v_fract, and fed to the
-// hardware sine and cosine.
-typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
-
 // Screen and pupil values of one window as the lanes of stage A's field
-// operand need them (field_load), and the operand itself (field_eval).
-//  * Q == 2 (P16 = 32): lane l takes x = 8 (l >> 4) + e at the rows
-//    y = (l & 15) and 16 + (l & 15); Er is the ks = 0 operand, Ei ks = 1.
-//  * Q == 1 (P16 = 16): lane l takes the four points x = 8 h + 4 u + j
-//    (h = (l >> 4) & 1, u = l >> 5, j < 4) of row y = l & 15; one
-//    v_permlane32_swap per point pair hands lanes < 32 the Er and lanes >= 32
-//    the Ei of x = 8 h .. 8 h + 7.
-// The operand is A_E[y][k] with y = 16 ma + (lane & 15) and
-// k = 32 ks + 8 (lane >> 4) + e: k < P16 holds Er at x = k, k >= P16 holds Ei
-// at x = k - P16 (0 outside the pdiam window).  Screen reads: x consecutive
-// within a lane, 16 rows per instruction (16-byte loads for full windows).
+// operand need them (field_load), and the operand itself (field_eval).  The
+// operand is A_E[y][slot] with y = 16 ma + (lane & 15); lane l evaluates the
+// NP points x = field_px(l, j) of its row(s), which are the coordinates of its
+// operand slots: Er in the re slots, Ei in the im slots (0 outside the pdiam
+// window).
+//  * Q == 1 (P16 = 16): x = 4 g + j (g = l >> 4, j < 4) of row y = l & 15;
+//    slots 0-3 take Er, 4-7 Ei.
+//  * Q == 2 (P16 = 32): x = 4 g + j and 16 + 4 g + j at the rows y = (l & 15)
+//    and 16 + (l & 15); Er is the ks = 0 operand, Ei ks = 1.
+// Screen reads: four consecutive x per lane, 16 rows per instruction (16-byte
+// loads for full windows).
 template <int Q>
 struct FieldIn {
   static constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
   float ph[Q][NP], pu[Q][NP];
 };
 
-template <int N, int Q>
-__device__ __forceinline__ int field_x0(int lane) {
-  return Q == 1 ? 8 * ((lane >> 4) & 1) + 4 * (lane >> 5) : 8 * (lane >> 4);
+// x of point j of the lane (the coordinate of operand slot j).
+template <int Q>
+__device__ __forceinline__ int field_px(int lane, int j) {
+  return slot_coord<Q>(lane >> 4, j);
 }
 
 typedef float f32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
@@ -240,20 +235,20 @@ __device__ __forceinline__ void field_offsets(const ImageParams& P, int lane,
                                               float (&of)[Q][FieldIn<Q>::NP]) {
   constexpr int NP = FieldIn<Q>::NP;
   const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
-  const int x0 = field_x0<N, Q>(lane);
 #pragma unroll
   for (int ma = 0; ma < Q; ++ma) {
     const int y = 16 * ma + (lane & 15);
     if constexpr (FULL) {
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
-        const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x0 + j + y * pdiam));
+        const int x = field_px<Q>(lane, j);  // x .. x + 3 are points j .. j + 3
+        const f32x4a4 o = *reinterpret_cast<const f32x4a4*>(P.offsets + (x + y * pdiam));
         of[ma][j] = o.x; of[ma][j + 1] = o.y; of[ma][j + 2] = o.z; of[ma][j + 3] = o.w;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < NP; ++j) {
-        const int x = x0 + j;
+        const int x = field_px<Q>(lane, j);
         const bool in = x < pdiam && y < pdiam;
         of[ma][j] = P.offsets[(unsigned)(in ? x + y * pdiam : 0)];
       }
@@ -272,33 +267,34 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
   constexpr int NP = FieldIn<Q>::NP;
   const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
   const unsigned npup = (unsigned)P.npup;
-  const int x0 = field_x0<N, Q>(lane);
 #pragma unroll
   for (int ma = 0; ma < Q; ++ma) {
     const int y = 16 * ma + (lane & 15);
     if constexpr (FULL) {
-      // the NP points of a lane are consecutive and inside the window: 16-byte
+      // points j .. j + 3 are consecutive and inside the window: 16-byte
       // loads (4-byte aligned; gfx950 global loads need dword alignment only)
-      const unsigned g = win + (unsigned)x0 + (unsigned)y * npup;
+      const unsigned g = win + (unsigned)y * npup;
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
+        const unsigned gj = g + (unsigned)field_px<Q>(lane, j);
         const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
-            reinterpret_cast<const char*>(P.phase) + ((g + j) << 2));
+            reinterpret_cast<const char*>(P.phase) + (gj << 2));
         f.ph[ma][j] = v.x; f.ph[ma][j + 1] = v.y; f.ph[ma][j + 2] = v.z; f.ph[ma][j + 3] = v.w;
         f.pu[ma][j] = f.pu[ma][j + 1] = f.pu[ma][j + 2] = f.pu[ma][j + 3] = 1.0f;
       }
       if (!unit) {
 #pragma unroll
         for (int j = 0; j < NP; j += 4) {
+          const unsigned gj = g + (unsigned)field_px<Q>(lane, j);
           const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
-              reinterpret_cast<const char*>(P.pupil) + ((g + j) << 2));
+              reinterpret_cast<const char*>(P.pupil) + (gj << 2));
           f.pu[ma][j] = v.x; f.pu[ma][j + 1] = v.y; f.pu[ma][j + 2] = v.z; f.pu[ma][j + 3] = v.w;
         }
       }
     } else {
 #pragma unroll
       for (int j = 0; j < NP; ++j) {
-        const int x = x0 + j;
+        const int x = field_px<Q>(lane, j);
         const bool in = x < pdiam && y < pdiam;
         const unsigned g = win + (in ? (unsigned)x + (unsigned)y * npup : 0u);  // clamped
         f.ph[ma][j] =
@@ -341,10 +337,8 @@ __device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const
   if constexpr (Q == 1) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(er[0][j]),
-                                                       __float_as_uint(ei[0][j]), false, false);
-      ev[0][0][j] = __uint_as_float(sw[0]);
-      ev[0][0][4 + j] = __uint_as_float(sw[1]);
+      ev[0][0][j] = er[0][j];
+      ev[0][0][4 + j] = ei[0][j];
     }
   } else {
 #pragma unroll
@@ -444,8 +438,9 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
   // back by 2^(-2 (e_total + t)) (may be subnormal)
   inten_scale = ldexpf(1.0f, -2 * (e_total + t));
 
-  // ---- stage B per kx tile: F = A_W [Gr ; Gi]; the 32 K rows of step ks are
-  // (re/im, y) rows 32 ks .. 32 ks + 31, halves of 16 from two ga tiles.
+  // ---- stage B per kx tile: F = A_W [Gr ; Gi]; K step ks takes the Gr and
+  // Gi tiles of the kx block (Q == 1), or the two row blocks of Gr (ks = 0)
+  // or Gi (ks = 1) (Q == 2), in slot order (slot_coord).
   // |F|^2 goes to LDS: binned in registers (nr > 0) or as the whole image for
   // the general binmap gather (nr == 0).  The field planes are dead by now.
   const int o = P.bin_origin, npix = P.npix;
@@ -456,10 +451,10 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
 #pragma unroll
     for (int ks = 0; ks < L::KS; ++ks) {
       constexpr int H = L::MT / 2;  // im column tiles start at H
-      const int r0 = 32 * ks, r1 = 32 * ks + 16;
-      const f32x4& X = ga[(r0 % L::P16) / 16][(r0 / L::P16) * H + nt];
-      const f32x4& Y = ga[(r1 % L::P16) / 16][(r1 / L::P16) * H + nt];
-      bg[ks] = rows_to_operand<PREC>(X, Y, 1.0f);
+      if constexpr (Q == 1)
+        bg[ks] = acc_to_operand<PREC>(ga[0][nt], ga[0][H + nt]);
+      else
+        bg[ks] = acc_to_operand<PREC>(ga[0][ks * H + nt], ga[Q - 1][ks * H + nt]);
     }
     f32x4 fb[L::MT];
 #pragma unroll
