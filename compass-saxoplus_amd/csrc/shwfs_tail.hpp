@@ -184,39 +184,31 @@ struct Moments {
   double dI = 0.0, dX = 0.0, dY = 0.0;
 };
 
-// The pixels p = lane, lane + LANES, ... of sub-aperture s: flux
-// normalisation, noise, bincube, moments and binimg.  KIND (centroider) and
-// FP64 (float64 moments) are fixed per instantiation so the loop carries no
-// per-pixel tests of them.
-template <int LANES, bool NOISE, int KIND, bool FP64>
-__device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lane, const float* bin,
-                                            float fnorm, int vx, int vy, unsigned call_add,
-                                            Moments& m) {
-  const int npix = P.npix;
-  const int npix2 = npix * npix;
-  const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
-  for (int p = lane; p < npix2; p += LANES) {
-    float b = bin[p] * fnorm;
-    const unsigned e = (unsigned)(p + s * npix2);
-    if constexpr (NOISE) {
-      const unsigned call = P.call + call_add;
-      if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
-      if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
-    }
-    store_at(P.bincube, e, b);
-    const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
-    float out = b;
-    if constexpr (KIND != SHWFS_CENTROID_NONE) {
-      float d = b;
-      if constexpr (KIND != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
-      if constexpr (KIND == SHWFS_CENTROID_TCOG) out = d;
-      const float fx = (float)px, fy = (float)py;
-      float w = 1.0f;
-      if constexpr (KIND == SHWFS_CENTROID_WCOG) w = P.weights[e];
-      {
-        // each term rounded before it is added (no fused multiply-add): the
-        // sums then depend only on the order of the terms
+// One detector pixel (px, py) of sub-aperture s, flux-normalised value b,
+// flat bincube index e, detector image index img: noise, bincube, moments
+// and binimg.  KIND (centroider) and FP64 (float64 moments) are fixed per
+// instantiation so the pixel loops carry no per-pixel tests of them.
+template <bool NOISE, int KIND, bool FP64>
+__device__ __forceinline__ void tail_pixel(const ImageParams& P, unsigned e, float b, float fx,
+                                           float fy, unsigned img, unsigned call_add,
+                                           Moments& m) {
+  if constexpr (NOISE) {
+    const unsigned call = P.call + call_add;
+    if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
+    if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
+  }
+  store_at(P.bincube, e, b);
+  float out = b;
+  if constexpr (KIND != SHWFS_CENTROID_NONE) {
+    float d = b;
+    if constexpr (KIND != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
+    if constexpr (KIND == SHWFS_CENTROID_TCOG) out = d;
+    {
+      // each term rounded before it is added (no fused multiply-add): the
+      // sums then depend only on the order of the terms
 #pragma clang fp contract(off)
+      if constexpr (KIND == SHWFS_CENTROID_WCOG) {
+        const float w = P.weights[e];
         if constexpr (FP64) {
           m.dI += (double)d * w;
           m.dX += (double)d * fx * w;
@@ -226,9 +218,53 @@ __device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lan
           m.cX += d * fx * w;
           m.cY += d * fy * w;
         }
+      } else if constexpr (FP64) {
+        // d * fx is exact in double (a float times an integer below 2^29),
+        // so the fused form rounds once, as the product-then-sum does
+        const double dd = d;
+        m.dI += dd;
+        m.dX = fma(dd, (double)fx, m.dX);
+        m.dY = fma(dd, (double)fy, m.dY);
+      } else {
+        m.cI += d;
+        m.cX += d * fx;
+        m.cY += d * fy;
       }
     }
-    if (P.binimg) store_at(P.binimg, (unsigned)(vx + px) + __umul24(vy + py, P.nsub), out);
+  }
+  if (P.binimg) store_at(P.binimg, img, out);
+}
+
+// The pixels p = lane, lane + LANES, ... of sub-aperture s (tail_pixel), in
+// that order.  NPIX > 0: npix == NPIX is known and, LANES being a multiple
+// of it, the lane's pixel column is fixed and its row advances by
+// LANES / NPIX from one pixel to the next (no per-pixel index work).
+template <int LANES, bool NOISE, int KIND, bool FP64, int NPIX = 0>
+__device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lane, const float* bin,
+                                            float fnorm, int vx, int vy, unsigned call_add,
+                                            Moments& m) {
+  if constexpr (NPIX > 0 && LANES % NPIX == 0) {
+    constexpr int NP2 = NPIX * NPIX, DY = LANES / NPIX;
+    const int px = lane % NPIX, py0 = lane / NPIX;
+    const unsigned e0 = (unsigned)(lane + s * NP2);
+    const unsigned img0 = (unsigned)(vx + px) + __umul24(vy + py0, P.nsub);
+    const unsigned dimg = __umul24(DY, P.nsub);
+#pragma unroll
+    for (int j = 0; j * LANES < NP2; ++j)
+      if (lane + j * LANES < NP2)
+        tail_pixel<NOISE, KIND, FP64>(P, e0 + j * LANES, bin[lane + j * LANES] * fnorm,
+                                      (float)px, (float)(py0 + j * DY), img0 + j * dimg,
+                                      call_add, m);
+  } else {
+    const int npix = P.npix;
+    const int npix2 = npix * npix;
+    const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
+    for (int p = lane; p < npix2; p += LANES) {
+      const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
+      tail_pixel<NOISE, KIND, FP64>(P, (unsigned)(p + s * npix2), bin[p] * fnorm, (float)px,
+                                    (float)py, (unsigned)(vx + px) + __umul24(vy + py, P.nsub),
+                                    call_add, m);
+    }
   }
 }
 
@@ -266,7 +302,12 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     if (lane == 0) P.intensities[s] = I * inten_scale;
     // photometric normalisation b * flux / I * nphot as one factor
     const float fnorm = (I != 0.0f) ? fl * __builtin_amdgcn_rcpf(I) * P.nphot : 1.0f;
-#define SHWFS_TAIL(K, F) tail_pixels<LANES, NOISE, K, F>(P, s, lane, bin, fnorm, vx, vy, call_add, m)
+    // noise-free 8x8-pixel spots (the usual SH detector) take the fixed-size
+    // form; with noise the samplers dominate the pixel work
+#define SHWFS_TAIL(K, F)                                                                      \
+  ((!NOISE && P.npix == 8)                                                                    \
+       ? tail_pixels<LANES, NOISE, K, F, NOISE ? 0 : 8>(P, s, lane, bin, fnorm, vx, vy, call_add, m) \
+       : tail_pixels<LANES, NOISE, K, F>(P, s, lane, bin, fnorm, vx, vy, call_add, m))
     switch (P.ckind) {
       case SHWFS_CENTROID_NONE: SHWFS_TAIL(SHWFS_CENTROID_NONE, false); break;
       case SHWFS_CENTROID_COG:
