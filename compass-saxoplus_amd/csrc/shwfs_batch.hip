@@ -184,8 +184,7 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   }
   if (b->d_tasks == nullptr) SHWFS_HIP(hipMalloc((void**)&b->d_tasks, sizeof(BatchTask) * n));
   SHWFS_HIP(hipMemcpy(b->d_tasks, sorted.data(), sizeof(BatchTask) * n, hipMemcpyHostToDevice));
-  b->starts = TaskStarts{};
-  for (int k = 0; k < n && k < kTaskStarts; ++k) b->starts.first[k] = sorted[k].first_tile;
+  for (int k = 0; k < kTaskStarts; ++k) b->starts.first[k] = k < n ? sorted[k].first_tile : kNoTask;
   b->args = TaskArgs{};
   for (int k = 0; k < n && k < kTaskArgs; ++k) b->args.P[k] = sorted[k].P;
   b->ntasks = n;

@@ -217,7 +217,7 @@ __device__ __forceinline__ Frag<PREC> acc_to_operand(const f32x4& X, const f32x4
 template <int Q>
 struct FieldIn {
   static constexpr int NP = Q == 1 ? 4 : 8;  // points per lane and row
-  float ph[Q][NP], pu[Q][NP];
+  float ph[Q][NP];
 };
 
 // x of point j of the lane (the coordinate of operand slot j).
@@ -258,12 +258,13 @@ __device__ __forceinline__ void field_offsets(const ImageParams& P, int lane,
   }
 }
 
-// Window win (flat screen index of its corner); unit: its pupil is all ones
-// (the pupil reads are skipped, amplitude 1).  Points outside a window
-// smaller than the operand tile get amplitude 0.
+// Screen values at the lane's points of window win (flat screen index of its
+// corner); points outside a window smaller than the operand tile read the
+// corner (their amplitude is 0).  Row offsets y * npup through the 24-bit
+// multiply (y < 32, npup < 2^15).
 template <int N, int Q, bool FULL>
-__device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, bool unit,
-                                           int lane, FieldIn<Q>& f) {
+__device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, int lane,
+                                           FieldIn<Q>& f) {
   constexpr int NP = FieldIn<Q>::NP;
   const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
   const unsigned npup = (unsigned)P.npup;
@@ -273,22 +274,50 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
     if constexpr (FULL) {
       // points j .. j + 3 are consecutive and inside the window: 16-byte
       // loads (4-byte aligned; gfx950 global loads need dword alignment only)
-      const unsigned g = win + (unsigned)y * npup;
+      const unsigned g = win + __umul24((unsigned)y, npup);
 #pragma unroll
       for (int j = 0; j < NP; j += 4) {
         const unsigned gj = g + (unsigned)field_px<Q>(lane, j);
         const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
             reinterpret_cast<const char*>(P.phase) + (gj << 2));
         f.ph[ma][j] = v.x; f.ph[ma][j + 1] = v.y; f.ph[ma][j + 2] = v.z; f.ph[ma][j + 3] = v.w;
-        f.pu[ma][j] = f.pu[ma][j + 1] = f.pu[ma][j + 2] = f.pu[ma][j + 3] = 1.0f;
       }
-      if (!unit) {
+    } else {
 #pragma unroll
-        for (int j = 0; j < NP; j += 4) {
+      for (int j = 0; j < NP; ++j) {
+        const int x = field_px<Q>(lane, j);
+        const bool in = x < pdiam && y < pdiam;
+        const unsigned g = win + (in ? (unsigned)x + __umul24((unsigned)y, npup) : 0u);  // clamped
+        f.ph[ma][j] =
+            *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
+      }
+    }
+  }
+}
+
+// Amplitude at the lane's points of window win: the pupil; 1 for a window
+// whose pupil is all ones (unit; nothing read); 0 outside a window smaller
+// than the operand tile.  Read where it is used (few windows are not unit).
+template <int N, int Q, bool FULL>
+__device__ __forceinline__ void field_amplitude(const ImageParams& P, unsigned win, bool unit,
+                                                int lane, float (&pu)[Q][FieldIn<Q>::NP]) {
+  constexpr int NP = FieldIn<Q>::NP;
+  const int pdiam = FULL ? DftLayout<N, Q>::P16 : P.pdiam;
+  const unsigned npup = (unsigned)P.npup;
+#pragma unroll
+  for (int ma = 0; ma < Q; ++ma) {
+    const int y = 16 * ma + (lane & 15);
+    if constexpr (FULL) {
+      const unsigned g = win + __umul24((unsigned)y, npup);
+#pragma unroll
+      for (int j = 0; j < NP; j += 4) {
+        if (unit) {
+          pu[ma][j] = pu[ma][j + 1] = pu[ma][j + 2] = pu[ma][j + 3] = 1.0f;
+        } else {
           const unsigned gj = g + (unsigned)field_px<Q>(lane, j);
           const f32x4a4 v = *reinterpret_cast<const f32x4a4*>(
               reinterpret_cast<const char*>(P.pupil) + (gj << 2));
-          f.pu[ma][j] = v.x; f.pu[ma][j + 1] = v.y; f.pu[ma][j + 2] = v.z; f.pu[ma][j + 3] = v.w;
+          pu[ma][j] = v.x; pu[ma][j + 1] = v.y; pu[ma][j + 2] = v.z; pu[ma][j + 3] = v.w;
         }
       }
     } else {
@@ -296,14 +325,11 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
       for (int j = 0; j < NP; ++j) {
         const int x = field_px<Q>(lane, j);
         const bool in = x < pdiam && y < pdiam;
-        const unsigned g = win + (in ? (unsigned)x + (unsigned)y * npup : 0u);  // clamped
-        f.ph[ma][j] =
-            *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
-        f.pu[ma][j] = 1.0f;
+        const unsigned g = win + (in ? (unsigned)x + __umul24((unsigned)y, npup) : 0u);
+        float a = 1.0f;
         if (!unit)
-          f.pu[ma][j] =
-              *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
-        if (!in) f.pu[ma][j] = 0.0f;
+          a = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
+        pu[ma][j] = in ? a : 0.0f;
       }
     }
   }
@@ -312,11 +338,13 @@ __device__ __forceinline__ void field_load(const ImageParams& P, unsigned win, b
 // Field E = amplitude * exp(i (phase scale - offset)) at the lane's points,
 // as stage A's operand.  The angle is taken in turns, reduced to [0, 1) by
 // v_fract, and fed to the hardware sine and cosine.
-// UNIT: a unit-pupil full window (amplitude 1 everywhere: no multiply).
-// of: the offsets in turns (field_offsets).
+// UNIT: a unit-pupil full window (amplitude 1 everywhere: pu is not read).
+// of: the offsets in turns (field_offsets); pu: the amplitude
+// (field_amplitude).
 template <int N, int Q, bool UNIT>
 __device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const FieldIn<Q>& f,
                                            const float (&of)[Q][FieldIn<Q>::NP],
+                                           const float (&pu)[Q][FieldIn<Q>::NP],
                                            float (&ev)[Q][Q][8]) {
   constexpr float kInv2Pi = 0.15915494309189533577f;
   constexpr int NP = FieldIn<Q>::NP;
@@ -330,8 +358,8 @@ __device__ __forceinline__ void field_eval(const ImageParams& P, int lane, const
       er[ma][j] = __builtin_amdgcn_cosf(r);
       ei[ma][j] = __builtin_amdgcn_sinf(r);
       if constexpr (!UNIT) {
-        er[ma][j] *= f.pu[ma][j];
-        ei[ma][j] *= f.pu[ma][j];
+        er[ma][j] *= pu[ma][j];
+        ei[ma][j] *= pu[ma][j];
       }
     }
   if constexpr (Q == 1) {
@@ -373,8 +401,8 @@ template <int N, int Q, int PREC, bool FULL, int NRC = -1>
 __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, float* hr,
                                          float* bin, const int* bm, const int nr_rt,
                                          const AwOperand<N, Q, PREC>& aw, const FieldIn<Q>& fin,
-                                         const float (&of)[Q][FieldIn<Q>::NP], const bool unit,
-                                         float& inten_scale) {
+                                         const float (&of)[Q][FieldIn<Q>::NP], const unsigned win,
+                                         const bool unit, float& inten_scale) {
   const int nr = NRC >= 0 ? NRC : nr_rt;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
@@ -389,9 +417,11 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
   int e_total = t;  // G carries 2^e_total
   float ev[Q][Q][8];
   if (FULL && unit) {
-    field_eval<N, Q, FULL>(P, lane, fin, of, ev);
+    field_eval<N, Q, FULL>(P, lane, fin, of, of, ev);
   } else {
-    field_eval<N, Q, false>(P, lane, fin, of, ev);
+    float pu[Q][FieldIn<Q>::NP];
+    field_amplitude<N, Q, FULL>(P, win, unit, lane, pu);
+    field_eval<N, Q, false>(P, lane, fin, of, pu, ev);
     if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
       if (!unit) {
         float m = 0.0f;
@@ -533,10 +563,12 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   const bool unit = unit_window(P, s);
   float of[Q][FieldIn<Q>::NP];
   field_offsets<N, Q, FULL>(P, lane, of);
+  const unsigned win = window_corner(P, s);
   FieldIn<Q> fin;
-  field_load<N, Q, FULL>(P, window_corner(P, s), unit, lane, fin);
+  field_load<N, Q, FULL>(P, win, lane, fin);
   float inten_scale;
-  dft_spot<N, Q, PREC, FULL, NRC>(P, lane, smem, bin, bm, nr_rt, aw, fin, of, unit, inten_scale);
+  dft_spot<N, Q, PREC, FULL, NRC>(P, lane, smem, bin, bm, nr_rt, aw, fin, of, win, unit,
+                                  inten_scale);
   wave_lds_sync();
   const float I = spot_sum<64>(bin, npix * npix, true, lane);
   void* red = nr > 0 ? smem + spot_f : smem;
@@ -590,14 +622,14 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
   float of[Q][FieldIn<Q>::NP];
   field_offsets<N, Q, FULL>(P, lane, of);
   FieldIn<Q> fin[2];
-  field_load<N, Q, FULL>(P, win[0], unit[0], lane, fin[0]);
+  field_load<N, Q, FULL>(P, win[0], lane, fin[0]);
 #pragma unroll
   for (int k = 0; k < SPW; ++k) {
     if (s0 + k > last) break;  // uniform
-    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], unit[k + 1], lane, fin[(k + 1) & 1]);
+    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], lane, fin[(k + 1) & 1]);
     float inten_scale;
     dft_spot<N, Q, PREC, FULL, NRC>(P, lane, hr, spots + k * spot_f, bm, nr_rt, aw, fin[k & 1],
-                                    of, unit[k], inten_scale);
+                                    of, win[k], unit[k], inten_scale);
     if (lane == 0) scales[k] = inten_scale;
   }
   wave_lds_sync();

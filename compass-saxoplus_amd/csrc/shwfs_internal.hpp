@@ -94,10 +94,11 @@ struct BatchTask {
 };
 
 // First tile of each task (ascending), passed by value as a kernel argument:
-// a workgroup finds its task with scalar compares instead of a chain of
+// a workgroup finds its task with scalar arithmetic instead of a chain of
 // dependent loads from the task table; tasks past kTaskStarts are found in
-// the table.
+// the table.  Entries past the last task hold kNoTask.
 constexpr int kTaskStarts = 16;
+constexpr int32_t kNoTask = 0x7fffffff;
 struct TaskStarts {
   int32_t first[kTaskStarts];
 };
@@ -111,11 +112,29 @@ struct TaskArgs {
 };
 
 #if defined(__HIPCC__)
+// Sign bit of a wave-uniform x, computed in a scalar register (written as a
+// scalar shift so that the compiler does not turn it into a per-lane compare
+// and select).
+__device__ __forceinline__ int uniform_sign_bit(int x) {
+  int r;
+  asm("s_lshr_b32 %0, %1, 31" : "=s"(r) : "s"(x));
+  return r;
+}
+
+// Number of entries first[1 .. K-1] <= t (t >= 0, wave-uniform):
+// t - first[k] >= 0 exactly when its sign bit is clear, and kNoTask entries
+// never count.
+template <int K>
+__device__ __forceinline__ int count_starts_le(const TaskStarts& ts, int t) {
+  int n = K - 1;
+#pragma unroll
+  for (int k = 1; k < K; ++k) n -= uniform_sign_bit(t - ts.first[k]);
+  return n;
+}
+
 __device__ __forceinline__ int find_task(const TaskStarts& ts, const BatchTask* tasks, int ntasks,
                                          int t) {
-  int i = 0;
-#pragma unroll
-  for (int k = 1; k < kTaskStarts; ++k) i += (k < ntasks && ts.first[k] <= t) ? 1 : 0;
+  int i = count_starts_le<kTaskStarts>(ts, t);
   if (ntasks > kTaskStarts && i == kTaskStarts - 1) {  // past the argument table
     while (i + 1 < ntasks && tasks[i + 1].first_tile <= t) ++i;
   }
@@ -124,10 +143,8 @@ __device__ __forceinline__ int find_task(const TaskStarts& ts, const BatchTask* 
 
 // The same for at most kTaskArgs tasks (no table lookup).
 __device__ __forceinline__ int find_task_args(const TaskStarts& ts, int ntasks, int t) {
-  int i = 0;
-#pragma unroll
-  for (int k = 1; k < kTaskArgs; ++k) i += (k < ntasks && ts.first[k] <= t) ? 1 : 0;
-  return i;
+  (void)ntasks;  // entries past the last task hold kNoTask
+  return count_starts_le<kTaskArgs>(ts, t);
 }
 #endif
 

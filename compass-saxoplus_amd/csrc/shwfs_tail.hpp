@@ -276,8 +276,14 @@ template <int G>
 __device__ __forceinline__ float spot_sum(const float* sp, int npix2, bool valid, int lane) {
   static_assert(G == 16 || G == 64, "spot_sum: groups of 16 or 64 lanes");
   float part = 0.0f;
-  if (valid && lane < 16)
-    for (int p = lane; p < npix2; p += 16) part += sp[p];
+  if (valid && lane < 16) {
+    if (npix2 == 64) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) part += sp[lane + 16 * j];  // 8x8 spots, same order
+    } else {
+      for (int p = lane; p < npix2; p += 16) part += sp[p];
+    }
+  }
   part = row_sum(part);
   if constexpr (G == 64) part = lane_value(part, 0);
   return part;
