@@ -199,6 +199,10 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
     b->uni_nfft = b->geoms[0].nfft;
     b->uni_pdiam = b->geoms[0].nphase;
     b->uni_nr = tasks[0].nr;
+    // the half-grid 4x4 form when every WFS of the batch allows it
+    bool half = true;
+    for (int i = 0; i < n; ++i) half = half && dft_half_grid(tasks[i].P, b->uni_nfft);
+    if (half) b->uni_nr |= kHalfGrid;
     b->uni_full = b->uni_pdiam == 16 || b->uni_pdiam == 32;
   }
   return SHWFS_OK;

@@ -35,6 +35,10 @@
 
 namespace shwfs {
 
+// Bit of a compile-time rebin code (rebin factor in bits 0-2) selecting the
+// half-grid 4x4 binning form (dft_half_grid).
+constexpr int kHalfGrid = 8;
+
 template <int N, int Q>
 struct DftLayout {
   static constexpr int P16 = 16 * Q;             // padded phase side (pdiam <= P16 <= N)
@@ -395,7 +399,8 @@ __device__ __forceinline__ bool unit_window(const ImageParams& P, int s) {
 // of 1/inten_scale (returned through inten_scale).  hr: N^2 floats of LDS for
 // the |F|^2 image (nr == 0 only); bm: the binmap table (LDS or global) used
 // when nr == 0; nr: register rebin factor (dft_register_rebin), fixed at
-// compile time by NRC >= 0; FULL: pdiam == P16 (the window fills the operand
+// compile time by NRC >= 0 (NRC & 7; bit kHalfGrid: the half-grid 4x4 form,
+// see dft_half_grid); FULL: pdiam == P16 (the window fills the operand
 // tiles; no masking).
 template <int N, int Q, int PREC, bool FULL, int NRC = -1>
 __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, float* hr,
@@ -403,7 +408,8 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
                                          const AwOperand<N, Q, PREC>& aw, const FieldIn<Q>& fin,
                                          const float (&of)[Q][FieldIn<Q>::NP], const unsigned win,
                                          const bool unit, float& inten_scale) {
-  const int nr = NRC >= 0 ? NRC : nr_rt;
+  const int nr = NRC >= 0 ? (NRC & 7) : nr_rt;
+  constexpr bool HALF = NRC >= 0 && (NRC & kHalfGrid) != 0;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
@@ -518,8 +524,21 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
         }
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
-        const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
-        if ((lane & 3) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
+        if constexpr (HALF) {
+          // the binned cell (cx, cy) = (kx / 4, ky / 4) is pixel (cx ^ H, cy ^ H):
+          // a per-lane base plus a constant offset per tile pair
+          constexpr int G = N / 4, H = G / 2;
+          const int a = (lane & 15) >> 2, c = lane >> 4;  // < 4
+          int off;
+          if constexpr (G >= 8)  // H >= 4: the XOR leaves a and c alone
+            off = (a + c * G) + (((4 * nt) ^ H) + ((4 * mt) ^ H) * G);
+          else
+            off = ((4 * nt + a) ^ H) + ((4 * mt + c) ^ H) * G;
+          if ((lane & 3) == 0) bin[off] = b;
+        } else {
+          const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
+          if ((lane & 3) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
+        }
       } else if (nr == 2) {
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
@@ -553,7 +572,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
                                           const AwOperand<N, Q, PREC>& aw,
                                           unsigned call_add = 0u) {
   using L = DftLayout<N, Q>;
-  const int nr = NRC >= 0 ? NRC : nr_rt;
+  const int nr = NRC >= 0 ? (NRC & 7) : nr_rt;
   SubapScalars pre;
   pre.load(P, s);
   const int npix = P.npix;
@@ -594,7 +613,7 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
                                            const AwOperand<N, Q, PREC>& aw,
                                            unsigned call_add = 0u) {
   static_assert(SPW == 4, "dft_subaps: four sub-apertures per wave");
-  const int nr = NRC >= 0 ? NRC : nr_rt;
+  const int nr = NRC >= 0 ? (NRC & 7) : nr_rt;
   const int npix = P.npix, npix2 = npix * npix;
   const int spot_f = (npix2 + 3) & ~3;
   float* hr = smem;
@@ -647,7 +666,7 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
 template <int PREC, bool NOISE, int N, bool FULL, int NR>
 constexpr int dft_waves_per_eu() {
   if (NOISE || N == 64) return 4;
-  return (PREC == SHWFS_METHOD_DFT_FP16 && FULL && NR == 4) ? 7 : 5;
+  return (PREC == SHWFS_METHOD_DFT_FP16 && FULL && (NR & 7) == 4) ? 7 : 5;
 }
 
 // Waves per workgroup of the spot kernels.  The waves are independent (one
@@ -667,7 +686,7 @@ template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
 void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
-  constexpr int nr = NR;
+  constexpr int nr = NR & 7;
   using L = DftLayout<N, Q>;
   extern __shared__ float smem[];
   const int lane = (int)threadIdx.x & 63;
@@ -681,7 +700,7 @@ void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
   }
   const AwOperand<N, Q, PREC> aw(wtab, lane);
   const int s = min(xcd_remap(blockIdx.x, gridDim.x) * kDftWaves + wave, P.nvalid - 1);
-  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, NR, aw);
+  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, nr, aw);
 }
 
 // Sub-apertures per wave of the same-kind batch kernel (its tiles).
@@ -713,7 +732,8 @@ void sh_dft_tasks_kernel(const BatchTask* __restrict__ tasks, const TaskStarts s
   const int i = find_task_args(starts, ntasks, t);
   const ImageParams& P = args.P[i];
   dft_subaps<N, Q, PREC, NOISE, FULL, NR, kDftTaskSubaps>(
-      P, (t - starts.first[i]) * kDftTaskSubaps, lane, smem + wave * slot, P.binmap, NR, aw, call);
+      P, (t - starts.first[i]) * kDftTaskSubaps, lane, smem + wave * slot, P.binmap, NR & 7, aw,
+      call);
 }
 
 template <int N, int Q, int PREC>
@@ -727,6 +747,7 @@ static int32_t launch_dft_tasks_n(bool full, int nr, bool noisy, const BatchTask
   decltype(&sh_dft_tasks_kernel<N, Q, PREC, false, true, 4>) k = nullptr;
 #define SHWFS_PICK(NZ, FL)                                                             \
   switch (nr) {                                                                        \
+    case 4 | kHalfGrid: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 4 | kHalfGrid>; break; \
     case 4: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 4>; break;                     \
     case 2: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 2>; break;                     \
     case 1: k = sh_dft_tasks_kernel<N, Q, PREC, NZ, FL, 1>; break;                     \
@@ -759,6 +780,14 @@ __host__ __device__ inline int dft_register_rebin(const ImageParams& p) {
   return (blk && (p.nrebin == 1 || p.nrebin == 2 || p.nrebin == 4)) ? p.nrebin : 0;
 }
 
+// 4x4 register binning of a spot that spans the whole transform (npix =
+// nfft / 4) with the fftshift of a natural guide star (block origin nfft / 2):
+// binned cell (c_x, c_y) = (k_x / 4, k_y / 4) is pixel (c_x, c_y) XOR nfft / 8,
+// so each lane's pixel per tile is fixed at compile time.
+inline bool dft_half_grid(const ImageParams& p, int nfft) {
+  return dft_register_rebin(p) == 4 && p.npix * 4 == nfft && p.bin_origin * 2 == nfft;
+}
+
 template <int N, int Q, int PREC, int NR>
 static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
   using L = DftLayout<N, Q>;
@@ -786,7 +815,9 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
 template <int N, int Q, int PREC>
 static int32_t launch_dft_n(const ImageParams& p, hipStream_t st) {
   switch (dft_register_rebin(p)) {
-    case 4: return launch_dft_t<N, Q, PREC, 4>(p, st);
+    case 4:
+      return dft_half_grid(p, N) ? launch_dft_t<N, Q, PREC, 4 | kHalfGrid>(p, st)
+                                 : launch_dft_t<N, Q, PREC, 4>(p, st);
     case 2: return launch_dft_t<N, Q, PREC, 2>(p, st);
     case 1: return launch_dft_t<N, Q, PREC, 1>(p, st);
     default: return launch_dft_t<N, Q, PREC, 0>(p, st);

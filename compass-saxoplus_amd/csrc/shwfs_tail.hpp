@@ -192,6 +192,10 @@ template <bool NOISE, int KIND, bool FP64>
 __device__ __forceinline__ void tail_pixel(const ImageParams& P, unsigned e, float b, float fx,
                                            float fy, unsigned img, unsigned call_add,
                                            Moments& m) {
+  // no fused multiply-add anywhere in the pixel: the thresholded value is
+  // taken from the stored (rounded) pixel, and the moment sums depend only on
+  // the order of their terms
+#pragma clang fp contract(off)
   if constexpr (NOISE) {
     const unsigned call = P.call + call_add;
     if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
@@ -204,9 +208,6 @@ __device__ __forceinline__ void tail_pixel(const ImageParams& P, unsigned e, flo
     if constexpr (KIND != SHWFS_CENTROID_COG) d = (b > P.threshold) ? b - P.threshold : 1.e-6f;
     if constexpr (KIND == SHWFS_CENTROID_TCOG) out = d;
     {
-      // each term rounded before it is added (no fused multiply-add): the
-      // sums then depend only on the order of the terms
-#pragma clang fp contract(off)
       if constexpr (KIND == SHWFS_CENTROID_WCOG) {
         const float w = P.weights[e];
         if constexpr (FP64) {
