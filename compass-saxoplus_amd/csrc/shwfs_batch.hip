@@ -101,15 +101,16 @@ int32_t fill_task(shwfs_batch* b, int i, const shwfs_batch_tile& tile, hipStream
   } else {
     if (!dft_kind(nfft, pdiam, tile.method, &t.kind))
       return fail(SHWFS_ERR_UNSUPPORTED, "batch: DFT stage needs nfft in {16, 32, 64}, pdiam <= 32");
-    t.per_tile = 1;
+    t.per_tile = kDftTaskSubaps;
     t.nr = dft_register_rebin(t.P);
+    const bool gather = t.nr == 0;
     int wf = 0;
     switch (t.kind - (tile.method == SHWFS_METHOD_DFT_BF16X3 ? BK_BF16_16 : BK_FP16_16)) {
-      case 0: wf = DftLayout<16, 1>::wave_floats(npix, t.nr == 0); break;
-      case 1: wf = DftLayout<32, 1>::wave_floats(npix, t.nr == 0); break;
-      case 2: wf = DftLayout<64, 1>::wave_floats(npix, t.nr == 0); break;
-      case 3: wf = DftLayout<32, 2>::wave_floats(npix, t.nr == 0); break;
-      default: wf = DftLayout<64, 2>::wave_floats(npix, t.nr == 0); break;
+      case 0: wf = subaps_floats<16, 1>(npix, gather, kDftTaskSubaps); break;
+      case 1: wf = subaps_floats<32, 1>(npix, gather, kDftTaskSubaps); break;
+      case 2: wf = subaps_floats<64, 1>(npix, gather, kDftTaskSubaps); break;
+      case 3: wf = subaps_floats<32, 2>(npix, gather, kDftTaskSubaps); break;
+      default: wf = subaps_floats<64, 2>(npix, gather, kDftTaskSubaps); break;
     }
     *lds = sizeof(float) * (size_t)wf;
     t.wtab = dft_operand_table(nfft, pdiam, tile.method, st);

@@ -2,7 +2,8 @@
 // size.  The work list is a table of tasks (one per WFS) in device memory;
 // task i covers the global tiles [first_tile, first_tile + ntiles); a tile is
 // one single-wave workgroup's worth of work: 64/nfft sub-apertures for the
-// FFT stages, one sub-aperture for the MFMA DFT stages.  Workgroup b handles
+// FFT stages, kDftTaskSubaps sub-apertures for the MFMA DFT stages (one
+// shared tail).  Workgroup b handles
 // tiles b, b + grid, ... (grid = number of tiles: the dispatcher balances the
 // workgroups as they drain; smaller grids make the launch persistent).
 //
@@ -44,19 +45,21 @@ __device__ __noinline__ void batch_fft(const BatchTask* task, int s0, unsigned c
   fft_tile<N, R, 64, NOISE>(T->P, s0, smem, T->P.binmap, call);
 }
 
+// A DFT tile: kDftTaskSubaps sub-apertures from s0 by one wave (spots one
+// after the other, one shared tail), rebin factor read from the task.
 template <int N, int Q, int PREC, bool NOISE>
-__device__ __noinline__ void batch_dft(const BatchTask* task, int s, unsigned call) {
+__device__ __noinline__ void batch_dft(const BatchTask* task, int s0, unsigned call) {
   extern __shared__ float smem[];
   const BatchTask* T = uniform_task(task);
-  s = __builtin_amdgcn_readfirstlane(s);
+  s0 = __builtin_amdgcn_readfirstlane(s0);
   call = __builtin_amdgcn_readfirstlane(call);
   const AwOperand<N, Q, PREC> aw(static_cast<const Frag<PREC>*>(T->wtab), (int)threadIdx.x);
   if (T->P.pdiam == DftLayout<N, Q>::P16)
-    dft_subap<N, Q, PREC, NOISE, true>(T->P, s, (int)threadIdx.x, smem, T->P.binmap, T->nr, aw,
-                                       call);
+    dft_subaps<N, Q, PREC, NOISE, true, -1, kDftTaskSubaps>(T->P, s0, (int)threadIdx.x, smem,
+                                                            T->P.binmap, T->nr, aw, call);
   else
-    dft_subap<N, Q, PREC, NOISE, false>(T->P, s, (int)threadIdx.x, smem, T->P.binmap, T->nr, aw,
-                                        call);
+    dft_subaps<N, Q, PREC, NOISE, false, -1, kDftTaskSubaps>(T->P, s0, (int)threadIdx.x, smem,
+                                                             T->P.binmap, T->nr, aw, call);
 }
 
 // Kind sets compiled into one batch kernel: the host launches the smallest

@@ -508,7 +508,7 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
-        iv[v] = fr * fr + fi * fi;
+        iv[v] = fmaf(fi, fi, fr * fr);  // one rounding order in every instantiation
       }
       const int ky0 = 16 * mt + 4 * (lane >> 4);  // this lane's 4 consecutive ky
       if (nr == 0) {
