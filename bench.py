@@ -141,14 +141,15 @@ def measure_batched(method, k, phase, thr, graphs=10, frames_per_graph=20):
                 grid=b.grid, remap=b.remap, splits=b.splits[:1])
 
 
-def measure_uneven(graphs=10, frames_per_graph=100):
+def measure_uneven(method_mask=3, graphs=10, frames_per_graph=100):
     """The eight uneven SH WFS of the batching workload, one launch per frame,
-    tiles from the autotuner (FP32-accurate methods), 1000 hipGraph frames."""
+    tiles from the autotuner over the methods of method_mask (3: the
+    FP32-accurate FFT and BF16x3; 7: FP16 too), 1000 hipGraph frames."""
     from shwfs import configs, system
     cfgs = configs.batch()
     bs = system.BatchSystem(cfgs)
     bs.set_phases([system.make_screens(c, 1)[0] for c in cfgs])
-    us = bs.batch.autotune(method_mask=3, reps=20)
+    us = bs.batch.autotune(method_mask=method_mask, reps=20)
     bs.batch.capture(frames_per_graph)
     bs.batch.replay(1)
     sync()
@@ -284,10 +285,13 @@ def main():
                   file=sys.stderr, flush=True)
     uneven = None
     if not a.no_uneven:
-        r = measure_uneven()
-        uneven = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
-        print("[bench] uneven 8-WFS batch: %.0f frames/s (%.2f us/frame), methods %s"
-              % (r["frames_per_s"], r["us_per_frame"], r["methods"]), file=sys.stderr, flush=True)
+        uneven = {}
+        for mask, key in ((3, "fp32_accurate"), (7, "any_method")):
+            r = measure_uneven(mask)
+            uneven[key] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+            print("[bench] uneven 8-WFS batch (%s): %.0f frames/s (%.2f us/frame), methods %s"
+                  % (key, r["frames_per_s"], r["us_per_frame"], r["methods"]),
+                  file=sys.stderr, flush=True)
     mfma = mfma_utilisation(w, results, batched)
     cpu = None
     if not a.no_cpu:
@@ -320,7 +324,8 @@ def main():
                       / plain["frames_per_s"], 3)
                 if batched and plain and any(batched.values()) else None)) if batched else None,
         uneven_batch=dict(workload="6x 74x74 (Nfft 16) + 40x40 (Nfft 32) + 8x8 (Nfft 64) SH, "
-                                   "one launch per frame, autotuned tiles, 1000 hipGraph frames",
+                                   "one launch per frame, autotuned tiles, 1000 hipGraph frames; "
+                                   "fp32_accurate: methods FFT/BF16x3, any_method: FP16 too",
                           **uneven) if uneven else None,
     )
     print(json.dumps(line), flush=True)

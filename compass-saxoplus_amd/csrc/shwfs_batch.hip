@@ -447,6 +447,34 @@ int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps,
   float best = 0.f;
   int32_t rc = time_config(b, reps, st, &best);
   if (rc) return rc;
+  // every WFS on one method (where its geometry allows): one kind set, so the
+  // batch kernel carries the registers of that set only; changing one WFS at
+  // a time from a uniform start mixes the sets and would not reach these
+  {
+    const std::vector<shwfs_batch_tile> start = b->tiles;
+    std::vector<shwfs_batch_tile> keep = start;
+    for (int m = 0; m < 3; ++m) {
+      if (!((method_mask >> m) & 1u)) continue;
+      b->tiles = start;
+      bool changed = false;
+      for (int i = 0; i < n; ++i) {
+        const shwfs_sh_geom& g = b->geoms[i];
+        int k;
+        const bool ok = check_geom(&g, m) == SHWFS_OK &&
+                        (m == SHWFS_METHOD_FFT_FP32 || dft_kind(g.nfft, g.nphase, m, &k));
+        if (ok && b->tiles[i].method != m) {
+          b->tiles[i] = shwfs_batch_tile{m, 0};
+          changed = true;
+        }
+      }
+      float us = 0.f;
+      if (changed && time_config(b, reps, st, &us) == SHWFS_OK && us < best) {
+        best = us;
+        keep = b->tiles;
+      }
+    }
+    b->tiles = keep;
+  }
   // coordinate descent over the WFS, heaviest first
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
