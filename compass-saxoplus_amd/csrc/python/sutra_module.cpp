@@ -7,6 +7,7 @@
 #include <pybind11/stl.h>
 
 #include <complex>
+#include <type_traits>
 
 #include "../sutra/sutra.hpp"
 
@@ -256,8 +257,12 @@ static void declare_sensors(py::module& m) {
            py::arg("nwfs"), py::arg("field_stop"), py::arg("N"));
 }
 
-static void declare_centroiders(py::module& m) {
-  py::class_<Centro>(m, "Centroider_FF")
+// Centroider classes for images of Tin (float: _FF, uint16 camera frames: _UF;
+// the uint16 image is calibrated into the float image by calibrate_img).
+template <typename Tin>
+static void declare_centroiders(py::module& m, const std::string& sfx) {
+  using Centro = SutraCentroider<Tin, float>;
+  py::class_<Centro>(m, ("Centroider" + sfx).c_str())
       .def_property_readonly("device", [](Centro& c) { return c.device; })
       .def_property_readonly("type", [](Centro& c) { return c.get_type(); })
       .def_property_readonly("nslopes", [](Centro& c) { return c.nslopes; })
@@ -288,19 +293,19 @@ static void declare_centroiders(py::module& m) {
       .def("set_fp64_moments", &Centro::set_fp64_moments, py::arg("value"),
            "sum flux and first moments in float64")
       .def("load_img",
-           [](Centro& c, ArrayFStyle<float>& img, int32_t m_, int32_t n, int32_t loc) {
+           [](Centro& c, ArrayFStyle<Tin>& img, int32_t m_, int32_t n, int32_t loc) {
              if (img.size() != (py::ssize_t)m_ * n) throw std::runtime_error("load_img: wrong size");
              return c.load_img(img.mutable_data(), m_, n, loc);
            },
            py::arg("img"), py::arg("m"), py::arg("n"), py::arg("location"))
       .def("load_img",
-           [](Centro& c, ArrayFStyle<float>& img, int32_t n, int32_t loc) {
+           [](Centro& c, ArrayFStyle<Tin>& img, int32_t n, int32_t loc) {
              if (img.size() != (py::ssize_t)n * n) throw std::runtime_error("load_img: wrong size");
              return c.load_img(img.mutable_data(), n, loc);
            },
            py::arg("img"), py::arg("n"), py::arg("location") = -1)
       .def("load_img",
-           [](Centro& c, CarmaObj<float>* img) { return c.load_img(img); }, py::arg("img"))
+           [](Centro& c, CarmaObj<Tin>* img) { return c.load_img(img); }, py::arg("img"))
       .def("calibrate_img", [](Centro& c) { return c.calibrate_img(); })
       .def("set_centroids_ref",
            [](Centro& c, ArrayFStyle<float>& r) {
@@ -322,16 +327,16 @@ static void declare_centroiders(py::module& m) {
            [](Centro& c, ArrayFStyle<int32_t>& l, int32_t n) { return c.set_lutPix(l.mutable_data(), n); },
            py::arg("lutPix"), py::arg("n"));
 
-  using Cog = SutraCentroiderCog<float, float>;
-  py::class_<Cog, Centro>(m, "CentroiderCOG_FF");
+  using Cog = SutraCentroiderCog<Tin, float>;
+  py::class_<Cog, Centro>(m, ("CentroiderCOG" + sfx).c_str());
 
-  using Tcog = SutraCentroiderTcog<float, float>;
-  py::class_<Tcog, Centro>(m, "CentroiderTCOG_FF")
+  using Tcog = SutraCentroiderTcog<Tin, float>;
+  py::class_<Tcog, Centro>(m, ("CentroiderTCOG" + sfx).c_str())
       .def_property_readonly("threshold", [](Tcog& c) { return c.threshold; })
       .def("set_threshold", &Tcog::set_threshold, py::arg("thresh"));
 
-  using Wcog = SutraCentroiderWcog<float, float>;
-  py::class_<Wcog, Centro>(m, "CentroiderWCOG_FF")
+  using Wcog = SutraCentroiderWcog<Tin, float>;
+  py::class_<Wcog, Centro>(m, ("CentroiderWCOG" + sfx).c_str())
       .def_property_readonly("threshold", [](Wcog& c) { return c.threshold; })
       .def_property_readonly("d_weights", [](Wcog& c) { return c.d_weights; },
                              py::return_value_policy::reference_internal)
@@ -345,35 +350,37 @@ static void declare_centroiders(py::module& m) {
            py::arg("weights"), py::arg("ndim"))
       .def("set_threshold", &Wcog::set_threshold, py::arg("thresh"));
 
-  using Mlp = SutraCentroiderMlp<float, float>;
-  py::class_<Mlp, Centro>(m, "CentroiderMLP_FF")
-      .def_property_readonly("d_params", [](Mlp& c) { return c.d_params; },
-                             py::return_value_policy::reference_internal)
-      .def_property_readonly("d_grad", [](Mlp& c) { return c.d_grad; },
-                             py::return_value_policy::reference_internal)
-      .def_property_readonly("d_fallback", [](Mlp& c) { return c.d_fallback; },
-                             py::return_value_policy::reference_internal)
-      .def_property_readonly("x_scale", [](Mlp& c) { return c.x_scale; })
-      .def_property_readonly("flux_range", [](Mlp& c) { return py::make_tuple(c.flux_lo, c.flux_hi); })
-      .def_property_readonly("step", [](Mlp& c) { return c.step; })
-      .def("init_params", &Mlp::init_params, py::arg("seed"))
-      .def("set_input_scale", &Mlp::set_input_scale, py::arg("scale"))
-      .def("set_flux_range", &Mlp::set_flux_range, py::arg("lo"), py::arg("hi"))
-      .def("loss_grad", &Mlp::loss_grad, py::arg("x"), py::arg("t"), py::arg("n"),
-           py::arg("t_scale") = 1.0f, "mean loss; the gradient is left in d_grad")
-      .def("forward", &Mlp::forward, py::arg("x"), py::arg("n"), py::arg("y"))
-      .def("train",
-           [](Mlp& c, CarmaObj<float>* x, CarmaObj<float>* t, int64_t n, float t_scale,
-              int32_t batch, int32_t epochs, float lr, uint64_t seed) {
-             py::gil_scoped_release nogil;
-             return c.train(x, t, n, t_scale, batch, epochs, lr, seed);
-           },
-           py::arg("x"), py::arg("t"), py::arg("n"), py::arg("t_scale") = 1.0f,
-           py::arg("batch") = 4096, py::arg("epochs") = 1, py::arg("lr") = 1e-3f,
-           py::arg("seed") = 1u, "Adam epochs over shuffled batches; last epoch's mean loss");
+  if constexpr (std::is_same<Tin, float>::value) {  // the learned centroider takes float images
+    using Mlp = SutraCentroiderMlp<float, float>;
+    py::class_<Mlp, Centro>(m, "CentroiderMLP_FF")
+        .def_property_readonly("d_params", [](Mlp& c) { return c.d_params; },
+                               py::return_value_policy::reference_internal)
+        .def_property_readonly("d_grad", [](Mlp& c) { return c.d_grad; },
+                               py::return_value_policy::reference_internal)
+        .def_property_readonly("d_fallback", [](Mlp& c) { return c.d_fallback; },
+                               py::return_value_policy::reference_internal)
+        .def_property_readonly("x_scale", [](Mlp& c) { return c.x_scale; })
+        .def_property_readonly("flux_range", [](Mlp& c) { return py::make_tuple(c.flux_lo, c.flux_hi); })
+        .def_property_readonly("step", [](Mlp& c) { return c.step; })
+        .def("init_params", &Mlp::init_params, py::arg("seed"))
+        .def("set_input_scale", &Mlp::set_input_scale, py::arg("scale"))
+        .def("set_flux_range", &Mlp::set_flux_range, py::arg("lo"), py::arg("hi"))
+        .def("loss_grad", &Mlp::loss_grad, py::arg("x"), py::arg("t"), py::arg("n"),
+             py::arg("t_scale") = 1.0f, "mean loss; the gradient is left in d_grad")
+        .def("forward", &Mlp::forward, py::arg("x"), py::arg("n"), py::arg("y"))
+        .def("train",
+             [](Mlp& c, CarmaObj<float>* x, CarmaObj<float>* t, int64_t n, float t_scale,
+                int32_t batch, int32_t epochs, float lr, uint64_t seed) {
+               py::gil_scoped_release nogil;
+               return c.train(x, t, n, t_scale, batch, epochs, lr, seed);
+             },
+             py::arg("x"), py::arg("t"), py::arg("n"), py::arg("t_scale") = 1.0f,
+             py::arg("batch") = 4096, py::arg("epochs") = 1, py::arg("lr") = 1e-3f,
+             py::arg("seed") = 1u, "Adam epochs over shuffled batches; last epoch's mean loss");
+  }
 }
 
-static void declare_rtc(py::module& m) {
+static void declare_controller(py::module& m) {
   py::class_<Ctrl>(m, "Controller_FF")
       .def_property_readonly("type", [](Ctrl& c) { return c.type; })
       .def_property_readonly("nslope", [](Ctrl& c) { return c.nslope; })
@@ -383,23 +390,27 @@ static void declare_rtc(py::module& m) {
       .def_property_readonly("centro_idx", [](Ctrl& c) { return c.centro_idx; })
       .def_property_readonly("d_centroids", [](Ctrl& c) { return c.d_centroids; },
                              py::return_value_policy::reference_internal);
+}
 
-  py::class_<Rtc>(m, "Rtc_FFF")
+// RTC for images of Tin (Rtc_FFF, Rtc_UFF).
+template <typename Tin>
+static void declare_rtc(py::module& m, const char* name) {
+  using Rtc = SutraRtc<Tin, float, float>;
+  py::class_<Rtc>(m, name)
       .def(py::init([]() { return std::unique_ptr<Rtc>(new Rtc()); }))
       .def_property_readonly(
           "d_centro",
           [](Rtc& r) {
             py::list l;
             for (auto* c : r.d_centro) {
-              if (auto* t = dynamic_cast<SutraCentroiderTcog<float, float>*>(c))
+              if (auto* t = dynamic_cast<SutraCentroiderTcog<Tin, float>*>(c))
                 l.append(py::cast(t, py::return_value_policy::reference));
-              else if (auto* w = dynamic_cast<SutraCentroiderWcog<float, float>*>(c))
+              else if (auto* w = dynamic_cast<SutraCentroiderWcog<Tin, float>*>(c))
                 l.append(py::cast(w, py::return_value_policy::reference));
-              else if (auto* m = dynamic_cast<SutraCentroiderMlp<float, float>*>(c))
-                l.append(py::cast(m, py::return_value_policy::reference));
+              else if (auto* cg = dynamic_cast<SutraCentroiderCog<Tin, float>*>(c))
+                l.append(py::cast(cg, py::return_value_policy::reference));
               else
-                l.append(py::cast(dynamic_cast<SutraCentroiderCog<float, float>*>(c),
-                                  py::return_value_policy::reference));
+                l.append(py::cast(c, py::return_value_policy::reference));
             }
             return l;
           })
@@ -461,8 +472,11 @@ PYBIND11_MODULE(sutra, m) {
   declare_wfs(m);
   declare_wfs_sh(m);
   declare_sensors(m);
-  declare_centroiders(m);
-  declare_rtc(m);
+  declare_centroiders<float>(m, "_FF");
+  declare_centroiders<uint16_t>(m, "_UF");
+  declare_controller(m);
+  declare_rtc<float>(m, "Rtc_FFF");
+  declare_rtc<uint16_t>(m, "Rtc_UFF");
   m.def("comp_frame",
         [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, bool noise) {
           if (slopes.get_nb_elements() < 2 * w.nvalid) throw std::runtime_error("comp_frame: slope vector too short");

@@ -135,12 +135,27 @@ __global__ __launch_bounds__(256) void fill_weights_kernel(float* out, const flo
     out[i] = in[i % npix2];
 }
 
-__global__ __launch_bounds__(256) void calibrate_kernel(const float* raw, float* cal,
+template <typename Tin>
+__global__ __launch_bounds__(256) void calibrate_kernel(const Tin* raw, float* cal,
                                                         const float* dark, const float* flat,
                                                         const int* lut, long n) {
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
     const int j = lut[i];
-    cal[i] = (j >= 0 && j < n) ? (raw[j] - dark[i]) * flat[i] : 0.0f;
+    cal[i] = (j >= 0 && j < n) ? ((float)raw[j] - dark[i]) * flat[i] : 0.0f;
+  }
+}
+
+// Camera image: u16(min(fma(img, flat, dark) / mfpp, 1) * mpv), converted
+// toward zero and saturated to [0, 65535] (NaN to 0).
+__global__ __launch_bounds__(256) void digitalize_kernel(uint16_t* cam, const float* img,
+                                                         const float* dark, const float* flat,
+                                                         float mfpp, float mpv, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+    float t = fmaf(img[i], flat[i], dark[i]) / mfpp;
+    t = t < 1.0f ? t : 1.0f;
+    const float v = truncf(t * mpv);
+    // !(v > 0) also catches NaN
+    cam[i] = !(v > 0.0f) ? (uint16_t)0 : (v >= 65535.0f ? (uint16_t)65535 : (uint16_t)v);
   }
 }
 
@@ -390,9 +405,38 @@ int32_t shwfs_calibrate(const float* d_raw, float* d_cal, const float* d_dark, c
   if (n == 0) return SHWFS_OK;
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(calibrate_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
-                     d_raw, d_cal, d_dark, d_flat, d_lutPix, (long)n);
+  hipLaunchKernelGGL(calibrate_kernel<float>, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, d_raw, d_cal, d_dark, d_flat, d_lutPix, (long)n);
   SHWFS_LAUNCHED("calibrate_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_calibrate_u16(const uint16_t* d_raw, float* d_cal, const float* d_dark,
+                            const float* d_flat, const int32_t* d_lutPix, int64_t n, void* stream) {
+  if (!d_raw || !d_cal || !d_dark || !d_flat || !d_lutPix || n < 0)
+    return fail(SHWFS_ERR_ARG, "calibrate_u16: bad args");
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(calibrate_kernel<uint16_t>, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, d_raw, d_cal, d_dark, d_flat, d_lutPix, (long)n);
+  SHWFS_LAUNCHED("calibrate_kernel<uint16_t>");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_digitalize(uint16_t* d_cam, const float* d_img, const float* d_dark,
+                         const float* d_flat, int32_t max_flux_per_pix, int32_t max_pix_value,
+                         int64_t n, void* stream) {
+  if (!d_cam || !d_img || !d_dark || !d_flat || n < 0)
+    return fail(SHWFS_ERR_ARG, "digitalize: bad args");
+  if (max_flux_per_pix <= 0) return fail(SHWFS_ERR_ARG, "digitalize: max_flux_per_pix must be > 0");
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(digitalize_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     d_cam, d_img, d_dark, d_flat, (float)max_flux_per_pix, (float)max_pix_value,
+                     (long)n);
+  SHWFS_LAUNCHED("digitalize_kernel");
   return SHWFS_OK;
 }
 

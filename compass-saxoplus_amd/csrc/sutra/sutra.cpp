@@ -219,8 +219,35 @@ int32_t SutraWfs::set_flat(float* flat, int32_t nElem) {
 }
 
 int32_t SutraWfs::set_fakecam(bool f) {
-  if (f) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
   fakecam = f;
+  if (f) alloc_camera();
+  return EXIT_SUCCESS;
+}
+
+// Camera image (uint16), dark (0) and flat (1) with the detector image's
+// extents, as the reference allocates them for fakecam
+// (libsutra/src.cpp/sutra_wfs_sh.cpp:96-102); no-op before the detector image
+// exists (allocate_buffers calls it again) or once allocated.
+void SutraWfs::alloc_camera() {
+  if (d_binimg == nullptr || d_camimg != nullptr) return;
+  current_context->set_active_device(device, 1);
+  d_camimg = new CarmaObj<uint16_t>(current_context, d_binimg->get_dims());
+  d_dark = new CarmaObj<float>(current_context, d_binimg->get_dims());
+  d_dark->reset();
+  d_flat = new CarmaObj<float>(current_context, d_binimg->get_dims());
+  d_flat->memset(1.0f);
+}
+
+// The camera image from the detector image (fakecam; reference:
+// libsutra/src.cpp/sutra_wfs_sh.cpp:635-640).
+int32_t SutraWfs::digitalize_image() {
+  if (!fakecam) return EXIT_SUCCESS;
+  alloc_camera();
+  if (d_camimg == nullptr) throw std::runtime_error("fakecam: the detector image is not allocated");
+  shwfs_throw_if(shwfs_digitalize(d_camimg->get_data(), d_binimg->get_data(), d_dark->get_data(),
+                                  d_flat->get_data(), max_flux_per_pix, max_pix_value,
+                                  d_binimg->get_nb_elements(), nullptr),
+                 "digitalize");
   return EXIT_SUCCESS;
 }
 int32_t SutraWfs::set_max_flux_per_pix(int32_t v) {
@@ -270,9 +297,7 @@ SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
       bin_block(0),
       bin_origin(0),
       d_pupil_unit(nullptr),
-      noise_this_call(true) {
-  if (fakecam_) throw std::runtime_error("fakecam (uint16 camera image) is not part of this build");
-}
+      noise_this_call(true) {}
 
 // reference: libsutra/src.cpp/sutra_wfs_sh.cpp:48-75 (vestigial MPI split;
 // kept for API completeness, the kernels always see the local nvalid).
@@ -300,6 +325,7 @@ int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_pla
   if (rank == 0) {
     d_binimg = new CarmaObj<float>(current_context, dims2(d, npix * nxsub, npix * nxsub));
     if (roket) d_binimg_notnoisy = new CarmaObj<float>(current_context, dims2(d, npix * nxsub, npix * nxsub));
+    if (fakecam) alloc_camera();
   }
   if (d_camplipup == nullptr)
     d_camplipup = new CarmaObj<hipFloatComplex>(current_context, dims3(d, nfft, nfft, nvalid));
@@ -360,6 +386,9 @@ SutraWfsSH::~SutraWfsSH() {
   del(d_submask);
   del(d_fsamplipup);
   del(d_fsamplifoc);
+  del(d_camimg);
+  del(d_dark);
+  del(d_flat);
   del(d_gs);
   // shared scratch (d_camplipup/d_camplifoc/d_fttotim) belongs to SutraSensors
   // unless this WFS created it
@@ -503,6 +532,7 @@ int32_t SutraWfsSH::comp_image(bool noise_) {
   int32_t result = comp_generic();
   noise_this_call = true;
   result *= fill_binimage(0);
+  if (fakecam) digitalize_image();  // throws on failure
   return result;
 }
 

@@ -135,6 +135,8 @@ class SutraWfs {
   int32_t set_dark(float* dark, int32_t nElem);
   int32_t set_flat(float* flat, int32_t nElem);
   int32_t set_fakecam(bool fakecam);
+  void alloc_camera();
+  int32_t digitalize_image();
   int32_t set_max_flux_per_pix(int32_t max_flux_per_pix);
   int32_t set_max_pix_value(int32_t max_pix_value);
   int32_t slopes_geom(float* slopes, int32_t type = 0);

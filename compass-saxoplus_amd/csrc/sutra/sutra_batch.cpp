@@ -22,6 +22,8 @@ SutraWfsBatch::SutraWfsBatch(CarmaContext* context, vector<SutraWfsSH*> wfs_,
     if (!w->arrays_loaded || !w->d_gs)
       throw std::runtime_error("WfsBatch: load_arrays / initgs have not been called");
     if (w->device != device) throw std::invalid_argument("WfsBatch: WFS on different devices");
+    if (w->fakecam)
+      throw std::invalid_argument("WfsBatch: fakecam WFS (camera image) are run through comp_image");
     int64_t d[2] = {1, 2 * w->nvalid};
     d_slopes.push_back(new CarmaObj<float>(context, d));
     geoms.push_back(w->geom());

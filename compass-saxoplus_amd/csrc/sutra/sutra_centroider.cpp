@@ -1,5 +1,6 @@
 // CoG-family centroiders and the RTC dispatch of the SH WFS path.
 // reference: libsutra/src.cpp/sutra_centroider{,_cog,_wcog,_tcog}.cpp, sutra_rtc.cpp
+#include <type_traits>
 #include <iostream>
 
 #include "sutra.hpp"
@@ -169,10 +170,16 @@ int32_t SutraCentroider<Tin, Tout>::calibrate_img(hipStream_t stream) {
   init_calib((int32_t)d_img_raw->get_dims(1), (int32_t)d_img_raw->get_dims(2));
   if (d_img->get_nb_elements() != d_img_raw->get_nb_elements())
     throw std::runtime_error("calibrate_img: raw and calibrated images differ in size");
-  shwfs_throw_if(shwfs_calibrate(d_img_raw->get_data(), d_img->get_data(), d_dark->get_data(),
-                                 d_flat->get_data(), d_lutPix->get_data(),
-                                 d_img->get_nb_elements(), stream),
-                 "calibrate_img");
+  if constexpr (std::is_same<Tin, uint16_t>::value)
+    shwfs_throw_if(shwfs_calibrate_u16(d_img_raw->get_data(), d_img->get_data(),
+                                       d_dark->get_data(), d_flat->get_data(),
+                                       d_lutPix->get_data(), d_img->get_nb_elements(), stream),
+                   "calibrate_img");
+  else
+    shwfs_throw_if(shwfs_calibrate(d_img_raw->get_data(), d_img->get_data(), d_dark->get_data(),
+                                   d_flat->get_data(), d_lutPix->get_data(),
+                                   d_img->get_nb_elements(), stream),
+                   "calibrate_img");
   return EXIT_SUCCESS;
 }
 
@@ -546,6 +553,12 @@ template class SutraCentroider<float, float>;
 template class SutraCentroiderCog<float, float>;
 template class SutraCentroiderTcog<float, float>;
 template class SutraCentroiderWcog<float, float>;
+// uint16 camera frames (Centroider_UF; calibrate_img turns them into the
+// float image the centroids are taken from)
+template class SutraCentroider<uint16_t, float>;
+template class SutraCentroiderCog<uint16_t, float>;
+template class SutraCentroiderTcog<uint16_t, float>;
+template class SutraCentroiderWcog<uint16_t, float>;
 
 // ============================================================ controller / rtc
 template <class T, class Tout>
@@ -604,10 +617,13 @@ int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nv
   else if (typec == "wcog")
     d_centro.push_back(new SutraCentroiderWcog<Tin, T>(context, wfs, nvalid, offset, scale,
                                                        filter_TT, (int32_t)device));
-  else if (typec == "mlp")  // learned centroider (this build's addition)
-    d_centro.push_back(new SutraCentroiderMlp<Tin, T>(context, wfs, nvalid, offset, scale,
-                                                      filter_TT, (int32_t)device));
-  else {
+  else if (typec == "mlp") {  // learned centroider (this build's addition), float images
+    if constexpr (std::is_same<Tin, float>::value)
+      d_centro.push_back(new SutraCentroiderMlp<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                        filter_TT, (int32_t)device));
+    else
+      throw std::runtime_error("add_centroider: the mlp centroider takes float images (Rtc_FFF)");
+  } else {
     std::cerr << "centroider unknown" << std::endl;
     return EXIT_FAILURE;
   }
@@ -682,3 +698,4 @@ int32_t SutraRtc<Tin, T, Tout>::set_centroids_ref(float* centroids_ref) {
 }
 
 template class SutraRtc<float, float, float>;
+template class SutraRtc<uint16_t, float, float>;

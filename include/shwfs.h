@@ -164,6 +164,21 @@ int32_t shwfs_prng(float* d_data, int64_t n, char gtype, float alpha, float beta
 int32_t shwfs_calibrate(const float* d_raw, float* d_cal, const float* d_dark, const float* d_flat,
                         const int32_t* d_lutPix, int64_t n, void* stream);
 
+/* Same for a uint16 camera image: calib_krnl<uint16_t>
+ * (libsutra/src.cu/sutra_centroider.cu:107-114, instantiated at :140-160). */
+int32_t shwfs_calibrate_u16(const uint16_t* d_raw, float* d_cal, const float* d_dark,
+                            const float* d_flat, const int32_t* d_lutPix, int64_t n, void* stream);
+
+/* ---- simulated camera image (fakecam) ----------------------------------------
+ * cam[i] = u16(min(fma(img[i], flat[i], dark[i]) / max_flux_per_pix, 1) * max_pix_value),
+ * u16() converting toward zero with saturation to [0, 65535] (NaN -> 0), as the
+ * reference's float-to-integer conversion does; replaces digitalize_krnl,
+ * libsutra/src.cu/sutra_wfs.cu:2243-2255, called from SutraWfsSH::comp_image
+ * (libsutra/src.cpp/sutra_wfs_sh.cpp:635-640).  max_flux_per_pix > 0. */
+int32_t shwfs_digitalize(uint16_t* d_cam, const float* d_img, const float* d_dark,
+                         const float* d_flat, int32_t max_flux_per_pix, int32_t max_pix_value,
+                         int64_t n, void* stream);
+
 /* ---- geometric slopes (mean phase gradient per sub-aperture) -----------------
  * type 0: edge differences, reduce_phasex/y (libsutra/src.cu/sutra_wfs.cu:868-913);
  * type 2: projection on ttprojmat (1036-1090).  phase in micrometres; phasemap

@@ -61,6 +61,8 @@ def load():
     lib.shwfs_fill_weights.argtypes = [vp, vp, i32, i32, vp]
     lib.shwfs_prng.argtypes = [vp, C.c_int64, C.c_char, f32, f32, u64, u32, u32, vp]
     lib.shwfs_calibrate.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
+    lib.shwfs_calibrate_u16.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
+    lib.shwfs_digitalize.argtypes = [vp, vp, vp, vp, i32, i32, C.c_int64, vp]
     lib.shwfs_slopes_geom.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp]
     lib.shwfs_mfma_probe.argtypes = [vp, vp, vp, i32, vp]
     lib.shwfs_batch_create.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), i32,
