@@ -350,6 +350,16 @@ static void declare_centroiders(py::module& m, const std::string& sfx) {
            py::arg("weights"), py::arg("ndim"))
       .def("set_threshold", &Wcog::set_threshold, py::arg("thresh"));
 
+  using Bpcog = SutraCentroiderBpcog<Tin, float>;
+  py::class_<Bpcog, Centro>(m, ("CentroiderBPCOG" + sfx).c_str())
+      .def_property_readonly("nmax", [](Bpcog& c) { return c.nmax; })
+      .def_property_readonly("d_bpix", [](Bpcog& c) { return c.d_bpix; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_bpind", [](Bpcog& c) { return c.d_bpind; },
+                             py::return_value_policy::reference_internal)
+      .def("set_nmax", &Bpcog::set_nmax, py::arg("nmax"),
+           "number of brightest pixels kept per sub-aperture");
+
   if constexpr (std::is_same<Tin, float>::value) {  // the learned centroider takes float images
     using Mlp = SutraCentroiderMlp<float, float>;
     py::class_<Mlp, Centro>(m, "CentroiderMLP_FF")
@@ -403,7 +413,9 @@ static void declare_rtc(py::module& m, const char* name) {
           [](Rtc& r) {
             py::list l;
             for (auto* c : r.d_centro) {
-              if (auto* t = dynamic_cast<SutraCentroiderTcog<Tin, float>*>(c))
+              if (auto* b = dynamic_cast<SutraCentroiderBpcog<Tin, float>*>(c))
+                l.append(py::cast(b, py::return_value_policy::reference));
+              else if (auto* t = dynamic_cast<SutraCentroiderTcog<Tin, float>*>(c))
                 l.append(py::cast(t, py::return_value_policy::reference));
               else if (auto* w = dynamic_cast<SutraCentroiderWcog<Tin, float>*>(c))
                 l.append(py::cast(w, py::return_value_policy::reference));

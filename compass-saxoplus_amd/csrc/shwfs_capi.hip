@@ -97,6 +97,72 @@ static int32_t launch_centroid_k(const float* in, float* out, const shwfs_centro
   return launch_centroid_g<64, KIND>(in, out, a, st);
 }
 
+// Brightest-pixel CoG, one wave per sub-aperture, four per workgroup.  The
+// window (npix^2 pixels, 0 outside the image) goes to the wave's LDS slot.
+// Pixel p has rank #{q: v_q > v_p} in decreasing order; the value with
+// #{q: v_q > v} <= nbpix < #{q: v_q >= v} is element nbpix of the window
+// sorted in decreasing order (ties included), the threshold.  Pixels above
+// it keep v - threshold, the others 0 (written back into the image); then the
+// float CoG moments and the double division of the other centroiders.
+__global__ __launch_bounds__(256) void bpcog_kernel(float* img, int nbpix, shwfs_centroid_args a) {
+  extern __shared__ float sm[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 4 + wave;
+  const bool valid = s < a.nvalid;  // uniform over the wave
+  const int npix = a.npix, n2 = npix * npix;
+  float* v = sm + wave * (n2 + 1);
+  const long lim = (long)a.size * a.size;
+  int xv = 0, yv = 0;
+  if (valid) {
+    xv = a.d_validx[s];
+    yv = a.d_validy[s];
+  }
+  for (int p = lane; p < n2; p += 64) {
+    const long idim = (long)(p % npix + xv) + (long)(p / npix + yv) * a.size;
+    v[p] = (valid && idim < lim) ? img[idim] : 0.0f;
+  }
+  if (lane == 0) v[n2] = 0.0f;  // threshold when no value qualifies (NaN in the window)
+  wave_lds_sync();
+  for (int p = lane; p < n2; p += 64) {
+    const float vp = v[p];
+    int gt = 0, ge = 0;
+    for (int q = 0; q < n2; ++q) {
+      const float w = v[q];  // same address in every lane: an LDS broadcast
+      gt += w > vp ? 1 : 0;
+      ge += w >= vp ? 1 : 0;
+    }
+    if (gt <= nbpix && nbpix < ge) v[n2] = vp;  // every such lane writes the same value
+  }
+  wave_lds_sync();
+  const float thr = v[n2];
+  float I = 0.0f, Sx = 0.0f, Sy = 0.0f;
+  if (valid) {
+    for (int p = lane; p < n2; p += 64) {
+      const int x = p % npix, y = p / npix;
+      const long idim = (long)(x + xv) + (long)(y + yv) * a.size;
+      if (idim < lim) {
+        const float d = v[p] > thr ? v[p] - thr : 0.0f;
+        img[idim] = d;
+        I += d;
+        Sx += d * (float)x;
+        Sy += d * (float)y;
+      }
+    }
+  }
+  I = wave_sum(I);
+  Sx = wave_sum(Sx);
+  Sy = wave_sum(Sy);
+  if (valid && lane == 0) {
+    const int ix = a.slope_order == 0 ? s : 2 * s;
+    const int iy = a.slope_order == 0 ? s + a.nvalid : 2 * s + 1;
+    const float rx = a.d_ref ? a.d_ref[ix] : 0.0f;
+    const float ry = a.d_ref ? a.d_ref[iy] : 0.0f;
+    a.d_slopes[ix] = ((float)((double)Sx / ((double)I + 1.e-6)) - a.offset) * a.scale - rx;
+    a.d_slopes[iy] = ((float)((double)Sy / ((double)I + 1.e-6)) - a.offset) * a.scale - ry;
+    a.d_intensities[s] = I;
+  }
+}
+
 // ---------------------------------------------------------------- noise on an array
 __global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype, float alpha,
                                                    float beta, unsigned long long seed,
@@ -384,6 +450,25 @@ int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_cen
       return launch_centroid_k<SHWFS_CENTROID_WCOG>(d_img_in, nullptr, *a, st);
     default: return fail(SHWFS_ERR_ARG, "centroids: unknown kind");
   }
+}
+
+int32_t shwfs_centroids_bpcog(float* d_img, int32_t nbpix, const shwfs_centroid_args* a,
+                              void* stream) {
+  if (!a || !d_img) return fail(SHWFS_ERR_ARG, "centroids_bpcog: NULL argument");
+  if (a->nvalid <= 0 || a->npix <= 0 || a->size <= 0)
+    return fail(SHWFS_ERR_ARG, "centroids_bpcog: sizes must be positive");
+  const int n2 = a->npix * a->npix;
+  if (n2 > 1024) return fail(SHWFS_ERR_UNSUPPORTED, "centroids_bpcog: npix^2 above 1024");
+  if (nbpix < 0 || nbpix >= n2)
+    return fail(SHWFS_ERR_ARG, "centroids_bpcog: nbpix must lie in [0, npix^2)");
+  if (!a->d_validx || !a->d_validy || !a->d_slopes || !a->d_intensities)
+    return fail(SHWFS_ERR_STATE, "centroids_bpcog: valid positions or outputs not set");
+  const int grid = (a->nvalid + 3) / 4;
+  const size_t lds = sizeof(float) * 4 * (size_t)(n2 + 1);
+  hipLaunchKernelGGL(bpcog_kernel, dim3(grid), dim3(256), lds, (hipStream_t)stream, d_img, nbpix,
+                     *a);
+  SHWFS_LAUNCHED("bpcog_kernel");
+  return SHWFS_OK;
 }
 
 int32_t shwfs_fill_weights(float* d_out, const float* d_in, int32_t npix, int32_t n_total,

@@ -341,6 +341,29 @@ class SutraCentroiderCog : public SutraCentroider<Tin, T> {
                                     int32_t size) override;
 };
 
+// Brightest-pixel CoG (reference: libsutra/include/sutra_centroider_bpcog.hpp,
+// src.cpp/sutra_centroider_bpcog.cpp): per sub-aperture, the nmax brightest
+// pixels above the (nmax+1)-th.  Taken from the detector image after the
+// frame (not fused into the spot kernels).
+template <class Tin, class T>
+class SutraCentroiderBpcog : public SutraCentroider<Tin, T> {
+ public:
+  int32_t nmax;
+  CarmaObj<T>* d_bpix;          // [nmax, nvalid], allocated as the reference does
+  CarmaObj<uint32_t>* d_bpind;  // [nmax, nvalid]
+  SutraCentroiderBpcog(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                       float scale, bool filter_TT, int32_t device, int32_t nmax);
+  ~SutraCentroiderBpcog();
+  string get_type() override { return "bpcog"; }
+  int32_t set_nmax(int32_t nmax);
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+};
+
 template <class Tin, class T>
 class SutraCentroiderTcog : public SutraCentroider<Tin, T> {
  public:

@@ -356,6 +356,84 @@ int32_t SutraCentroiderCog<Tin, T>::get_cog() {
   return EXIT_SUCCESS;
 }
 
+// ============================================================ Bpcog
+// reference: libsutra/src.cpp/sutra_centroider_bpcog.cpp:26-124
+template <class Tin, class T>
+SutraCentroiderBpcog<Tin, T>::SutraCentroiderBpcog(CarmaContext* context, SutraWfs* wfs,
+                                                   int64_t nvalid, float offset, float scale,
+                                                   bool filter_TT, int32_t device, int32_t nmax_)
+    : SutraCentroider<Tin, T>(context, wfs, nvalid, offset, scale, filter_TT, device),
+      nmax(0),
+      d_bpix(nullptr),
+      d_bpind(nullptr) {
+  set_nmax(nmax_);
+}
+template <class Tin, class T>
+SutraCentroiderBpcog<Tin, T>::~SutraCentroiderBpcog() {
+  del(d_bpix);
+  del(d_bpind);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderBpcog<Tin, T>::set_nmax(int32_t n) {
+  if (n <= 0) throw std::runtime_error("set_nmax: nmax must be positive");
+  this->current_context->set_active_device(this->device, 1);
+  if (n != nmax || d_bpix == nullptr) {
+    nmax = n;
+    del(d_bpix);
+    del(d_bpind);
+    int64_t d[3] = {2, n, this->nvalid};
+    d_bpix = new CarmaObj<T>(this->current_context, d);
+    d_bpind = new CarmaObj<uint32_t>(this->current_context, d);
+  }
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+shwfs_centroid_args SutraCentroiderBpcog<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                                int32_t nvalid, int32_t npix,
+                                                                int32_t size) {
+  // the kind is not used by shwfs_centroids_bpcog; the fused paths refuse bpcog
+  return base_args(this, SHWFS_CENTROID_NONE, intensities, slopes, nvalid, npix, size);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderBpcog<Tin, T>::get_cog(float* img, float* intensities, T* centroids,
+                                              int32_t nvalid, int32_t npix, int32_t ntot,
+                                              hipStream_t stream) {
+  this->current_context->set_active_device(this->device, 1);
+  if (this->d_validx == nullptr)
+    throw std::runtime_error("get_cog: load_validpos has not been called");
+  if (npix <= 0) throw std::runtime_error("get_cog: set_npix has not been called");
+  if (this->d_validx->get_nb_elements() < nvalid)
+    throw std::runtime_error("get_cog: fewer valid positions loaded than sub-apertures");
+  shwfs_centroid_args a = centroid_args(intensities, centroids, nvalid, npix, ntot);
+  shwfs_throw_if(shwfs_centroids_bpcog(img, nmax, &a, stream), "get_cog (bpcog)");
+  if (this->filter_TT) this->apply_TT_filter(centroids);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderBpcog<Tin, T>::get_cog(float* intensities, T* slopes, bool noise) {
+  if (this->wfs != nullptr) {
+    CarmaObj<float>* img = (noise || !this->wfs->roket) ? this->wfs->d_binimg
+                                                        : this->wfs->d_binimg_notnoisy;
+    return get_cog(img->get_data(), intensities, slopes, (int32_t)this->wfs->nvalid,
+                   (int32_t)this->wfs->npix, (int32_t)this->wfs->d_binimg->get_dims(1),
+                   this->wfs->stream);
+  }
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderBpcog<Tin, T>::get_cog() {
+  if (this->wfs != nullptr)
+    return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
 // ============================================================ Tcog
 // reference: libsutra/src.cpp/sutra_centroider_tcog.cpp:26-108
 template <class Tin, class T>
@@ -510,7 +588,7 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
       throw std::runtime_error("comp_frame: WCoG weights not initialised for this WFS");
   }
   wfs->current_context->set_active_device(wfs->device, 1);
-  if (c->get_type() == "mlp") {  // not fusable: spots, then the network
+  if (c->get_type() == "mlp" || c->get_type() == "bpcog") {  // not fused: spots, then centroids
     wfs->comp_image(noise);
     return c->get_cog(intensities, slopes, noise);
   }
@@ -559,6 +637,8 @@ template class SutraCentroider<uint16_t, float>;
 template class SutraCentroiderCog<uint16_t, float>;
 template class SutraCentroiderTcog<uint16_t, float>;
 template class SutraCentroiderWcog<uint16_t, float>;
+template class SutraCentroiderBpcog<float, float>;
+template class SutraCentroiderBpcog<uint16_t, float>;
 
 // ============================================================ controller / rtc
 template <class T, class Tout>
@@ -608,7 +688,10 @@ template <typename Tin, typename T, typename Tout>
 int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nvalid, float offset,
                                                float scale, bool filter_TT, int64_t device,
                                                string typec, SutraWfs* wfs) {
-  if (typec == "cog")
+  if (typec == "bpcog")  // ten brightest pixels, as the reference's factory
+    d_centro.push_back(new SutraCentroiderBpcog<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                        filter_TT, (int32_t)device, 10));
+  else if (typec == "cog")
     d_centro.push_back(new SutraCentroiderCog<Tin, T>(context, wfs, nvalid, offset, scale,
                                                       filter_TT, (int32_t)device));
   else if (typec == "tcog")

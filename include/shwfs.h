@@ -151,6 +151,17 @@ int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_cen
 int32_t shwfs_fill_weights(float* d_out, const float* d_in, int32_t npix, int32_t n_total,
                            void* stream);
 
+/* Brightest-pixel CoG over the image d_img (a->kind, threshold and weights are
+ * not used): per sub-aperture the window value of rank nbpix in decreasing
+ * order (0: the brightest; pixels outside the image count as 0) is the
+ * threshold t; pixels keep v > t ? v - t : 0, written back into d_img, and the
+ * CoG of those values gives the slopes and intensities.  Replaces
+ * get_centroids of libsutra/src.cu/sutra_centroider_bpcog.cu:28-78 (run by
+ * SutraCentroiderBpcog::get_cog, src.cpp/sutra_centroider_bpcog.cpp:77-90).
+ * 0 <= nbpix < npix^2 <= 1024. */
+int32_t shwfs_centroids_bpcog(float* d_img, int32_t nbpix, const shwfs_centroid_args* a,
+                              void* stream);
+
 /* ---- counter-based noise on a device array ----------------------------------
  * Replaces CarmaObj<float>::prng (libcarma/src.cpp/carma_obj.cpp:490-527,
  * src.cu/carma_rng.cu:98-141,287-303) with Philox4x32-10 streams:

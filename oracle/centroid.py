@@ -3,6 +3,9 @@
 * cog   libsutra/src.cu/sutra_centroider_cog.cu:28-81
 * wcog  libsutra/src.cu/sutra_centroider_wcog.cu:58-116 (+ fill_weights 27-50)
 * tcog  libsutra/src.cu/sutra_centroider_tcog.cu:27-76 (writes d' back to the image)
+* bpcog libsutra/src.cu/sutra_centroider_bpcog.cu:28-78: t = element nbpix of the
+        window sorted in decreasing order (pixels outside the image count as 0),
+        d = img > t ? img - t : 0 (written back), float moments
 * slope ordering: libsutra/src.cu/sutra_centroider_utils.cuh:31-51
       untied:     x slopes at [0, nvalid), y slopes at [nvalid, 2*nvalid)
       interlaced: x at 2*s, y at 2*s+1
@@ -105,3 +108,32 @@ def gaussian_weights(npix, width):
             r = np.sqrt((j - c) ** 2 + (i - c) ** 2)
             out[i, j] = np.exp(-((r / (width / 1.66)) ** 2.0))
     return out.astype(np.float32)
+
+
+def bpcog(img, validx, validy, npix, nbpix, offset, scale, ref=None, order=0):
+    """Brightest-pixel CoG.  Pixel (x, y) of sub-aperture s is the flat
+    (x fastest) image element (x + validx[s]) + (y + validy[s]) * size, read
+    only below size^2, as the reference indexes it.  Returns (slopes,
+    intensities, thresholded image)."""
+    img = np.array(img, dtype=np.float64)
+    size = img.shape[0]
+    flat = img.ravel(order="F")
+    nvalid = len(validx)
+    slopes = np.zeros(2 * nvalid)
+    inten = np.zeros(nvalid)
+    p = np.arange(npix * npix)
+    x, y = p % npix, p // npix
+    for s in range(nvalid):
+        idim = (x + validx[s]) + (y + validy[s]) * size
+        inside = idim < flat.size
+        v = np.where(inside, flat[np.minimum(idim, flat.size - 1)], 0.0)
+        t = np.sort(v)[::-1][nbpix]
+        d = np.where(v > t, v - t, 0.0)
+        I, Sx, Sy = d[inside].sum(), (d * x)[inside].sum(), (d * y)[inside].sum()
+        flat[idim[inside]] = d[inside]
+        cx, cy = _finish(I, Sx, Sy, offset, scale)
+        ix, iy = _slot(s, nvalid, order)
+        slopes[ix] = cx - (0 if ref is None else ref[ix])
+        slopes[iy] = cy - (0 if ref is None else ref[iy])
+        inten[s] = I
+    return slopes, inten, flat.reshape(img.shape, order="F")
