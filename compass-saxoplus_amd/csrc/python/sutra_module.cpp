@@ -506,6 +506,37 @@ PYBIND11_MODULE(sutra, m) {
         py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
         py::arg("noise") = true, py::arg("fused") = true,
         "issues nframes frames back to back; returns hipEvent-timed milliseconds");
+  py::class_<SutraAcquisim>(m, "Acquisim")
+      .def(py::init([](SutraSensors& sensors, int32_t wfs_num) {
+             return new SutraAcquisim(&sensors, wfs_num);
+           }),
+           py::arg("sensors"), py::arg("wfs_num"), py::keep_alive<1, 2>(),
+           "spots of an SH WFS from an external detector image")
+      .def_property_readonly("npix", [](SutraAcquisim& a) { return a.npix; })
+      .def_property_readonly("nxsub", [](SutraAcquisim& a) { return a.nxsub; })
+      .def_property_readonly("nvalid", [](SutraAcquisim& a) { return a.nvalid; })
+      .def("set_validsubs",
+           [](SutraAcquisim& a, ArrayFStyle<int32_t>& vx, ArrayFStyle<int32_t>& vy) {
+             if (vx.size() != vy.size()) throw std::runtime_error("set_validsubs: sizes differ");
+             return a.set_validsubs(vx.size(), vx.mutable_data(), vy.mutable_data());
+           },
+           py::arg("validsubsx"), py::arg("validsubsy"))
+      .def("comp_image",
+           [](SutraAcquisim& a, ArrayFStyle<float>& img) {
+             if (img.ndim() != 2) throw std::runtime_error("comp_image: a 2-D image is expected");
+             int64_t dims[3] = {2, img.shape(0), img.shape(1)};
+             return a.comp_image(dims, img.mutable_data());
+           },
+           py::arg("image"), "gathers the spots into the WFS's bincube")
+      .def("comp_image_2D",
+           [](SutraAcquisim& a, ArrayFStyle<float>& img, ArrayFStyle<int32_t>& num) {
+             if (img.ndim() != 2) throw std::runtime_error("comp_image_2D: a 2-D image is expected");
+             if (num.size() != a.nxsub * a.nxsub)
+               throw std::runtime_error("comp_image_2D: num_ssp must have nxsub^2 entries");
+             int64_t dims[3] = {2, img.shape(0), img.shape(1)};
+             return a.comp_image_2D(dims, img.mutable_data(), num.mutable_data());
+           },
+           py::arg("image"), py::arg("num_ssp"));
   py::class_<SutraWfsBatch>(m, "WfsBatch")
       .def(py::init([](CarmaContext& ctx, std::vector<SutraWfsSH*> wfs,
                        std::vector<Centro*> centro, bool noise) {

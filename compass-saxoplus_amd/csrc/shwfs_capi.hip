@@ -163,6 +163,36 @@ __global__ __launch_bounds__(256) void bpcog_kernel(float* img, int nbpix, shwfs
   }
 }
 
+// Spots of an external detector image (acquisim), one thread per element.
+__global__ __launch_bounds__(256) void fill_bincube_kernel(const float* image, float* bcube,
+                                                           const int* vx, const int* vy, int npix,
+                                                           long n, int size) {
+  const int npix2 = npix * npix;
+  const long lim = (long)size * size;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256L) {
+    const int s = (int)(t / npix2), p = (int)(t - (long)s * npix2);
+    const long idim = (long)(p % npix + vx[s]) + (long)(p / npix + vy[s]) * size;
+    bcube[t] = (idim >= 0 && idim < lim) ? image[idim] : 0.0f;
+  }
+}
+
+// The same from a sub-aperture number map: element (x, y) of grid cell
+// (i, j) of the image goes to spot num_ssp[i + nxsub j] - 1.
+__global__ __launch_bounds__(256) void fill_bincube_2d_kernel(const float* image, float* bcube,
+                                                              const int* num, int npix, int nxsub,
+                                                              int nvalid) {
+  const int npix2 = npix * npix;
+  const long n = (long)nxsub * nxsub * npix2;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256L) {
+    const int cell = (int)(t / npix2), p = (int)(t - (long)cell * npix2);
+    const int i = cell % nxsub, j = cell / nxsub, x = p % npix, y = p / npix;
+    const int k = num[cell];
+    if (k > 0 && k <= nvalid)
+      bcube[(long)(k - 1) * npix2 + p] =
+          image[(long)(x + npix * i) + (long)nxsub * npix * (y + npix * j)];
+  }
+}
+
 // ---------------------------------------------------------------- noise on an array
 __global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype, float alpha,
                                                    float beta, unsigned long long seed,
@@ -468,6 +498,34 @@ int32_t shwfs_centroids_bpcog(float* d_img, int32_t nbpix, const shwfs_centroid_
   hipLaunchKernelGGL(bpcog_kernel, dim3(grid), dim3(256), lds, (hipStream_t)stream, d_img, nbpix,
                      *a);
   SHWFS_LAUNCHED("bpcog_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_fill_bincube(const float* d_image, float* d_bincube, const int32_t* d_validx,
+                           const int32_t* d_validy, int32_t npix, int32_t nvalid, int32_t size,
+                           void* stream) {
+  if (!d_image || !d_bincube || !d_validx || !d_validy || npix <= 0 || nvalid < 0 || size <= 0)
+    return fail(SHWFS_ERR_ARG, "fill_bincube: bad args");
+  const long n = (long)nvalid * npix * npix;
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_bincube_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, d_image, d_bincube, d_validx, d_validy, npix, n, size);
+  SHWFS_LAUNCHED("fill_bincube_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_fill_bincube_2d(const float* d_image, float* d_bincube, const int32_t* d_num_ssp,
+                              int32_t npix, int32_t nxsub, int32_t nvalid, void* stream) {
+  if (!d_image || !d_bincube || !d_num_ssp || npix <= 0 || nxsub <= 0 || nvalid < 0)
+    return fail(SHWFS_ERR_ARG, "fill_bincube_2d: bad args");
+  const long n = (long)nxsub * nxsub * npix * npix;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_bincube_2d_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, d_image, d_bincube, d_num_ssp, npix, nxsub, nvalid);
+  SHWFS_LAUNCHED("fill_bincube_2d_kernel");
   return SHWFS_OK;
 }
 

@@ -711,3 +711,79 @@ int32_t SutraSensors::set_field_stop(int32_t nwfs, float* field_stop, int32_t N)
   if (wfs) return wfs->set_field_stop(campli_plans, field_stop, N);
   return EXIT_SUCCESS;
 }
+
+// ============================================================ acquisim
+// reference: libsutra/src.cpp/sutra_acquisim.cpp:24-83
+SutraAcquisim::SutraAcquisim(SutraSensors* sensors, int32_t wfs_num)
+    : device(0), nxsub(0), nvalid(0), npix(0), current_context(nullptr), wfs(nullptr),
+      d_validsubsx(nullptr), d_validsubsy(nullptr), own_valid(false) {
+  if (sensors == nullptr || wfs_num < 0 || wfs_num >= (int32_t)sensors->d_wfs.size())
+    throw std::runtime_error("Acquisim: no such WFS");
+  SutraWfs* w = sensors->d_wfs[wfs_num];
+  wfs = dynamic_cast<SutraWfsSH*>(w);
+  if (wfs == nullptr) throw std::runtime_error("Acquisim: the WFS is not a Shack-Hartmann WFS");
+  device = w->device;
+  type = w->type;
+  nxsub = w->nxsub;
+  nvalid = w->nvalid;
+  npix = w->npix;
+  current_context = w->current_context;
+  d_validsubsx = wfs->d_validsubsx;
+  d_validsubsy = wfs->d_validsubsy;
+}
+
+SutraAcquisim::~SutraAcquisim() {
+  if (own_valid) {
+    delete d_validsubsx;
+    delete d_validsubsy;
+  }
+}
+
+int32_t SutraAcquisim::set_validsubs(int64_t n, int32_t* validsubsx, int32_t* validsubsy) {
+  current_context->set_active_device(device, 1);
+  int64_t d[2] = {1, n};
+  if (own_valid) {
+    delete d_validsubsx;
+    delete d_validsubsy;
+  }
+  d_validsubsx = new CarmaObj<int32_t>(current_context, d, validsubsx);
+  d_validsubsy = new CarmaObj<int32_t>(current_context, d, validsubsy);
+  own_valid = true;
+  nvalid = n;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraAcquisim::comp_image(int64_t* dims, float* bimage) {
+  return comp_image(dims, bimage, wfs->d_bincube);
+}
+
+int32_t SutraAcquisim::comp_image(int64_t* dims, float* bimage, CarmaObj<float>* d_bincube) {
+  current_context->set_active_device(device, 1);
+  if (d_validsubsx == nullptr || d_bincube == nullptr)
+    throw std::runtime_error("Acquisim: valid positions or bincube not set");
+  if (d_bincube->get_nb_elements() < nvalid * npix * npix)
+    throw std::runtime_error("Acquisim: bincube smaller than nvalid * npix^2");
+  CarmaObj<float> img(current_context, dims, bimage);
+  shwfs_throw_if(shwfs_fill_bincube(img.get_data(), d_bincube->get_data(),
+                                    d_validsubsx->get_data(), d_validsubsy->get_data(),
+                                    (int32_t)npix, (int32_t)nvalid, (int32_t)(npix * nxsub),
+                                    wfs->stream),
+                 "Acquisim.comp_image");
+  carma_safe_call(hipStreamSynchronize(wfs->stream));  // before the image copy is freed
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraAcquisim::comp_image_2D(int64_t* dims, float* bimage, int32_t* num_ssp) {
+  current_context->set_active_device(device, 1);
+  if (dims[1] * dims[2] < (npix * nxsub) * (npix * nxsub))
+    throw std::runtime_error("Acquisim.comp_image_2D: image smaller than (npix * nxsub)^2");
+  CarmaObj<float> img(current_context, dims, bimage);
+  int64_t d1[2] = {1, nxsub * nxsub};
+  CarmaObj<int32_t> num(current_context, d1, num_ssp);
+  shwfs_throw_if(shwfs_fill_bincube_2d(img.get_data(), wfs->d_bincube->get_data(), num.get_data(),
+                                       (int32_t)npix, (int32_t)nxsub, (int32_t)wfs->nvalid,
+                                       wfs->stream),
+                 "Acquisim.comp_image_2D");
+  carma_safe_call(hipStreamSynchronize(wfs->stream));
+  return EXIT_SUCCESS;
+}

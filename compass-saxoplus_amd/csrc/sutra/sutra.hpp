@@ -248,6 +248,31 @@ class SutraSensors {
                  float* G, float* thetaML, float* dx, float* dy);
 };
 
+// Spots of an SH WFS from an external detector image (reference:
+// libsutra/include/sutra_acquisim.hpp, src.cpp/sutra_acquisim.cpp): the image
+// (side npix * nxsub) is copied to the device and its npix x npix windows at
+// the valid positions are gathered into the WFS's bincube.
+class SutraAcquisim {
+ public:
+  int32_t device;
+  string type;
+  int64_t nxsub;
+  int64_t nvalid;
+  int64_t npix;
+  CarmaContext* current_context;
+  SutraWfsSH* wfs;
+  CarmaObj<int32_t>* d_validsubsx;  // the WFS's, or owned after set_validsubs
+  CarmaObj<int32_t>* d_validsubsy;
+  bool own_valid;
+
+  SutraAcquisim(SutraSensors* sensors, int32_t wfs_num);
+  ~SutraAcquisim();
+  int32_t set_validsubs(int64_t nvalid, int32_t* validsubsx, int32_t* validsubsy);
+  int32_t comp_image(int64_t* dims, float* bimage);
+  int32_t comp_image(int64_t* dims, float* bimage, CarmaObj<float>* d_bincube);
+  int32_t comp_image_2D(int64_t* dims, float* bimage, int32_t* num_ssp);
+};
+
 enum class SlopeOrder { untied = 0, interlaced = 1 };
 
 template <class Tin, class Tout>

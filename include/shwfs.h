@@ -190,6 +190,22 @@ int32_t shwfs_digitalize(uint16_t* d_cam, const float* d_img, const float* d_dar
                          const float* d_flat, int32_t max_flux_per_pix, int32_t max_pix_value,
                          int64_t n, void* stream);
 
+/* ---- external detector image -> spots (acquisim) -----------------------------
+ * bincube[s*npix^2 + x + y*npix] = image[(x + validx[s]) + (y + validy[s]) * size]
+ * (size: the image side, npix * nxsub; 0 for an element past size^2, which the
+ * reference does not guard); replaces bcube_krnl, libsutra/src.cu/
+ * sutra_acquisim.cu:65-87 (SutraAcquisim::comp_image, src.cpp/
+ * sutra_acquisim.cpp:52-66). */
+int32_t shwfs_fill_bincube(const float* d_image, float* d_bincube, const int32_t* d_validx,
+                           const int32_t* d_validy, int32_t npix, int32_t nvalid, int32_t size,
+                           void* stream);
+/* The same from a sub-aperture number map over the nxsub x nxsub grid:
+ * num_ssp[i + nxsub*j] = 0 for an invalid sub-aperture, else 1 + its index in
+ * the bincube (entries above nvalid are skipped); replaces bcube_krnl_2D,
+ * libsutra/src.cu/sutra_acquisim.cu:26-48 (SutraAcquisim::comp_image_2D). */
+int32_t shwfs_fill_bincube_2d(const float* d_image, float* d_bincube, const int32_t* d_num_ssp,
+                              int32_t npix, int32_t nxsub, int32_t nvalid, void* stream);
+
 /* ---- geometric slopes (mean phase gradient per sub-aperture) -----------------
  * type 0: edge differences, reduce_phasex/y (libsutra/src.cu/sutra_wfs.cu:868-913);
  * type 2: projection on ttprojmat (1036-1090).  phase in micrometres; phasemap
