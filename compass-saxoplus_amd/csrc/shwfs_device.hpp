@@ -9,6 +9,8 @@
 //  * In-register radix-2 FFT of compile-time size with compile-time twiddles.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -394,6 +396,28 @@ __device__ __forceinline__ void wave_lds_sync() {
 // that order, the rows reduce in registers and the totals are read from
 // lanes 0, 16 and 32.  (The order is that of 16 lanes each accumulating the
 // values of lanes r, r + 16, ... in turn, then one row sum.)
+// Sums of G <= 16 groups of 16 doubles in LDS (red[16 g + r]): four partial
+// sums in turn, s_k = ((x_4k + x_4k+1) + x_4k+2) + x_4k+3, then
+// (s_0 + s_1) + (s_2 + s_3) through two quad DPP steps; lane 4 g returns
+// group g's sum.  The one-sub-aperture and four-sub-aperture tails both sum
+// their float64 moments this way (so they agree bit for bit), with three
+// instructions per DPP step of a double against twelve for a 16-lane tree.
+__device__ __forceinline__ double sum16_quads(const double* red, int lane, int G) {
+  double s = 0.0;
+  if (lane < 4 * G) {
+    const double* p = red + 4 * lane;  // group lane / 4, quarter lane % 4
+    s = ((p[0] + p[1]) + p[2]) + p[3];
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {  // lane ^ 1, then lane ^ 2
+    const long long b = __double_as_longlong(s);
+    const int lo = dpp_row_perm((int)(b & 0xffffffffLL), c);
+    const int hi = dpp_row_perm((int)(b >> 32), c);
+    s += __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  return s;
+}
+
 template <typename T>
 __device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
   red[lane] = a;
@@ -405,10 +429,21 @@ __device__ __forceinline__ void wave_sum3(T* red, int lane, T& a, T& b, T& c) {
     const T* p = red + (lane >> 4) * 64 + (lane & 15);
     v = ((p[0] + p[16]) + p[32]) + p[48];
   }
-  v = row_sum(v);
-  a = lane_value(v, 0);
-  b = lane_value(v, 16);
-  c = lane_value(v, 32);
+  if constexpr (std::is_same<T, double>::value) {
+    // the 3 x 16 partials (lane 16 q + r) through sum16_quads
+    wave_lds_sync();
+    if (lane < 48) red[lane] = v;
+    wave_lds_sync();
+    v = sum16_quads(red, lane, 3);
+    a = lane_value(v, 0);
+    b = lane_value(v, 4);
+    c = lane_value(v, 8);
+  } else {
+    v = row_sum(v);
+    a = lane_value(v, 0);
+    b = lane_value(v, 16);
+    c = lane_value(v, 32);
+  }
 }
 
 __device__ __forceinline__ float wave_max(float v) {

@@ -524,6 +524,9 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
         }
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
         b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
+        // both sums with every lane active (two DPP adds): kept from being moved
+        // into the store's lane mask, where the second needs a separate DPP move
+        asm volatile("" : "+v"(b));
         if constexpr (HALF) {
           // the binned cell (cx, cy) = (kx / 4, ky / 4) is pixel (cx ^ H, cy ^ H):
           // a per-lane base plus a constant offset per tile pair
@@ -596,11 +599,12 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
 }
 
 // LDS floats per wave of dft_subaps: the |F|^2 image (nr == 0), then SPW
-// spots (16-byte aligned), then SPW spot scales.
+// spots (16-byte aligned), then SPW spot scales, then 192 doubles for the
+// float64 moment sums of the shared tail.
 template <int N, int Q>
 __host__ __device__ inline int subaps_floats(int npix, bool gather, int spw) {
   const int spot = (npix * npix + 3) & ~3;
-  return (gather ? N * N : 0) + spw * spot + ((spw + 3) & ~3);
+  return (gather ? N * N : 0) + spw * spot + ((spw + 3) & ~3) + 384;
 }
 
 // Sub-apertures s0 .. s0 + SPW - 1 (those < nvalid) by one wave: their spots
@@ -654,7 +658,8 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
   wave_lds_sync();
   const float* sp = spots + q * spot_f;
   const float I = spot_sum<16>(sp, npix2, valid, r);
-  finish_spot<16, NOISE>(P, s, valid, r, sp, I, scales[q], call_add, nullptr, &pre);
+  double* red = reinterpret_cast<double*>(scales + ((SPW + 3) & ~3));  // 8-byte aligned
+  finish_spot<16, NOISE>(P, s, valid, r, sp, I, scales[q], call_add, red, &pre);
   wave_lds_sync();
 }
 

@@ -165,6 +165,36 @@ __device__ __forceinline__ void centroid_finish64(const ImageParams& P, int s, b
   }
 }
 
+// Four sub-apertures per wave, 16 lanes each (group q = wave lane / 16),
+// float64 moments: the 12 x 16 partials go through LDS (red: 192 doubles) and
+// are summed by sum16_quads, the order of the one-sub-aperture tail
+// (wave_sum3); lanes 0 and 1 of each group divide and write the x and y
+// slopes.  r: the lane within its group.
+__device__ __forceinline__ void centroid_finish16x4(const ImageParams& P, int s, bool valid, int r,
+                                                    double I, double Sx, double Sy, double* red,
+                                                    float rx, float ry) {
+  const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const int q = lane >> 4;
+  red[(3 * q) * 16 + r] = I;
+  red[(3 * q + 1) * 16 + r] = Sx;
+  red[(3 * q + 2) * 16 + r] = Sy;
+  wave_lds_sync();
+  const double t = sum16_quads(red, lane, 12);
+  wave_lds_sync();
+  if (lane < 48 && (lane & 3) == 0) red[lane >> 2] = t;  // group g = 3 q + moment
+  wave_lds_sync();
+  I = red[3 * q];
+  Sx = red[3 * q + 1];
+  Sy = red[3 * q + 2];
+  if (valid && r < 2) {
+    const int iy = r;  // lane 0: x slope, lane 1: y slope
+    const int ix = P.slope_order == 0 ? s + iy * P.nvalid : 2 * s + iy;
+    const double num = iy ? Sy : Sx;
+    P.slopes[ix] = ((float)(num / (I + 1.e-6)) - P.coffset) * P.cscale - (iy ? ry : rx);
+    if (iy == 0) P.cintens[s] = (float)I;
+  }
+}
+
 // a[i] = v through a 32-bit byte offset from a base held in scalar registers
 // (the arrays written here are under 4 GiB: i < 2^30).
 __device__ __forceinline__ void store_at(float* a, unsigned i, float v) {
@@ -349,6 +379,12 @@ __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool va
     } else if (valid && P.ref && lane < 2) {
       rx = P.ref[P.slope_order == 0 ? s : 2 * s];
       ry = P.ref[P.slope_order == 0 ? s + P.nvalid : 2 * s + 1];
+    }
+    if constexpr (LANES == 16) {
+      if (red != nullptr && P.fp64) {
+        centroid_finish16x4(P, s, valid, lane, dI, dX, dY, static_cast<double*>(red), rx, ry);
+        return;
+      }
     }
     if (P.fp64)
       centroid_finish<LANES, double>(P, s, valid, lane, dI, dX, dY, rx, ry);
