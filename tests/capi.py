@@ -63,6 +63,9 @@ def load():
     lib.shwfs_calibrate.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
     lib.shwfs_calibrate_u16.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
     lib.shwfs_digitalize.argtypes = [vp, vp, vp, vp, i32, i32, C.c_int64, vp]
+    lib.shwfs_centroids_bpcog.argtypes = [vp, i32, C.POINTER(CentroidArgs), vp]
+    lib.shwfs_fill_bincube.argtypes = [vp, vp, vp, vp, i32, i32, i32, vp]
+    lib.shwfs_fill_bincube_2d.argtypes = [vp, vp, vp, i32, i32, i32, vp]
     lib.shwfs_slopes_geom.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp]
     lib.shwfs_mfma_probe.argtypes = [vp, vp, vp, i32, vp]
     lib.shwfs_batch_create.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), i32,

@@ -50,6 +50,29 @@ def test_null_and_size_arguments_rejected_without_launch(lib):
     assert lib.shwfs_mfma_probe(None, None, None, 1, None) == capi.SHWFS_ERR_ARG
 
 
+def test_camera_bpcog_and_acquisim_arguments_rejected_without_launch(lib):
+    dev = C.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    assert lib.shwfs_calibrate_u16(None, None, None, None, None, 4, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_digitalize(None, dev, dev, dev, 100, 65535, 4, None) == capi.SHWFS_ERR_ARG
+    # max_flux_per_pix must be positive (it divides)
+    assert lib.shwfs_digitalize(dev, dev, dev, dev, 0, 65535, 4, None) == capi.SHWFS_ERR_ARG
+    a = capi.CentroidArgs()
+    a.nvalid, a.npix, a.size = 4, 8, 64
+    assert lib.shwfs_centroids_bpcog(None, 3, C.byref(a), None) == capi.SHWFS_ERR_ARG
+    # nbpix outside [0, npix^2)
+    assert lib.shwfs_centroids_bpcog(dev, 64, C.byref(a), None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_centroids_bpcog(dev, -1, C.byref(a), None) == capi.SHWFS_ERR_ARG
+    # windows above 1024 pixels
+    a.npix = 33
+    assert lib.shwfs_centroids_bpcog(dev, 3, C.byref(a), None) == capi.SHWFS_ERR_UNSUPPORTED
+    a.npix = 8
+    # valid positions and outputs must be set
+    assert lib.shwfs_centroids_bpcog(dev, 3, C.byref(a), None) == capi.SHWFS_ERR_STATE
+    assert lib.shwfs_fill_bincube(None, dev, dev, dev, 8, 4, 64, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_fill_bincube(dev, dev, dev, dev, 0, 4, 64, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_fill_bincube_2d(dev, dev, None, 8, 8, 4, None) == capi.SHWFS_ERR_ARG
+
+
 @pytest.mark.parametrize("Ntot,npix,nrebin", [(16, 8, 2), (32, 8, 4), (16, 6, 2), (64, 16, 4), (16, 5, 3)])
 def test_binmap_block_analysis_recognises_the_ngs_binmap(lib, Ntot, npix, nrebin):
     bm = np.ascontiguousarray(og.binmap_table(Ntot, npix, nrebin).T)  # [p][i] = F-order of [i, p]
