@@ -7,7 +7,11 @@ with float64 moments, threshold 0.1 * max): phase screen -> spots -> detector
 image -> slopes, inputs resident in HBM.  Frames are issued back to back by a
 native loop; wall clock around K frames, synchronised on both sides.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: the metric and its value (frames/s of the best method,
+one frame per launch), every method's rate and error against the float64
+oracle, the roofline of the spot kernel (algorithmic bytes over the measured
+kernel time, HBM traffic from the committed rocprofv3 summary), the matrix-core
+use, the batched and uneven-batch forms, and the float64 CPU oracle's rate.
 """
 import argparse
 import json

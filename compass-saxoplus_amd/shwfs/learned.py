@@ -4,7 +4,7 @@ the geometric slopes of the same phase (mean gradient, arcsec).
 
 The network predicts the spot displacement in pixels; slopes are that times
 the pixel scale.  The CoG of the same pixels is both the fallback (spot flux
-outside the training range) and the judge.
+outside the training range) and the baseline its slope error is compared with.
 """
 import numpy as np
 
