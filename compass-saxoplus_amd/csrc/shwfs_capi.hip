@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void bpcog_kernel(float* img, int nbpix, shwfs
   }
   for (int p = lane; p < n2; p += 64) {
     const long idim = (long)(p % npix + xv) + (long)(p / npix + yv) * a.size;
-    v[p] = (valid && idim < lim) ? img[idim] : 0.0f;
+    v[p] = (valid && idim >= 0 && idim < lim) ? img[idim] : 0.0f;
   }
   if (lane == 0) v[n2] = 0.0f;  // threshold when no value qualifies (NaN in the window)
   wave_lds_sync();
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void bpcog_kernel(float* img, int nbpix, shwfs
     for (int p = lane; p < n2; p += 64) {
       const int x = p % npix, y = p / npix;
       const long idim = (long)(x + xv) + (long)(y + yv) * a.size;
-      if (idim < lim) {
+      if (idim >= 0 && idim < lim) {
         const float d = v[p] > thr ? v[p] - thr : 0.0f;
         img[idim] = d;
         I += d;

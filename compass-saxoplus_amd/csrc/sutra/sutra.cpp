@@ -763,6 +763,9 @@ int32_t SutraAcquisim::comp_image(int64_t* dims, float* bimage, CarmaObj<float>*
     throw std::runtime_error("Acquisim: valid positions or bincube not set");
   if (d_bincube->get_nb_elements() < nvalid * npix * npix)
     throw std::runtime_error("Acquisim: bincube smaller than nvalid * npix^2");
+  // the kernel reads elements below (npix * nxsub)^2 of the image
+  if (dims[0] < 1 || dims[1] * (dims[0] >= 2 ? dims[2] : 1) < (npix * nxsub) * (npix * nxsub))
+    throw std::runtime_error("Acquisim.comp_image: image smaller than (npix * nxsub)^2");
   CarmaObj<float> img(current_context, dims, bimage);
   shwfs_throw_if(shwfs_fill_bincube(img.get_data(), d_bincube->get_data(),
                                     d_validsubsx->get_data(), d_validsubsy->get_data(),
