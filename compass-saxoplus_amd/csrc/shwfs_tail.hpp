@@ -278,7 +278,10 @@ __device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lan
     constexpr int NP2 = NPIX * NPIX, DY = LANES / NPIX;
     const int px = lane % NPIX, py0 = lane / NPIX;
     const unsigned e0 = (unsigned)(lane + s * NP2);
-    const unsigned img0 = (unsigned)(vx + px) + __umul24(vy + py0, P.nsub);
+    unsigned img0 = (unsigned)(vx + px) + __umul24(vy + py0, P.nsub);
+    // held as computed: the row step stays one add per pixel instead of being
+    // refolded into a 32-bit multiply (a quarter-rate instruction) per pixel
+    asm volatile("" : "+v"(img0));
     const unsigned dimg = __umul24(DY, P.nsub);
 #pragma unroll
     for (int j = 0; j * LANES < NP2; ++j)
