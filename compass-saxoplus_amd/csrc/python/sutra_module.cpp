@@ -313,6 +313,14 @@ static void declare_centroiders(py::module& m, const std::string& sfx) {
              return c.set_centroids_ref(r.mutable_data());
            },
            py::arg("refslopes"))
+      .def_property(
+          "slope_order", [](Centro& c) { return (int)c.slope_order; },
+          [](Centro& c, int o) {
+            if (o != 0 && o != 1) throw std::invalid_argument("slope_order: 0 (untied) or 1 (interlaced)");
+            c.slope_order = static_cast<SlopeOrder>(o);
+          },
+          "slope layout (the C++ member of libsutra/include/sutra_centroider.hpp:31-53): 0 untied "
+          "(x slopes then y slopes), 1 interlaced (x, y per sub-aperture)")
       .def("set_scale", &Centro::set_scale, py::arg("scale"))
       .def("set_offset", &Centro::set_offset, py::arg("offset"))
       .def("set_dark",
@@ -506,6 +514,16 @@ PYBIND11_MODULE(sutra, m) {
         py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
         py::arg("noise") = true, py::arg("fused") = true,
         "issues nframes frames back to back; returns hipEvent-timed milliseconds");
+  m.def("run_frames_graph",
+        [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, int32_t n, int32_t reps) {
+          if (slopes.get_nb_elements() < 2 * w.nvalid) throw std::runtime_error("run_frames_graph: slope vector too short");
+          py::gil_scoped_release nogil;
+          return run_frames_graph(&w, &c, slopes.get_data(), n, reps);
+        },
+        py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
+        py::arg("reps"),
+        "noise-free fused frames captured in a hipGraph of nframes launches, replayed reps "
+        "times; returns hipEvent-timed milliseconds of the reps replays");
   py::class_<SutraAcquisim>(m, "Acquisim")
       .def(py::init([](SutraSensors& sensors, int32_t wfs_num) {
              return new SutraAcquisim(&sensors, wfs_num);

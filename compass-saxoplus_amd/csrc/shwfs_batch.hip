@@ -31,7 +31,8 @@ struct shwfs_batch {
   // same-kind kernel (method >= 0) instead of the per-kind dispatch kernel
   int uni_method = -1, uni_nfft = 0, uni_pdiam = 0, uni_nr = 0;
   bool uni_full = false;
-  size_t lds = 0;
+  size_t lds = 0;  // bytes of LDS per wave
+  int waves = 1;   // waves per workgroup of the mixed kernel (kBatchWaves when the slots fit)
   uint32_t* d_frame = nullptr;
   hipStream_t own_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -172,6 +173,9 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   }
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "batch: LDS footprint exceeds the per-workgroup limit");
+  // waves per workgroup of the mixed kernel: kBatchWaves slots if they fit
+  int waves = kBatchWaves;
+  while (waves > 1 && lds * waves > max_lds_per_block()) waves /= 2;
   // heaviest tiles first (they start first and the light ones fill in)
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
@@ -191,6 +195,7 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   b->ntasks = n;
   b->total_tiles = first;
   b->lds = lds;
+  b->waves = waves;
   int set = 0;
   for (const auto& t : sorted) set |= batch_kind_set(t.kind);
   b->kind_set = (set == BKS_FFT || set == BKS_BF16 || set == BKS_FP16) ? set : BKS_ALL;
@@ -232,7 +237,9 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
       if (rc) return rc;
     }
   }
-  const int grid = b->grid > 0 ? std::min(b->grid, b->total_tiles) : b->total_tiles;
+  // mixed kernel: workgroups of b->waves waves, one tile per wave
+  const int wgs = (b->total_tiles + b->waves - 1) / b->waves;
+  const int grid = b->grid > 0 ? std::min(b->grid, wgs) : wgs;
   if (b->uni_method >= 0)
     return launch_dft_tasks(b->uni_method, b->uni_nfft, b->uni_pdiam, b->uni_full, b->uni_nr,
                             b->noisy, b->d_tasks, b->starts, b->args, b->ntasks, b->total_tiles, grid,
@@ -240,11 +247,9 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
                             b->d_frame, frame_off, b->remap, st);
   if (b->noisy)
     return launch_batch_noisy(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
-                              b->lds,
-                              b->d_frame, frame_off, b->remap, st);
+                              b->waves, b->lds, b->d_frame, frame_off, b->remap, st);
   return launch_batch_quiet(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
-                            b->lds,
-                            b->d_frame, frame_off, b->remap, st);
+                            b->waves, b->lds, b->d_frame, frame_off, b->remap, st);
 }
 
 int32_t advance(shwfs_batch* b, uint32_t n, hipStream_t st) {
@@ -256,8 +261,8 @@ int32_t advance(shwfs_batch* b, uint32_t n, hipStream_t st) {
 int resident_blocks(shwfs_batch* b) {
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-  const int per = b->noisy ? batch_blocks_per_cu_noisy(b->kind_set, b->lds)
-                           : batch_blocks_per_cu_quiet(b->kind_set, b->lds);
+  const int per = b->noisy ? batch_blocks_per_cu_noisy(b->kind_set, b->waves, b->lds)
+                           : batch_blocks_per_cu_quiet(b->kind_set, b->waves, b->lds);
   return cus * (per > 0 ? per : 1);
 }
 

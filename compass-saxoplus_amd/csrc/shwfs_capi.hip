@@ -197,6 +197,9 @@ __global__ __launch_bounds__(256) void fill_bincube_2d_kernel(const float* image
 __global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype, float alpha,
                                                    float beta, unsigned long long seed,
                                                    unsigned call, unsigned domain) {
+  // no fused multiply-add: beta * d + alpha * N(0, 1) rounds like the spot
+  // kernels' readout-noise sum, so the two agree bit for bit
+#pragma clang fp contract(off)
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
     const unsigned e = (unsigned)i;
     if (gtype == 'P') {
@@ -255,12 +258,17 @@ __global__ __launch_bounds__(256) void digitalize_kernel(uint16_t* cam, const fl
   }
 }
 
-// Geometric slopes, one group of 64 lanes per sub-aperture.
+// Geometric slopes, one group of 64 lanes per sub-aperture; phi(x, y) =
+// phase[pmap[x + y n + s n^2]].
 // type 0: slope_x = alpha/n * sum_y (phi(n-1, y) - phi(0, y)), same along y.
+// type 1: slope_x = alpha/(2 n flux[s]) * sum_{x,y} D_x phi(x, y) * pupil(x, y),
+//         D_x the central difference, one-sided (not halved) at x = 0, n-1.
 // type 2: slope_x = sum_k ttprojmat[k, s] * phi(k), slope_y with column s + nvalid.
-__global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float* phase, float* slopes,
-                                                          const int* pmap, const float* proj,
-                                                          int n, int nvalid, float alpha) {
+__global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float* phase,
+                                                          const float* pupil, const float* flux,
+                                                          float* slopes, const int* pmap,
+                                                          const float* proj, int n, int nvalid,
+                                                          float alpha) {
   const int slot = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int s = blockIdx.x * 4 + slot;
   const bool valid = s < nvalid;
@@ -272,6 +280,15 @@ __global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float*
       for (int t = lane; t < n; t += 64) {
         ax += phase[m[t * n + n - 1]] - phase[m[t * n]];
         ay += phase[m[t + (n - 1) * n]] - phase[m[t]];
+      }
+    } else if (type == 1) {
+      for (int k = lane; k < n2; k += 64) {
+        const int x = k % n, y = k / n;
+        const float w = pupil[m[k]];
+        const int xp = x + 1 < n ? k + 1 : k, xm = x > 0 ? k - 1 : k;
+        const int yp = y + 1 < n ? k + n : k, ym = y > 0 ? k - n : k;
+        ax += (phase[m[xp]] - phase[m[xm]]) * w;
+        ay += (phase[m[yp]] - phase[m[ym]]) * w;
       }
     } else {
       const float* px = proj + (size_t)s * n2;
@@ -289,10 +306,60 @@ __global__ __launch_bounds__(256) void slopes_geom_kernel(int type, const float*
     if (type == 0) {
       slopes[s] = ax / n * alpha;
       slopes[s + nvalid] = ay / n * alpha;
+    } else if (type == 1) {
+      slopes[s] = ax / n * alpha / flux[s] / 2;
+      slopes[s + nvalid] = ay / n / 2 * alpha / flux[s];
     } else {
       slopes[s] = ax;
       slopes[s + nvalid] = ay;
     }
+  }
+}
+
+// Tip/tilt removal of a slope vector by one workgroup: both projections are
+// taken on the unfiltered slopes, then both are subtracted.
+constexpr int kTtThreads = 1024;
+__global__ __launch_bounds__(kTtThreads) void tt_filter_kernel(float* c, const float* tip,
+                                                               const float* tilt, float* tt,
+                                                               int n) {
+  __shared__ float red[2][kTtThreads / 64];
+  float a = 0.f, b = 0.f;
+  for (int i = threadIdx.x; i < n; i += kTtThreads) {
+    const float v = c[i];
+    a += v * tip[i];
+    b += v * tilt[i];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+  }
+  __syncthreads();
+  a = 0.f;
+  b = 0.f;
+  for (int k = 0; k < kTtThreads / 64; ++k) {
+    a += red[0][k];
+    b += red[1][k];
+  }
+  for (int i = threadIdx.x; i < n; i += kTtThreads) c[i] = (c[i] - a * tip[i]) - b * tilt[i];
+  if (threadIdx.x == 0 && tt) {
+    tt[0] = a;
+    tt[1] = b;
+  }
+}
+
+// Spots into the detector image: img[(vx[s] + px) + (vy[s] + py) * nsub] =
+// bincube[px + py npix + s npix^2], one thread per spot pixel.
+__global__ __launch_bounds__(256) void fill_binimg_kernel(const float* bcube, float* img,
+                                                          const int* vx, const int* vy, int npix,
+                                                          long n, int nsub) {
+  const int npix2 = npix * npix;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256L) {
+    const int s = (int)(t / npix2), p = (int)(t - (long)s * npix2);
+    const int px = p % npix, py = p / npix;
+    img[(long)(vx[s] + px) + (long)(vy[s] + py) * nsub] = bcube[t];
   }
 }
 
@@ -583,16 +650,70 @@ int32_t shwfs_digitalize(uint16_t* d_cam, const float* d_img, const float* d_dar
   return SHWFS_OK;
 }
 
-int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
-                          const int32_t* d_phasemap, const float* d_ttprojmat, int32_t nphase,
-                          int32_t nvalid, float alpha, void* stream) {
+int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, const float* d_pupil,
+                          const float* d_fluxPerSub, float* d_slopes, const int32_t* d_phasemap,
+                          const float* d_ttprojmat, int32_t nphase, int32_t nvalid, float alpha,
+                          void* stream) {
   if (!d_phase || !d_slopes || !d_phasemap || nphase < 2 || nvalid <= 0)
     return fail(SHWFS_ERR_ARG, "slopes_geom: bad args");
-  if (type != 0 && type != 2) return fail(SHWFS_ERR_UNSUPPORTED, "slopes_geom: type must be 0 or 2");
+  if (type < 0 || type > 2) return fail(SHWFS_ERR_ARG, "slopes_geom: type must be 0, 1 or 2");
+  if (type == 1 && (!d_pupil || !d_fluxPerSub))
+    return fail(SHWFS_ERR_ARG, "slopes_geom: type 1 needs the pupil and fluxPerSub");
   if (type == 2 && !d_ttprojmat) return fail(SHWFS_ERR_ARG, "slopes_geom: type 2 needs ttprojmat");
   hipLaunchKernelGGL(slopes_geom_kernel, dim3((nvalid + 3) / 4), dim3(256), 0, (hipStream_t)stream,
-                     type, d_phase, d_slopes, d_phasemap, d_ttprojmat, nphase, nvalid, alpha);
+                     type, d_phase, d_pupil, d_fluxPerSub, d_slopes, d_phasemap, d_ttprojmat,
+                     nphase, nvalid, alpha);
   SHWFS_LAUNCHED("slopes_geom_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_tt_filter(float* d_slopes, const float* d_ref_tip, const float* d_ref_tilt,
+                        float* d_tt, int32_t nslopes, void* stream) {
+  if (!d_slopes || !d_ref_tip || !d_ref_tilt || nslopes < 0)
+    return fail(SHWFS_ERR_ARG, "tt_filter: bad args");
+  if (nslopes == 0) return SHWFS_OK;
+  hipLaunchKernelGGL(tt_filter_kernel, dim3(1), dim3(kTtThreads), 0, (hipStream_t)stream, d_slopes,
+                     d_ref_tip, d_ref_tilt, d_tt, (int)nslopes);
+  SHWFS_LAUNCHED("tt_filter_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_fill_binimg(const float* d_bincube, float* d_binimg, const int32_t* d_validsubsx,
+                          const int32_t* d_validsubsy, int32_t npix, int32_t nvalid, int32_t nsub,
+                          float noise_background, uint64_t seed, uint32_t call, uint32_t domain,
+                          void* stream) {
+  if (!d_bincube || !d_binimg || !d_validsubsx || !d_validsubsy || npix <= 0 || nvalid < 0 ||
+      nsub <= 0)
+    return fail(SHWFS_ERR_ARG, "fill_binimg: bad args");
+  if ((long long)nsub * nsub >= (1LL << 31)) return fail(SHWFS_ERR_UNSUPPORTED, "fill_binimg: image too large");
+  hipStream_t st = (hipStream_t)stream;
+  if (noise_background > 0.0f) {
+    int32_t rc = launch_background(d_binimg, (long)nsub * nsub, noise_background, seed, call, st,
+                                   nullptr, domain);
+    if (rc) return rc;
+  }
+  const long n = (long)nvalid * npix * npix;
+  if (n == 0) return SHWFS_OK;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_binimg_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d_bincube,
+                     d_binimg, d_validsubsx, d_validsubsy, npix, n, nsub);
+  SHWFS_LAUNCHED("fill_binimg_kernel");
+  return SHWFS_OK;
+}
+
+int32_t shwfs_upload(void* d_dst, int64_t dst_elems, const void* h_src, int64_t src_elems,
+                     int32_t elem_size, void* stream) {
+  if (src_elems != dst_elems)
+    return (set_error("upload: %lld elements given, the device array holds %lld",
+                      (long long)src_elems, (long long)dst_elems),
+            SHWFS_ERR_ARG);
+  if (!d_dst || !h_src || dst_elems < 0 || elem_size <= 0)
+    return fail(SHWFS_ERR_ARG, "upload: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  SHWFS_HIP(hipMemcpyAsync(d_dst, h_src, (size_t)dst_elems * (size_t)elem_size,
+                           hipMemcpyHostToDevice, st));
+  SHWFS_HIP(hipStreamSynchronize(st));
   return SHWFS_OK;
 }
 

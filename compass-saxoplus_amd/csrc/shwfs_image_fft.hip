@@ -57,18 +57,19 @@ int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t st) {
 // ---- background readout noise of the detector image: img = sigma * N(0,1)
 __global__ __launch_bounds__(256) void background_kernel(float* img, long n, float sigma,
                                                          unsigned long long seed, unsigned call,
-                                                         const unsigned* call_base) {
+                                                         const unsigned* call_base,
+                                                         unsigned domain) {
   if (call_base) call += call_base[0];
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L)
-    img[i] = sigma * (float)rng_normal((unsigned)i, call, 2u, seed);
+    img[i] = sigma * (float)rng_normal((unsigned)i, call, domain, seed);
 }
 
 int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
-                          hipStream_t st, const unsigned* call_base) {
+                          hipStream_t st, const unsigned* call_base, unsigned domain) {
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(background_kernel, dim3((unsigned)blocks), dim3(256), 0, st, img, n, sigma,
-                     seed, call, call_base);
+                     seed, call, call_base, domain);
   SHWFS_LAUNCHED("background_kernel");
   return SHWFS_OK;
 }

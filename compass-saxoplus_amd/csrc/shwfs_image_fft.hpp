@@ -39,22 +39,35 @@ struct FftLayout {
   }
 };
 
-// One tile: the TPB / N sub-apertures s0, s0 + 1, ... of a workgroup of TPB
-// lanes (group `slot` of N lanes owns s0 + slot).  smem: the workgroup's LDS
-// (S slots of slot_floats, then the binmap if bm is not the global table).
-// Contains workgroup barriers: every lane of the workgroup calls it.
-// call_add: added to the Philox call counter of the noise.
+// LDS hand-over between the lanes of one tile: a workgroup barrier, or for a
+// tile of one wave (TPB == 64) the wave's own ordering (other waves of the
+// workgroup, working on other tiles, are not waited for).
+template <int TPB>
+__device__ __forceinline__ void tile_sync() {
+  if constexpr (TPB == 64)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+// One tile: the TPB / N sub-apertures s0, s0 + 1, ... of TPB lanes (group
+// `slot` of N lanes owns s0 + slot); tid: the lane's index in the tile (the
+// thread index, or the lane in the wave for one-wave tiles).  smem: the
+// tile's LDS (S slots of slot_floats, then the binmap if bm is not the global
+// table).  Contains tile barriers (tile_sync): every lane of the tile calls
+// it.  call_add: added to the Philox call counter of the noise.
 template <int N, int R, int TPB, bool NOISE>
 __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* smem,
-                                         const int* bm, unsigned call_add = 0u) {
+                                         const int* bm, unsigned call_add = 0u,
+                                         int tid = (int)threadIdx.x) {
   using L = FftLayout<N, R>;
   constexpr int M = L::M;
   constexpr int PS = L::PS;
   const int pdiam = P.pdiam;
   const int PE = L::pe(pdiam);
   const int slot_f = L::slot_floats(pdiam, P.npix);
-  const int slot = threadIdx.x / N;
-  const int lane = threadIdx.x % N;
+  const int slot = tid / N;
+  const int lane = tid % N;
   const int s = s0 + slot;
   const bool valid = s < P.nvalid;
   float* base = smem + slot * slot_f;
@@ -65,7 +78,7 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
 
   // ---- 0. fields
   load_fields<N, L::EPL, false>(P, s, valid, lane, (float*)E, PE, 0);
-  __syncthreads();
+  tile_sync<TPB>();
 
   // ---- 1. row transforms
   float re[M], im[M];
@@ -83,12 +96,12 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
     if (r) PreTw<N, M, 0>::run(re, im);
   }
   fft_reg<M>(re, im);
-  __syncthreads();
+  tile_sync<TPB>();
   if (row_active) {
 #pragma unroll
     for (int k = 0; k < M; ++k) G[(R * k + r) * PS + y] = make_float2(re[k], im[k]);
   }
-  __syncthreads();
+  tile_sync<TPB>();
 
   // ---- 2. column transforms (lane = kx)
   float gr[M], gi[M];
@@ -99,7 +112,7 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
     gr[yy] = v.x;
     gi[yy] = v.y;
   }
-  __syncthreads();
+  tile_sync<TPB>();
 #pragma unroll
   for (int rr = 0; rr < R; ++rr) {
     float zr[M], zi[M];
@@ -115,11 +128,11 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
       for (int k = 0; k < M; ++k) hr[lane + (R * k + rr) * N] = zr[k] * zr[k] + zi[k] * zi[k];
     }
   }
-  __syncthreads();
+  tile_sync<TPB>();
 
   // ---- 3. tail
   spot_tail<N, N, NOISE>(P, s, valid, lane, hr, bin, bm, 1.0f, call_add);
-  __syncthreads();
+  tile_sync<TPB>();
 }
 
 }  // namespace shwfs

@@ -155,10 +155,12 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s);
 int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, hipStream_t s);
-// img = sigma * N(0,1); the Philox call is `call` plus *call_base when
-// call_base is not NULL (device frame counter of a batch)
+// img = sigma * N(0,1) (Philox domain `domain`, element = image index); the
+// call is `call` plus *call_base when call_base is not NULL (device frame
+// counter of a batch).  The spot path draws its background with domain 2.
 int32_t launch_background(float* img, long n, float sigma, unsigned long long seed, unsigned call,
-                          hipStream_t s, const unsigned* call_base = nullptr);
+                          hipStream_t s, const unsigned* call_base = nullptr,
+                          unsigned domain = 2u);
 // Same-kind batches of the MFMA stage: every task uses one (nfft, pdiam
 // class, method, full window, rebin) variant; one single-wave workgroup per
 // sub-aperture of the flattened task range.

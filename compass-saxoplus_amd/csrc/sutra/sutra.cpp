@@ -190,13 +190,13 @@ int32_t SutraWfs::set_pupil(float* pupil) {
   return rc;
 }
 
-// reference: libsutra/src.cpp/sutra_wfs.cpp:247-261 (wrong size only prints)
+// reference: libsutra/src.cpp/sutra_wfs.cpp:253-261 (there a wrong size only
+// prints; here it raises with the error of shwfs_upload)
 int32_t SutraWfs::set_binimg(float* binimg, int32_t nElem) {
   current_context->set_active_device(device, 1);
-  if (nElem == d_binimg->get_nb_elements())
-    d_binimg->host2device(binimg);
-  else
-    std::cerr << "SutraWfs::set_binimg: wrong dimension of binimg" << std::endl;
+  shwfs_throw_if(shwfs_upload(d_binimg->get_data(), d_binimg->get_nb_elements(), binimg, nElem,
+                              (int32_t)sizeof(float), nullptr),
+                 "set_binimg");
   return EXIT_SUCCESS;
 }
 
@@ -264,9 +264,10 @@ int32_t SutraWfs::slopes_geom(float* slopes, int32_t type_) {
   current_context->set_active_device(device, 1);
   if (!d_phasemap) throw std::runtime_error("slopes_geom: arrays not loaded");
   float alpha = float(0.206265 / subapd * nphase / (nphase - 1));
-  shwfs_throw_if(shwfs_slopes_geom(type_, d_gs->d_phase->d_screen->get_data(), slopes,
-                                   d_phasemap->get_data(), d_ttprojmat->get_data(),
-                                   (int32_t)nphase, (int32_t)nvalid, alpha, stream),
+  shwfs_throw_if(shwfs_slopes_geom(type_, d_gs->d_phase->d_screen->get_data(), d_pupil->get_data(),
+                                   d_fluxPerSub->get_data(), slopes, d_phasemap->get_data(),
+                                   d_ttprojmat->get_data(), (int32_t)nphase, (int32_t)nvalid,
+                                   alpha, stream),
                  "slopes_geom");
   return EXIT_SUCCESS;
 }
@@ -501,6 +502,12 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
 // One fused launch: fillcamplipup + FFT + abs2c + fillbincube + subap_reduce +
 // subap_norm + prng 'P' / 'N' + (readout background) + fillbinimg
 // (reference sequence: libsutra/src.cpp/sutra_wfs_sh.cpp:337-621).
+// With roket the reference keeps the spots before the noise in
+// d_binimg_notnoisy (572-578): the spot kernel then runs noise-free into that
+// image, the noise is drawn on the spot cube with the same Philox draws the
+// fused kernel takes (domains 0 and 1, element = spot-cube index), and the
+// spots go to d_binimg with the same background (domain 2): the same bincube
+// and binimg as the fused launch, bit for bit.
 int32_t SutraWfsSH::comp_generic() {
   if (d_bincube == nullptr)
     throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
@@ -511,17 +518,51 @@ int32_t SutraWfsSH::comp_generic() {
   current_context->set_active_device(device, 1);
   shwfs_sh_geom g = geom();
   shwfs_frame_args f = frame_args(noise_this_call);
-  shwfs_throw_if(shwfs_sh_comp_image(&g, &f, stream), "SutraWfsSH::comp_image");
+  if (roket && d_binimg_notnoisy != nullptr && d_binimg != nullptr) {
+    shwfs_frame_args clean = f;
+    clean.noise = -1.0f;
+    clean.noise_background = 0.0f;
+    clean.d_binimg = d_binimg_notnoisy->get_data();
+    shwfs_throw_if(shwfs_sh_comp_image(&g, &clean, stream), "SutraWfsSH::comp_image (roket)");
+    const int64_t n = d_bincube->get_nb_elements();
+    if (f.noise > -1.0f)
+      shwfs_throw_if(shwfs_prng(d_bincube->get_data(), n, 'P', 0.f, 1.f, f.seed, f.call, 0u, stream),
+                     "SutraWfsSH::comp_image (roket, photon noise)");
+    if (f.noise > 0.0f)
+      shwfs_throw_if(shwfs_prng(d_bincube->get_data(), n, 'N', f.noise, 1.f, f.seed, f.call, 1u,
+                                stream),
+                     "SutraWfsSH::comp_image (roket, readout noise)");
+    scatter_binimg(f.noise_background, f.call, 2u);
+  } else {
+    shwfs_throw_if(shwfs_sh_comp_image(&g, &f, stream), "SutraWfsSH::comp_image");
+  }
   noise_call++;
   return EXIT_SUCCESS;
 }
 
+// Background readout noise (noise > 0) and the spots of d_bincube into
+// d_binimg (fillbinimg, libsutra/src.cu/sutra_wfs.cu:183-229).
+void SutraWfsSH::scatter_binimg(float background, uint32_t call, uint32_t domain) {
+  shwfs_throw_if(shwfs_fill_binimg(d_bincube->get_data(), d_binimg->get_data(),
+                                   d_validsubsx->get_data(), d_validsubsy->get_data(),
+                                   (int32_t)npix, (int32_t)nvalid, (int32_t)(npix * nxsub),
+                                   background, noise_seed, call, domain, stream),
+                 "fill_binimage");
+}
+
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:599-621: background readout
+// noise when noise > 0, then the spots of d_bincube (e.g. after set_bincube)
+// scattered into d_binimg.  The background of this standalone call is drawn
+// from its own Philox domain (3) at the current frame counter, so it never
+// repeats a frame's own background.  comp_image does not call it: the spot
+// kernel writes the image in the same launch.
 int32_t SutraWfsSH::fill_binimage(int32_t async) {
   (void)async;
-  // comp_generic already scattered the spots (and the background readout
-  // noise when noise > 0) into d_binimg in the same launch.
   if (d_binimg == nullptr)
     throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
+  if (!arrays_loaded) throw std::runtime_error("fill_binimage: load_arrays has not been called");
+  current_context->set_active_device(device, 1);
+  scatter_binimg(noise, noise_call, 3u);
   return EXIT_SUCCESS;
 }
 
@@ -529,9 +570,8 @@ int32_t SutraWfsSH::fill_binimage(int32_t async) {
 int32_t SutraWfsSH::comp_image(bool noise_) {
   current_context->set_active_device(device, 1);
   noise_this_call = noise_;
-  int32_t result = comp_generic();
+  int32_t result = comp_generic();  // includes fill_binimage's image (same launch)
   noise_this_call = true;
-  result *= fill_binimage(0);
   if (fakecam) digitalize_image();  // throws on failure
   return result;
 }
@@ -551,12 +591,13 @@ int32_t SutraWfsSH::comp_nphot(float ittime, float optthroughput, float diam, in
   return EXIT_SUCCESS;
 }
 
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:665-672 (there a wrong size
+// only prints; here it raises with the error of shwfs_upload)
 int32_t SutraWfsSH::set_bincube(float* bincube, int32_t nElem) {
   current_context->set_active_device(device, 1);
-  if (nElem == d_bincube->get_nb_elements())
-    d_bincube->host2device(bincube);
-  else
-    std::cerr << "SutraWfsSH::set_bincube: wrong size of cube" << std::endl;
+  shwfs_throw_if(shwfs_upload(d_bincube->get_data(), d_bincube->get_nb_elements(), bincube, nElem,
+                              (int32_t)sizeof(float), nullptr),
+                 "set_bincube");
   return EXIT_SUCCESS;
 }
 

@@ -189,6 +189,7 @@ class SutraWfsSH : public SutraWfs {
                       float* fluxPerSub, int32_t* validsubsx, int32_t* validsubsy, int32_t* istart,
                       int32_t* jstart, float* ttprojmat, hipFloatComplex* kernel);
   int32_t fill_binimage(int32_t async);
+  void scatter_binimg(float background, uint32_t call, uint32_t domain);
   int32_t comp_image(bool noise = true);
   int32_t comp_nphot(float ittime, float optthroughput, float diam, int32_t nxsub, float zerop = 0,
                      float gsmag = 0, float lgsreturnperwatt = 0, float laserpower = 0);
@@ -508,6 +509,12 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
 // between hipEvents recorded on the WFS stream around the loop.
 double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
                   int32_t nframes, bool noise, bool fused);
+// Captures `nframes` fused frames (one launch each) into a hipGraph on a
+// stream of its own, replays it `reps` times after one untimed replay, and
+// returns the elapsed ms of the timed replays between hipEvents on that
+// stream (frames issued by the device back to back, without host launches).
+double run_frames_graph(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
+                        int32_t nframes, int32_t reps);
 
 // Several SH WFS (uneven sizes) computed by one launch per frame, with their
 // fused centroiders; replaces the per-WFS loop over SutraSensors::d_wfs.

@@ -245,23 +245,21 @@ int32_t SutraCentroider<Tin, Tout>::init_TT_filter() {
   return EXIT_SUCCESS;
 }
 
-// reference: libsutra/src.cpp/sutra_centroider.cpp:322-338.  The projection on
-// the two reference vectors runs on the host here (2*nslopes flops, off the
-// default path: filter_TT is false unless asked for).
+// reference: libsutra/src.cpp/sutra_centroider.cpp:322-338: tip and tilt are
+// both projected from the unfiltered slopes, then both removed; one device
+// launch (no host round trip), d_centro_filtered keeps a copy as the
+// reference's does.
 template <class Tin, class Tout>
 int32_t SutraCentroider<Tin, Tout>::apply_TT_filter(Tout* centroids) {
-  std::vector<float> c(nslopes), tip(nslopes), tilt(nslopes);
-  carma_safe_call(hipMemcpy(c.data(), centroids, sizeof(float) * nslopes, hipMemcpyDeviceToHost));
-  d_ref_Tip->device2host(tip.data());
-  d_ref_Tilt->device2host(tilt.data());
-  float a = 0.f, b = 0.f;
-  for (int32_t i = 0; i < nslopes; ++i) a += c[i] * tip[i];
-  for (int32_t i = 0; i < nslopes; ++i) c[i] -= a * tip[i];
-  for (int32_t i = 0; i < nslopes; ++i) b += c[i] * tilt[i];
-  for (int32_t i = 0; i < nslopes; ++i) c[i] -= b * tilt[i];
-  carma_safe_call(hipMemcpy(centroids, c.data(), sizeof(float) * nslopes, hipMemcpyHostToDevice));
-  float tt[2] = {a, b};
-  d_TT_slopes->host2device(tt);
+  static_assert(std::is_same<Tout, float>::value, "TT filter on float slopes");
+  current_context->set_active_device(device, 1);
+  if (d_ref_Tip == nullptr) init_TT_filter();
+  hipStream_t st = wfs ? wfs->stream : nullptr;
+  shwfs_throw_if(shwfs_tt_filter(centroids, d_ref_Tip->get_data(), d_ref_Tilt->get_data(),
+                                 d_TT_slopes->get_data(), nslopes, st),
+                 "apply_TT_filter");
+  carma_safe_call(hipMemcpyAsync(d_centro_filtered->get_data(), centroids, sizeof(float) * nslopes,
+                                 hipMemcpyDeviceToDevice, st));
   return EXIT_SUCCESS;
 }
 
@@ -588,7 +586,9 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
       throw std::runtime_error("comp_frame: WCoG weights not initialised for this WFS");
   }
   wfs->current_context->set_active_device(wfs->device, 1);
-  if (c->get_type() == "mlp" || c->get_type() == "bpcog") {  // not fused: spots, then centroids
+  // not fused: spots, then centroids (roket keeps the image before the noise,
+  // which the fused launch does not write)
+  if (c->get_type() == "mlp" || c->get_type() == "bpcog" || wfs->roket) {
     wfs->comp_image(noise);
     return c->get_cog(intensities, slopes, noise);
   }
@@ -625,6 +625,57 @@ double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slop
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return ms;
+}
+
+double run_frames_graph(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
+                        int32_t nframes, int32_t reps) {
+  if (nframes <= 0 || reps <= 0) throw std::runtime_error("run_frames_graph: counts must be positive");
+  wfs->current_context->set_active_device(wfs->device, 1);
+  // one frame outside the capture: first-use work (device tables) happens here
+  comp_frame(wfs, c, c->d_intensities->get_data(), slopes, false);
+  carma_safe_call(hipDeviceSynchronize());
+  hipStream_t st = nullptr;
+  carma_safe_call(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipStream_t keep = wfs->stream;
+  wfs->stream = st;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  double ms_total = -1.0;
+  try {
+    carma_safe_call(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int32_t k = 0; k < nframes; ++k) comp_frame(wfs, c, c->d_intensities->get_data(), slopes, false);
+    carma_safe_call(hipStreamEndCapture(st, &g));
+    carma_safe_call(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    carma_safe_call(hipGraphLaunch(ex, st));
+    hipEvent_t e0, e1;
+    carma_safe_call(hipEventCreate(&e0));
+    carma_safe_call(hipEventCreate(&e1));
+    carma_safe_call(hipEventRecord(e0, st));
+    for (int32_t r = 0; r < reps; ++r) carma_safe_call(hipGraphLaunch(ex, st));
+    carma_safe_call(hipEventRecord(e1, st));
+    carma_safe_call(hipEventSynchronize(e1));
+    float ms = 0.f;
+    carma_safe_call(hipEventElapsedTime(&ms, e0, e1));
+    ms_total = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } catch (...) {
+    hipGraph_t dead = nullptr;
+    (void)hipStreamEndCapture(st, &dead);  // no-op unless the capture is still open
+    if (dead) (void)hipGraphDestroy(dead);
+    if (ex) (void)hipGraphExecDestroy(ex);
+    if (g) (void)hipGraphDestroy(g);
+    wfs->stream = keep;
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    throw;
+  }
+  (void)hipGraphExecDestroy(ex);
+  (void)hipGraphDestroy(g);
+  wfs->stream = keep;
+  carma_safe_call(hipStreamSynchronize(st));
+  carma_safe_call(hipStreamDestroy(st));
+  return ms_total;
 }
 
 template class SutraCentroider<float, float>;

@@ -36,7 +36,8 @@ class ShSystem:
         self.tel = self.sutra.Telescope(
             self.ctx, p_geom._spupil.shape[0], int(np.count_nonzero(p_geom._spupil > 0)),
             p_geom._spupil, p_geom._mpupil.shape[0], p_geom._mpupil)
-        self.sensors = wfs_init.wfs_init(self.ctx, self.tel, [p_wfs], p_tel, p_geom)
+        self.sensors = wfs_init.wfs_init(self.ctx, self.tel, [p_wfs], p_tel, p_geom,
+                                         roket=cfg.get("roket", False))
         self.wfs = self.sensors.d_wfs[0]
         if method:
             self.wfs.set_method(method)

@@ -20,7 +20,7 @@ def tel_init(context, p_geom, p_tel, r0, p_wfss, ittime=1.0):
                            p_geom._spupil.astype(np.float32), p_geom._mpupil.shape[0], p_geom._mpupil)
 
 
-def wfs_init(context, telescope, p_wfss, p_tel, p_geom, seeds=None):
+def wfs_init(context, telescope, p_wfss, p_tel, p_geom, seeds=None, roket=False):
     _, sutra = native()
     n = len(p_wfss)
     i64 = lambda f: np.array([f(w) for w in p_wfss], dtype=np.int64)  # noqa: E731
@@ -35,7 +35,7 @@ def wfs_init(context, telescope, p_wfss, p_tel, p_geom, seeds=None):
         np.array([w.fakecam for w in p_wfss], dtype=bool),
         np.array([w.max_flux_per_pix for w in p_wfss], dtype=np.int32),
         np.array([w.max_pix_value for w in p_wfss], dtype=np.int32),
-        context.active_device, False)
+        context.active_device, bool(roket))
     if seeds is None:
         seeds = np.arange(n, dtype=np.int64) + 1234
     sensors.initgs(f32(lambda w: w.xpos), f32(lambda w: w.ypos), f32(lambda w: w.Lambda),

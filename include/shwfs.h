@@ -207,12 +207,51 @@ int32_t shwfs_fill_bincube_2d(const float* d_image, float* d_bincube, const int3
                               int32_t npix, int32_t nxsub, int32_t nvalid, void* stream);
 
 /* ---- geometric slopes (mean phase gradient per sub-aperture) -----------------
+ * Replaces SutraWfs::slopes_geom (libsutra/src.cpp/sutra_wfs.cpp:199-245):
  * type 0: edge differences, reduce_phasex/y (libsutra/src.cu/sutra_wfs.cu:868-913);
- * type 2: projection on ttprojmat (1036-1090).  phase in micrometres; phasemap
- * [nphase^2 * nvalid] flat screen indices; slopes [2*nvalid] (x then y). */
-int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, float* d_slopes,
-                          const int32_t* d_phasemap, const float* d_ttprojmat, int32_t nphase,
-                          int32_t nvalid, float alpha, void* stream);
+ * type 1: pupil-weighted central differences over the window divided by
+ *         2 n fluxPerSub, derive_phasex/y (916-1033; needs d_pupil, the
+ *         screen-sized pupil, and d_fluxPerSub [nvalid]);
+ * type 2: projection on ttprojmat (1036-1090; needs d_ttprojmat
+ *         [nphase^2 * 2 nvalid]).
+ * phase in micrometres; phasemap [nphase^2 * nvalid] flat screen indices;
+ * slopes [2*nvalid] (x then y); alpha as the reference (0.206265 / subapd *
+ * nphase / (nphase - 1)) for types 0 and 1.  Unused pointers may be NULL. */
+int32_t shwfs_slopes_geom(int32_t type, const float* d_phase, const float* d_pupil,
+                          const float* d_fluxPerSub, float* d_slopes, const int32_t* d_phasemap,
+                          const float* d_ttprojmat, int32_t nphase, int32_t nvalid, float alpha,
+                          void* stream);
+
+/* ---- tip/tilt filter of a slope vector ---------------------------------------
+ * tip = <s, ref_tip>, tilt = <s, ref_tilt> on the unfiltered slopes, then
+ * s -= tip * ref_tip + tilt * ref_tilt in place; d_tt (may be NULL) receives
+ * {tip, tilt}.  One launch on the device (graph-capturable); replaces
+ * SutraCentroider::apply_TT_filter (libsutra/src.cpp/sutra_centroider.cpp:322-338). */
+int32_t shwfs_tt_filter(float* d_slopes, const float* d_ref_tip, const float* d_ref_tilt,
+                        float* d_tt, int32_t nslopes, void* stream);
+
+/* ---- spots into the detector image (fill_binimage) ----------------------------
+ * With noise_background > 0 the whole image first becomes noise_background *
+ * N(0,1) (Philox stream (seed, call, domain), element index i); then every spot
+ * pixel is written: binimg[(validsubsx[s] + px) + (validsubsy[s] + py) * nsub] =
+ * bincube[px + py*npix + s*npix^2].  Other pixels keep their values.  Replaces
+ * SutraWfsSH::fill_binimage (libsutra/src.cpp/sutra_wfs_sh.cpp:599-621):
+ * d_binimg->prng('N', noise) and fillbinimg (src.cu/sutra_wfs.cu:183-229).
+ * The spot kernels draw their own background with domain 2 and the frame's
+ * call, so this entry with domain 2 reproduces their image. */
+int32_t shwfs_fill_binimg(const float* d_bincube, float* d_binimg, const int32_t* d_validsubsx,
+                          const int32_t* d_validsubsy, int32_t npix, int32_t nvalid, int32_t nsub,
+                          float noise_background, uint64_t seed, uint32_t call, uint32_t domain,
+                          void* stream);
+
+/* ---- checked host-to-device copy -----------------------------------------------
+ * Copies src_elems elements of elem_size bytes to a device array of dst_elems
+ * elements and waits for the copy; SHWFS_ERR_ARG (nothing copied) when the
+ * sizes differ.  Used by set_bincube / set_binimg, where the reference prints
+ * a message and returns success (libsutra/src.cpp/sutra_wfs_sh.cpp:665-672,
+ * src.cpp/sutra_wfs.cpp:253-261): here a wrong size is a defined error. */
+int32_t shwfs_upload(void* d_dst, int64_t dst_elems, const void* h_src, int64_t src_elems,
+                     int32_t elem_size, void* stream);
 
 /* ---- batched frames over several SH WFS ---------------------------------------
  * One launch per frame covers every sub-aperture of every WFS of the batch

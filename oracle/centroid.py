@@ -137,3 +137,15 @@ def bpcog(img, validx, validy, npix, nbpix, offset, scale, ref=None, order=0):
         slopes[iy] = cy - (0 if ref is None else ref[iy])
         inten[s] = I
     return slopes, inten, flat.reshape(img.shape, order="F")
+
+
+def tt_filter(slopes, tip, tilt):
+    """Tip/tilt removal, SutraCentroider::apply_TT_filter
+    (libsutra/src.cpp/sutra_centroider.cpp:322-338): tip = <s, ref_tip> and
+    tilt = <s, ref_tilt> are both taken on the unfiltered slopes, then
+    s - tip * ref_tip - tilt * ref_tilt.  Returns (filtered, (tip, tilt))."""
+    s = np.asarray(slopes, dtype=np.float64)
+    tp = np.asarray(tip, dtype=np.float64)
+    tl = np.asarray(tilt, dtype=np.float64)
+    a, b = float(s @ tp), float(s @ tl)
+    return s - a * tp - b * tl, (a, b)

@@ -66,7 +66,10 @@ def load():
     lib.shwfs_centroids_bpcog.argtypes = [vp, i32, C.POINTER(CentroidArgs), vp]
     lib.shwfs_fill_bincube.argtypes = [vp, vp, vp, vp, i32, i32, i32, vp]
     lib.shwfs_fill_bincube_2d.argtypes = [vp, vp, vp, i32, i32, i32, vp]
-    lib.shwfs_slopes_geom.argtypes = [i32, vp, vp, vp, vp, i32, i32, f32, vp]
+    lib.shwfs_slopes_geom.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, i32, f32, vp]
+    lib.shwfs_tt_filter.argtypes = [vp, vp, vp, vp, i32, vp]
+    lib.shwfs_fill_binimg.argtypes = [vp, vp, vp, vp, i32, i32, i32, f32, u64, u32, u32, vp]
+    lib.shwfs_upload.argtypes = [vp, C.c_int64, vp, C.c_int64, i32, vp]
     lib.shwfs_mfma_probe.argtypes = [vp, vp, vp, i32, vp]
     lib.shwfs_batch_create.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), i32,
                                        C.POINTER(vp)]

@@ -45,9 +45,25 @@ def test_null_and_size_arguments_rejected_without_launch(lib):
     assert lib.shwfs_prng(C.c_void_p(16), 10, b"X", 1.0, 0.0, 1, 0, 0, None) == capi.SHWFS_ERR_ARG
     assert lib.shwfs_fill_weights(None, None, 8, 64, None) == capi.SHWFS_ERR_ARG
     assert lib.shwfs_calibrate(None, None, None, None, None, 4, None) == capi.SHWFS_ERR_ARG
-    assert lib.shwfs_slopes_geom(1, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16), None, 8, 4,
-                                 1.0, None) == capi.SHWFS_ERR_UNSUPPORTED
+    dev = C.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    # unknown type; type 1 without pupil / flux; type 2 without ttprojmat
+    assert lib.shwfs_slopes_geom(3, dev, dev, dev, dev, dev, dev, 8, 4, 1.0, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_slopes_geom(1, dev, None, None, dev, dev, None, 8, 4, 1.0, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_slopes_geom(2, dev, None, None, dev, dev, None, 8, 4, 1.0, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_tt_filter(None, dev, dev, None, 8, None) == capi.SHWFS_ERR_ARG
+    assert lib.shwfs_fill_binimg(dev, None, dev, dev, 8, 4, 64, 0.0, 1, 0, 2, None) == capi.SHWFS_ERR_ARG
     assert lib.shwfs_mfma_probe(None, None, None, 1, None) == capi.SHWFS_ERR_ARG
+
+
+def test_upload_with_a_wrong_size_is_a_defined_error(lib):
+    """set_bincube / set_binimg go through shwfs_upload: a size that differs
+    from the device array's is SHWFS_ERR_ARG, checked before any copy (the
+    reference only prints a message and returns success)."""
+    host = np.zeros(10, dtype=np.float32)
+    rc = lib.shwfs_upload(C.c_void_p(16), 12, host.ctypes.data_as(C.c_void_p), 10, 4, None)
+    assert rc == capi.SHWFS_ERR_ARG
+    assert "10 elements given" in capi.last_error(lib) and "holds 12" in capi.last_error(lib)
+    assert lib.shwfs_upload(None, 10, host.ctypes.data_as(C.c_void_p), 10, 4, None) == capi.SHWFS_ERR_ARG
 
 
 def test_camera_bpcog_and_acquisim_arguments_rejected_without_launch(lib):

@@ -36,8 +36,10 @@ def test_mfma_probe_matches_matmul(method):
 # Shapes that take the other operand and binning variants of the MFMA
 # kernels: 32-point windows (two 16-row operand blocks), windows smaller than
 # the operand tile (masked), N = 64, one-pixel and 2x2 register binning, and
-# the general binmap gather (block origin not on the rebin grid).
+# the general binmap gather (block origin not on the rebin grid); b74: the
+# 74x74 WFS of the uneven batch (8-point windows in a 16-point transform).
 SHAPES = {
+    "b74": dict(diam=8.0, nxsub=74, pdiam=8, npix=6, nfft=16, nrebin=2),
     "q2_n64": dict(diam=8.0, nxsub=8, pdiam=32, npix=16, nfft=64, nrebin=4),
     "q2_masked": dict(diam=8.0, nxsub=20, pdiam=24, npix=8, nfft=32, nrebin=4),
     "rebin1": dict(diam=4.0, nxsub=10, pdiam=16, npix=12, nfft=16, nrebin=1),

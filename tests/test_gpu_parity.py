@@ -48,8 +48,12 @@ def test_spots_match_oracle(name):
 
 
 # Plain-path FFT variants: N = 64, one-pixel binning, the binmap gather, and
-# windows smaller than the FFT (the shapes of the MFMA variant tests).
+# windows smaller than the FFT (the shapes of the MFMA variant tests); b74:
+# the 74x74 WFS of the uneven batch (8 phase points in a 16-point transform,
+# 2x2 binning to 6x6 pixels: the central 12x12 of the 16x16 grid, block
+# origin 10).
 FFT_SHAPES = {
+    "b74": dict(diam=8.0, nxsub=74, pdiam=8, npix=6, nfft=16, nrebin=2),
     "n64": dict(diam=8.0, nxsub=8, pdiam=32, npix=16, nfft=64, nrebin=4),
     "rebin1": dict(diam=4.0, nxsub=10, pdiam=16, npix=12, nfft=16, nrebin=1),
     "gather": dict(diam=8.0, nxsub=24, pdiam=16, npix=7, nfft=32, nrebin=4),
