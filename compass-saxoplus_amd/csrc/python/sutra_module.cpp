@@ -53,8 +53,29 @@ static void declare_source(py::module& m) {
       .def_property_readonly("zp", [](SutraSource& s) { return s.zp; })
       .def_property_readonly("scale", [](SutraSource& s) { return s.scale; })
       .def_property_readonly("lgs", [](SutraSource& s) { return s.lgs; })
-      .def_property_readonly("d_phase", [](SutraSource& s) { return s.d_phase->d_screen; },
+      .def_property_readonly("d_phase",
+                             [](SutraSource& s) {
+                               s.render_layers();  // pending raytrace layers, if any
+                               return s.d_phase->d_screen;
+                             },
                              py::return_value_policy::reference_internal)
+      .def_property_readonly("xoff", [](SutraSource& s) { return s.xoff; })
+      .def_property_readonly("yoff", [](SutraSource& s) { return s.yoff; })
+      .def_property_readonly("G", [](SutraSource& s) { return s.G; })
+      .def_property_readonly("thetaML", [](SutraSource& s) { return s.thetaML; })
+      .def_property_readonly("dx", [](SutraSource& s) { return s.dx; })
+      .def_property_readonly("dy", [](SutraSource& s) { return s.dy; })
+      .def_property("lazy_raytrace", [](SutraSource& s) { return s.lazy_raytrace; },
+                    [](SutraSource& s, bool v) { s.lazy_raytrace = v; },
+                    "True: raytrace keeps the layers for the spot kernel's field stage")
+      .def_property_readonly("npending", [](SutraSource& s) { return (int)s.pending.size(); })
+      .def("add_layer", &SutraSource::add_layer, py::arg("type"), py::arg("idx"), py::arg("xoff"),
+           py::arg("yoff"))
+      .def("remove_layer", &SutraSource::remove_layer, py::arg("type"), py::arg("idx"))
+      .def("raytrace",
+           [](SutraSource& s, SutraAtmos& atmos, bool async) { return s.raytrace(&atmos, async); },
+           py::arg("atmos"), py::arg("async") = false)
+      .def("render_layers", [](SutraSource& s) { return s.render_layers(); })
       .def("reset_phase", &SutraSource::reset_phase)
       .def("set_phase",
            [](SutraSource& s, ArrayFStyle<float>& data) {
@@ -63,6 +84,46 @@ static void declare_source(py::module& m) {
              return s.set_phase(data.mutable_data());
            },
            py::arg("data"));
+}
+
+static void declare_atmos(py::module& m) {
+  py::class_<SutraTurbuScreen>(m, "TurbuScreen")
+      .def_property_readonly("device", [](SutraTurbuScreen& t) { return t.device; })
+      .def_property_readonly("screen_size", [](SutraTurbuScreen& t) { return t.screen_size; })
+      .def_property_readonly("r0", [](SutraTurbuScreen& t) { return t.r0; })
+      .def_property_readonly("amplitude", [](SutraTurbuScreen& t) { return t.amplitude; })
+      .def_property_readonly("altitude", [](SutraTurbuScreen& t) { return t.altitude; })
+      .def_property_readonly("windspeed", [](SutraTurbuScreen& t) { return t.windspeed; })
+      .def_property_readonly("winddir", [](SutraTurbuScreen& t) { return t.winddir; })
+      .def_property_readonly("deltax", [](SutraTurbuScreen& t) { return t.deltax; })
+      .def_property_readonly("deltay", [](SutraTurbuScreen& t) { return t.deltay; })
+      .def_property_readonly("accumx", [](SutraTurbuScreen& t) { return t.accumx; })
+      .def_property_readonly("accumy", [](SutraTurbuScreen& t) { return t.accumy; })
+      .def_property_readonly("d_screen", [](SutraTurbuScreen& t) { return t.d_tscreen->d_screen; },
+                             py::return_value_policy::reference_internal);
+
+  py::class_<SutraAtmos>(m, "Atmos")
+      .def(py::init([](CarmaContext& c, int32_t nscreens, float global_r0,
+                       ArrayFStyle<float>& r0_per_layer, ArrayFStyle<int64_t>& size1,
+                       ArrayFStyle<int64_t>& size2, ArrayFStyle<float>& altitude,
+                       ArrayFStyle<float>& windspeed, ArrayFStyle<float>& winddir,
+                       ArrayFStyle<float>& deltax, ArrayFStyle<float>& deltay, int32_t device) {
+             for (auto n : {r0_per_layer.size(), size1.size(), size2.size(), altitude.size(),
+                            windspeed.size(), winddir.size(), deltax.size(), deltay.size()})
+               if (n < nscreens) throw std::runtime_error("Atmos: per-layer arrays shorter than nscreens");
+             return std::unique_ptr<SutraAtmos>(new SutraAtmos(
+                 &c, nscreens, global_r0, r0_per_layer.mutable_data(), size1.mutable_data(),
+                 size2.mutable_data(), altitude.mutable_data(), windspeed.mutable_data(),
+                 winddir.mutable_data(), deltax.mutable_data(), deltay.mutable_data(), device));
+           }),
+           py::arg("context"), py::arg("nscreens"), py::arg("global_r0"),
+           py::arg("r0_per_layer"), py::arg("size1"), py::arg("size2"), py::arg("altitude"),
+           py::arg("windspeed"), py::arg("winddir"), py::arg("deltax"), py::arg("deltay"),
+           py::arg("device"))
+      .def_property_readonly("nscreens", [](SutraAtmos& a) { return a.nscreens; })
+      .def_property_readonly("r0", [](SutraAtmos& a) { return a.r0; })
+      .def_property_readonly("d_screens", [](SutraAtmos& a) { return a.d_screens; },
+                             py::return_value_policy::reference_internal);
 }
 
 static void declare_wfs(py::module& m) {
@@ -488,6 +549,7 @@ PYBIND11_MODULE(sutra, m) {
   m.doc() = "sutra: Shack-Hartmann WFS hot path on MI355X (gfx950)";
   py::module::import("carma");  // registers the CarmaObj / CarmaContext types
   declare_telescope(m);
+  declare_atmos(m);
   declare_source(m);
   declare_wfs(m);
   declare_wfs_sh(m);

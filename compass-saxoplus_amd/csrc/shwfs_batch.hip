@@ -310,6 +310,14 @@ int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* f
     ImageParams p;
     rc = image_params(&geoms[i], &frames[i], &p);
     if (rc) break;
+    if (p.ckind == SHWFS_CENTROID_MLP) {
+      rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: the learned centroider is not fused in batches");
+      break;
+    }
+    if (frames[i].nlayers > 0) {
+      rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: batches read rendered phase screens (no layers)");
+      break;
+    }
     b->geoms.push_back(geoms[i]);
     shwfs_frame_args f = frames[i];
     f.centroid = nullptr;

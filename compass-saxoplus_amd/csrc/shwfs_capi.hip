@@ -7,6 +7,7 @@
 
 #include "shwfs_device.hpp"
 #include "shwfs_internal.hpp"
+#include "shwfs_raytrace.hpp"
 
 namespace shwfs {
 
@@ -394,8 +395,17 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* out) {
   int32_t rc = check_geom(g, f ? f->method : 0);
   if (rc) return rc;
-  if (!f || !f->d_phase || !f->d_pupil || !f->d_bincube || !f->d_intensities)
-    return fail(SHWFS_ERR_ARG, "frame: phase, pupil, bincube and intensities are required");
+  if (!f || (!f->d_phase && f->nlayers <= 0) || !f->d_pupil || !f->d_bincube ||
+      !f->d_intensities)
+    return fail(SHWFS_ERR_ARG, "frame: phase (or layers), pupil, bincube and intensities are required");
+  if (f->nlayers > 0) {
+    if (f->method != SHWFS_METHOD_FFT_FP32)
+      return fail(SHWFS_ERR_UNSUPPORTED, "frame: the raytrace in the field stage runs in the FFT method only");
+    if (f->wg_size != 0 && f->wg_size != 64)
+      return fail(SHWFS_ERR_UNSUPPORTED, "frame: the raytrace in the field stage uses 64-lane workgroups");
+    if (f->centroid && f->centroid->kind == SHWFS_CENTROID_MLP)
+      return fail(SHWFS_ERR_UNSUPPORTED, "frame: layers and the fused learned centroider are exclusive");
+  }
   // every sub-aperture window must lie inside the screen: checked by the caller
   // when the tables are loaded (the tables live on the device).
   ImageParams p{};
@@ -433,8 +443,17 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
   p.ckind = SHWFS_CENTROID_NONE;
   if (f->centroid && f->centroid->kind != SHWFS_CENTROID_NONE) {
     const shwfs_centroid_args* c = f->centroid;
-    if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_WCOG)
+    if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_MLP)
       return fail(SHWFS_ERR_ARG, "unknown centroider kind");
+    if (c->kind == SHWFS_CENTROID_MLP) {
+      if (f->method != SHWFS_METHOD_FFT_FP32)
+        return fail(SHWFS_ERR_UNSUPPORTED,
+                    "fused learned centroider: runs in the FFT spot kernel only (method 0)");
+      if (g->npix != 8 || (g->nfft != 16 && g->nfft != 32))
+        return fail(SHWFS_ERR_UNSUPPORTED,
+                    "fused learned centroider: needs 8x8-pixel spots and nfft 16 or 32");
+      if (!c->d_mlp_params) return fail(SHWFS_ERR_ARG, "fused learned centroider: NULL parameters");
+    }
     if (!c->d_slopes || !c->d_intensities) return fail(SHWFS_ERR_ARG, "centroid outputs are NULL");
     if (c->kind == SHWFS_CENTROID_WCOG && !c->d_weights)
       return fail(SHWFS_ERR_ARG, "WCOG needs weights");
@@ -525,6 +544,18 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, v
     const long n = (long)p.nsub * p.nsub;
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
     if (rc) return rc;
+  }
+  if (f->nlayers > 0) {
+    RayParams r;
+    rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
+    if (rc) return rc;
+    return launch_image_fft_ray(p, r, g->nfft, st);
+  }
+  if (p.ckind == SHWFS_CENTROID_MLP) {
+    const shwfs_centroid_args* c = f->centroid;
+    const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
+                      c->mlp_flux_lo, c->mlp_flux_hi};
+    return launch_image_fft_mlp(p, m, g->nfft, st);
   }
   if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
   return launch_image_dft(p, g->nfft, f->method, st);

@@ -2,7 +2,10 @@
 #include "shwfs_image_dft.hpp"
 
 namespace shwfs {
-template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 1>(const ImageParams&, int, hipStream_t);
+template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 1>(const ImageParams&, int, hipStream_t,
+                                                   const void*);
+template int32_t build_dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, void*, size_t, hipStream_t,
+                                                        size_t*);
 template const void* dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, hipStream_t);
 template int32_t launch_dft_tasks_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, bool, int, bool, const BatchTask*,
                                                    const TaskStarts&, const TaskArgs&, int, int,

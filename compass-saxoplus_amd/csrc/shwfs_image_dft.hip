@@ -5,9 +5,9 @@
 namespace shwfs {
 
 template <int PREC>
-static int32_t launch_dft_p(const ImageParams& p, int nfft, hipStream_t st) {
-  if (p.pdiam <= 16) return launch_dft_pq<PREC, 1>(p, nfft, st);
-  if (p.pdiam <= 32) return launch_dft_pq<PREC, 2>(p, nfft, st);
+static int32_t launch_dft_p(const ImageParams& p, int nfft, hipStream_t st, const void* wtab) {
+  if (p.pdiam <= 16) return launch_dft_pq<PREC, 1>(p, nfft, st, wtab);
+  if (p.pdiam <= 32) return launch_dft_pq<PREC, 2>(p, nfft, st, wtab);
   return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
 }
 
@@ -47,9 +47,26 @@ int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, boo
   return fail(SHWFS_ERR_ARG, "unknown DFT method");
 }
 
-int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t st) {
-  if (method == SHWFS_METHOD_DFT_FP16) return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st);
-  if (method == SHWFS_METHOD_DFT_BF16X3) return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st);
+int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t st,
+                         const void* wtab) {
+  if (method == SHWFS_METHOD_DFT_FP16) return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st, wtab);
+  if (method == SHWFS_METHOD_DFT_BF16X3)
+    return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st, wtab);
+  return fail(SHWFS_ERR_ARG, "unknown DFT method");
+}
+
+int32_t build_dft_operand_table(int nfft, int pdiam, int method, void* dst, size_t cap,
+                                hipStream_t st, size_t* bytes) {
+  if (!dst) return fail(SHWFS_ERR_ARG, "DFT operand table: NULL destination");
+  if (pdiam <= 0 || pdiam > 32)
+    return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs pdiam <= min(32, nfft)");
+  const bool q1 = pdiam <= 16;
+  if (method == SHWFS_METHOD_DFT_FP16)
+    return q1 ? build_dft_table_pq<SHWFS_METHOD_DFT_FP16, 1>(nfft, dst, cap, st, bytes)
+              : build_dft_table_pq<SHWFS_METHOD_DFT_FP16, 2>(nfft, dst, cap, st, bytes);
+  if (method == SHWFS_METHOD_DFT_BF16X3)
+    return q1 ? build_dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(nfft, dst, cap, st, bytes)
+              : build_dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 2>(nfft, dst, cap, st, bytes);
   return fail(SHWFS_ERR_ARG, "unknown DFT method");
 }
 

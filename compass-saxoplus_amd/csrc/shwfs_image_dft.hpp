@@ -656,7 +656,7 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
     if (lane == 0) scales[k] = inten_scale;
   }
   wave_lds_sync();
-  const float* sp = spots + q * spot_f;
+  float* sp = spots + q * spot_f;
   const float I = spot_sum<16>(sp, npix2, valid, r);
   double* red = reinterpret_cast<double*>(scales + ((SPW + 3) & ~3));  // 8-byte aligned
   finish_spot<16, NOISE>(P, s, valid, r, sp, I, scales[q], call_add, red, &pre);
@@ -794,7 +794,7 @@ inline bool dft_half_grid(const ImageParams& p, int nfft) {
 }
 
 template <int N, int Q, int PREC, int NR>
-static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
+static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wtab_in) {
   using L = DftLayout<N, Q>;
   const int nbm = (NR == 0 && !p.bin_block) ? p.nrebin * p.nrebin * p.npix * p.npix : 0;
   const size_t lds = sizeof(float) * ((size_t)kDftWaves * L::wave_floats(p.npix, NR == 0) + nbm);
@@ -808,7 +808,9 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
                             : sh_image_dft_kernel<N, Q, PREC, true, false, NR>)
                     : (full ? sh_image_dft_kernel<N, Q, PREC, false, true, NR>
                             : sh_image_dft_kernel<N, Q, PREC, false, false, NR>);
-  const Frag<PREC>* wtab = wfrag_table<N, Q, PREC>(st);
+  // the library's cached table, or one the caller has built (wtab_in)
+  const Frag<PREC>* wtab = wtab_in ? static_cast<const Frag<PREC>*>(wtab_in)
+                                   : wfrag_table<N, Q, PREC>(st);
   if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
   if (p.nvalid <= 0) return SHWFS_OK;
   const int grid = (p.nvalid + kDftWaves - 1) / kDftWaves;  // one wave per sub-aperture
@@ -818,32 +820,34 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st) {
 }
 
 template <int N, int Q, int PREC>
-static int32_t launch_dft_n(const ImageParams& p, hipStream_t st) {
+static int32_t launch_dft_n(const ImageParams& p, hipStream_t st, const void* wtab) {
   switch (dft_register_rebin(p)) {
     case 4:
-      return dft_half_grid(p, N) ? launch_dft_t<N, Q, PREC, 4 | kHalfGrid>(p, st)
-                                 : launch_dft_t<N, Q, PREC, 4>(p, st);
-    case 2: return launch_dft_t<N, Q, PREC, 2>(p, st);
-    case 1: return launch_dft_t<N, Q, PREC, 1>(p, st);
-    default: return launch_dft_t<N, Q, PREC, 0>(p, st);
+      return dft_half_grid(p, N) ? launch_dft_t<N, Q, PREC, 4 | kHalfGrid>(p, st, wtab)
+                                 : launch_dft_t<N, Q, PREC, 4>(p, st, wtab);
+    case 2: return launch_dft_t<N, Q, PREC, 2>(p, st, wtab);
+    case 1: return launch_dft_t<N, Q, PREC, 1>(p, st, wtab);
+    default: return launch_dft_t<N, Q, PREC, 0>(p, st, wtab);
   }
 }
 
 // Entry point per operand precision and padded phase side 16*Q; each
-// (PREC, Q) pair is instantiated in a translation unit of its own.
+// (PREC, Q) pair is instantiated in a translation unit of its own.  wtab:
+// NULL (the library's table of the shape, built on first use) or a table
+// the caller has built with build_dft_table_pq.
 template <int PREC, int Q>
-int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st) {
+int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st, const void* wtab) {
   if constexpr (Q == 1) {
     switch (nfft) {
-      case 16: return launch_dft_n<16, 1, PREC>(p, st);
-      case 32: return launch_dft_n<32, 1, PREC>(p, st);
-      case 64: return launch_dft_n<64, 1, PREC>(p, st);
+      case 16: return launch_dft_n<16, 1, PREC>(p, st, wtab);
+      case 32: return launch_dft_n<32, 1, PREC>(p, st, wtab);
+      case 64: return launch_dft_n<64, 1, PREC>(p, st, wtab);
       default: break;
     }
   } else {
     switch (nfft) {
-      case 32: return launch_dft_n<32, 2, PREC>(p, st);
-      case 64: return launch_dft_n<64, 2, PREC>(p, st);
+      case 32: return launch_dft_n<32, 2, PREC>(p, st, wtab);
+      case 64: return launch_dft_n<64, 2, PREC>(p, st, wtab);
       default: break;
     }
   }
@@ -894,9 +898,46 @@ const void* dft_table_pq(int nfft, hipStream_t st) {
   }
 }
 
+// The operand table of (nfft, PREC, Q) built into dst (cap bytes) on stream
+// st, with no allocation and no wait; its size in bytes into *bytes.
+// SHWFS_ERR_ARG when it does not fit in cap.
+template <int N, int Q, int PREC>
+static int32_t build_wfrag_into(void* dst, size_t cap, hipStream_t st, size_t* bytes) {
+  using L = DftLayout<N, Q>;
+  const size_t need = sizeof(Frag<PREC>) * L::MT * L::KS * 64;
+  if (bytes) *bytes = need;
+  if (need > cap) return fail(SHWFS_ERR_ARG, "DFT operand table larger than the scratch buffer");
+  hipLaunchKernelGGL((build_wfrag_kernel<N, Q, PREC>), dim3(1), dim3(64), 0, st,
+                     static_cast<Frag<PREC>*>(dst));
+  SHWFS_LAUNCHED("build_wfrag_kernel");
+  return SHWFS_OK;
+}
+
+template <int PREC, int Q>
+int32_t build_dft_table_pq(int nfft, void* dst, size_t cap, hipStream_t st, size_t* bytes) {
+  if constexpr (Q == 1) {
+    switch (nfft) {
+      case 16: return build_wfrag_into<16, 1, PREC>(dst, cap, st, bytes);
+      case 32: return build_wfrag_into<32, 1, PREC>(dst, cap, st, bytes);
+      case 64: return build_wfrag_into<64, 1, PREC>(dst, cap, st, bytes);
+      default: break;
+    }
+  } else {
+    switch (nfft) {
+      case 32: return build_wfrag_into<32, 2, PREC>(dst, cap, st, bytes);
+      case 64: return build_wfrag_into<64, 2, PREC>(dst, cap, st, bytes);
+      default: break;
+    }
+  }
+  return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
+}
+
 #define SHWFS_DFT_EXTERN(PREC, Q)                                                         \
-  extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t); \
+  extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t,  \
+                                                 const void*);                          \
   extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);                \
+  extern template int32_t build_dft_table_pq<PREC, Q>(int, void*, size_t, hipStream_t,  \
+                                                      size_t*);                         \
   extern template int32_t launch_dft_tasks_pq<PREC, Q>(                                 \
       int, bool, int, bool, const BatchTask*, const TaskStarts&, const TaskArgs&, int, int,   \
       int, size_t, const uint32_t*, uint32_t, int, hipStream_t);

@@ -1,6 +1,7 @@
 // Plain FP32 spot-image kernel (stages in shwfs_image_fft.hpp) and the
 // background readout-noise kernel of the detector image.
 #include "shwfs_image_fft.hpp"
+#include "shwfs_mlp.hpp"
 
 namespace shwfs {
 
@@ -51,6 +52,141 @@ int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t st) {
     case 32: return half ? launch_fft_nr<32, 2>(p, st) : launch_fft_nr<32, 1>(p, st);
     case 64: return half ? launch_fft_nr<64, 2>(p, st) : launch_fft_nr<64, 1>(p, st);
     default: return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
+  }
+}
+
+// ---- spots with the guide-star raytrace in the field stage ------------------
+// The FFT spot kernel (one-wave workgroups) whose window phases are traced
+// through the layers (ray_phase) instead of read from a rendered screen: the
+// npup^2 screen is neither written nor read back.
+template <int N, int R, bool NOISE>
+__global__ __launch_bounds__(64) void sh_image_fft_ray_kernel(ImageParams P, RayParams Ry) {
+  using L = FftLayout<N, R>;
+  constexpr int S = 64 / N;
+  extern __shared__ float smem[];
+  int* s_binmap = (int*)(smem + S * L::slot_floats(P.pdiam, P.npix));
+  if (!P.bin_block) {
+    const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
+    for (int i = threadIdx.x; i < nbm; i += 64) s_binmap[i] = P.binmap[i];
+  }
+  fft_tile<N, R, 64, NOISE, false, true>(P, xcd_remap(blockIdx.x, gridDim.x) * S, smem, s_binmap,
+                                         0u, (int)threadIdx.x, &Ry);
+}
+
+template <int N, int R>
+static int32_t launch_fft_ray_nr(const ImageParams& p, const RayParams& r, hipStream_t st) {
+  using L = FftLayout<N, R>;
+  constexpr int S = 64 / N;
+  const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * p.npix * p.npix;
+  const size_t lds = sizeof(float) * ((size_t)S * L::slot_floats(p.pdiam, p.npix) + nbm);
+  if (lds > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "spot kernel: LDS footprint exceeds the per-workgroup limit");
+  const int grid = (p.nvalid + S - 1) / S;
+  if (p.noise > -1.0f)
+    hipLaunchKernelGGL((sh_image_fft_ray_kernel<N, R, true>), dim3(grid), dim3(64), lds, st, p, r);
+  else
+    hipLaunchKernelGGL((sh_image_fft_ray_kernel<N, R, false>), dim3(grid), dim3(64), lds, st, p, r);
+  SHWFS_LAUNCHED("sh_image_fft_ray_kernel");
+  return SHWFS_OK;
+}
+
+int32_t launch_image_fft_ray(const ImageParams& p, const RayParams& r, int nfft, hipStream_t st) {
+  if (p.nvalid <= 0) return SHWFS_OK;
+  const bool half = 2 * p.pdiam <= nfft;
+  switch (nfft) {
+    case 8: return half ? launch_fft_ray_nr<8, 2>(p, r, st) : launch_fft_ray_nr<8, 1>(p, r, st);
+    case 16: return half ? launch_fft_ray_nr<16, 2>(p, r, st) : launch_fft_ray_nr<16, 1>(p, r, st);
+    case 32: return half ? launch_fft_ray_nr<32, 2>(p, r, st) : launch_fft_ray_nr<32, 1>(p, r, st);
+    case 64: return half ? launch_fft_ray_nr<64, 2>(p, r, st) : launch_fft_ray_nr<64, 1>(p, r, st);
+    default: return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
+  }
+}
+
+// ---- spots + learned centroider in one launch ------------------------------
+// 16 sub-apertures per workgroup (N lanes each, npix 8).  The tile's tail
+// leaves each spot's spot-cube values in its slot (fft_tile<..., KEEP>); the
+// 16 spots then go through one MLP forward on the matrix cores (weights
+// converted to bf16 as they are read) and each wave turns network outputs
+// into slopes, or takes the CoG of the same pixels where the spot flux is
+// outside the network's range.  Same arithmetic as shwfs_mlp_centroids over
+// the spot cube, so both give the same slopes.
+constexpr int kMlpSubaps = 16;
+
+template <int N, int R>
+__host__ __device__ constexpr int mlp_lds_floats_min() {
+  // spots (16 x 64 floats) + the MLP tile, laid over the dead slots
+  return kMlpSubaps * 64 + (int)(mlp::Lds<kMlpSubaps>::bytes() / sizeof(float));
+}
+
+template <int N, int R, bool NOISE>
+__global__ __launch_bounds__(kMlpSubaps * N) void sh_image_fft_mlp_kernel(ImageParams P,
+                                                                          MlpParams M) {
+  using L = FftLayout<N, R>;
+  constexpr int S = kMlpSubaps, TPB = kMlpSubaps * N, NW = TPB / 64;
+  extern __shared__ float smem[];
+  const int slot_f = L::slot_floats(P.pdiam, 8);
+  int* s_binmap = (int*)(smem + S * slot_f);
+  if (!P.bin_block) {
+    const int nbm = P.nrebin * P.nrebin * 64;
+    for (int i = threadIdx.x; i < nbm; i += TPB) s_binmap[i] = P.binmap[i];
+  }
+  const int s0 = xcd_remap(blockIdx.x, gridDim.x) * S;
+  fft_tile<N, R, TPB, NOISE, true>(P, s0, smem, s_binmap);  // ends with a barrier
+  const int tid = threadIdx.x;
+  const int rows = P.nvalid - s0 < S ? P.nvalid - s0 : S;
+  // the 16 spots (pixel c of spot k at slot k's spot area) into registers,
+  // then, past a barrier, to their places in the MLP tile
+  constexpr int PER = S * 64 / TPB;
+  float v[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = tid + e * TPB, k = i >> 6, c = i & 63;
+    v[e] = k < rows ? smem[k * slot_f + slot_f - 64 + c] : 0.0f;
+  }
+  __syncthreads();
+  float* spots = smem;  // [16][64]
+  const mlp::Lds<S> T(reinterpret_cast<char*>(smem + S * 64));
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = tid + e * TPB, k = i >> 6, c = i & 63;
+    spots[i] = v[e];
+    T.x[k * mlp::SX + c] = (__bf16)(v[e] * M.xs);
+  }
+  mlp::tile_forward<S>(M.params, mlp::Fp32Weights{M.params}, T, tid);  // starts with a barrier
+  const int lane = tid & 63, w = tid >> 6;
+  for (int k = w; k < rows; k += NW)
+    mlp::spot_slopes(spots[k * 64 + lane], lane, s0 + k, P.nvalid, T.y[k * mlp::NOUT],
+                     T.y[k * mlp::NOUT + 1], M.lo, M.hi, P.coffset, P.cscale, M.ys, P.ref,
+                     P.slope_order, P.slopes, P.cintens, M.fallback);
+}
+
+template <int N, int R>
+static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipStream_t st) {
+  using L = FftLayout<N, R>;
+  constexpr int S = kMlpSubaps;
+  const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * 64;
+  size_t floats = (size_t)S * L::slot_floats(p.pdiam, 8) + nbm;
+  if (floats < (size_t)mlp_lds_floats_min<N, R>()) floats = mlp_lds_floats_min<N, R>();
+  const size_t lds = sizeof(float) * floats;
+  if (lds > max_lds_per_block())
+    return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: LDS footprint exceeds the per-workgroup limit");
+  const int grid = (p.nvalid + S - 1) / S;
+  if (p.noise > -1.0f)
+    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, true>), dim3(grid), dim3(S * N), lds, st, p, m);
+  else
+    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, false>), dim3(grid), dim3(S * N), lds, st, p, m);
+  SHWFS_LAUNCHED("sh_image_fft_mlp_kernel");
+  return SHWFS_OK;
+}
+
+int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t st) {
+  if (p.npix != 8) return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: needs 8x8-pixel spots");
+  if (p.nvalid <= 0) return SHWFS_OK;
+  const bool half = 2 * p.pdiam <= nfft;
+  switch (nfft) {
+    case 16: return half ? launch_fft_mlp_nr<16, 2>(p, m, st) : launch_fft_mlp_nr<16, 1>(p, m, st);
+    case 32: return half ? launch_fft_mlp_nr<32, 2>(p, m, st) : launch_fft_mlp_nr<32, 1>(p, m, st);
+    default: return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: nfft must be 16 or 32");
   }
 }
 

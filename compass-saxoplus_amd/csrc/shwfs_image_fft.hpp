@@ -56,10 +56,15 @@ __device__ __forceinline__ void tile_sync() {
 // tile's LDS (S slots of slot_floats, then the binmap if bm is not the global
 // table).  Contains tile barriers (tile_sync): every lane of the tile calls
 // it.  call_add: added to the Philox call counter of the noise.
-template <int N, int R, int TPB, bool NOISE>
+// KEEP: the tail leaves the spot-cube values of each sub-aperture in its
+// slot's spot area (the last npix^2 floats, 4-aligned, of the slot) for a
+// centroider that runs after the tile.  RAY: the window phases are traced
+// through the layers of *ray (guide-star raytrace in the field stage).
+template <int N, int R, int TPB, bool NOISE, bool KEEP = false, bool RAY = false>
 __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* smem,
                                          const int* bm, unsigned call_add = 0u,
-                                         int tid = (int)threadIdx.x) {
+                                         int tid = (int)threadIdx.x,
+                                         const RayParams* ray = nullptr) {
   using L = FftLayout<N, R>;
   constexpr int M = L::M;
   constexpr int PS = L::PS;
@@ -77,7 +82,7 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
   float* bin = base + (slot_f - ((P.npix * P.npix + 3) & ~3));
 
   // ---- 0. fields
-  load_fields<N, L::EPL, false>(P, s, valid, lane, (float*)E, PE, 0);
+  load_fields<N, L::EPL, false, false, 0, RAY>(P, s, valid, lane, (float*)E, PE, 0, ray);
   tile_sync<TPB>();
 
   // ---- 1. row transforms
@@ -131,7 +136,7 @@ __device__ __forceinline__ void fft_tile(const ImageParams& P, int s0, float* sm
   tile_sync<TPB>();
 
   // ---- 3. tail
-  spot_tail<N, N, NOISE>(P, s, valid, lane, hr, bin, bm, 1.0f, call_add);
+  spot_tail<N, N, NOISE, KEEP>(P, s, valid, lane, hr, bin, bm, 1.0f, call_add);
   tile_sync<TPB>();
 }
 

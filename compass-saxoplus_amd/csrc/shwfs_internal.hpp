@@ -80,6 +80,14 @@ struct ImageParams {
   float* cintens;
 };
 
+// Learned centroider run after binning by the FFT spot kernel (a kernel
+// argument of its own, not part of ImageParams, which batches pass by value).
+struct MlpParams {
+  const float* params;  // [SHWFS_MLP_NPARAM]
+  int* fallback;        // [nvalid] or NULL
+  float xs, ys, lo, hi; // input scale, output scale, flux range of the network
+};
+
 // One WFS of a batch (device task table): its parameters, stage kind and
 // range of global tiles.
 struct BatchTask {
@@ -153,7 +161,17 @@ int32_t check_geom(const shwfs_sh_geom* g, int method);
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
 
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
-int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s);
+// The FFT spot kernel tracing its window phases through layers (RayParams,
+// shwfs_raytrace.hpp) instead of reading p.phase.
+struct RayParams;
+int32_t launch_image_fft_ray(const ImageParams& p, const RayParams& r, int nfft, hipStream_t s);
+// The FFT spot kernel with the learned centroider after binning (p.ckind ==
+// SHWFS_CENTROID_MLP): 16 sub-apertures per workgroup, npix 8, nfft 16 or 32.
+int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t s);
+// wtab: NULL (the library's operand table of the shape, built and cached on
+// first use) or a table built by build_dft_operand_table.
+int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s,
+                         const void* wtab = nullptr);
 int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, hipStream_t s);
 // img = sigma * N(0,1) (Philox domain `domain`, element = image index); the
 // call is `call` plus *call_base when call_base is not NULL (device frame
@@ -172,5 +190,9 @@ int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, boo
 // Device table of the DFT operand fragments for (nfft, pdiam, method), built
 // on first use on the current device; NULL (and the error set) otherwise.
 const void* dft_operand_table(int nfft, int pdiam, int method, hipStream_t s);
+// The same table built into caller memory dst of cap bytes by one launch on
+// s (no allocation, no wait); *bytes (may be NULL) receives its size.
+int32_t build_dft_operand_table(int nfft, int pdiam, int method, void* dst, size_t cap,
+                                hipStream_t s, size_t* bytes);
 
 }  // namespace shwfs

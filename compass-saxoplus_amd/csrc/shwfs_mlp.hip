@@ -15,71 +15,19 @@
 #include "shwfs_device.hpp"
 #include "shwfs_internal.hpp"
 #include "shwfs_mfma.hpp"
+#include "shwfs_mlp.hpp"
 
 namespace shwfs {
 namespace {
 
-constexpr int NIN = 64, NH = 128, NOUT = 2;
-constexpr int OW1 = 0, OB1 = OW1 + NH * NIN, OW2 = OB1 + NH, OB2 = OW2 + NH * NH,
-              OW3 = OB2 + NH, OB3 = OW3 + NOUT * NH, NPARAM = OB3 + NOUT;
+using namespace mlp;
 static_assert(NPARAM == SHWFS_MLP_NPARAM, "parameter layout");
 constexpr int TS = 32;          // samples per tile
 constexpr int TPB = 256;        // 4 waves
-// LDS row strides (elements), padded against bank conflicts
-constexpr int SX = NIN + 8;     // bf16 X
-constexpr int SZ = NH + 4;      // fp32 Z1 / Z2
-constexpr int SH = NH + 8;      // bf16 H1 / H2
-constexpr size_t LDS_BYTES = (size_t)TS * SX * 2 + 2 * (size_t)TS * SZ * 4 + 2 * (size_t)TS * SH * 2 +
-                             (size_t)TS * NOUT * 4 * 2;
+using Lds = mlp::Lds<TS>;
+constexpr size_t LDS_BYTES = Lds::bytes();
 // work buffer: bf16 copies W1 (128x64), W2 (128x128), W2^T (128x128)
 constexpr size_t WORK_ELEMS = (size_t)NH * NIN + 2 * (size_t)NH * NH;
-
-struct Lds {
-  __bf16* x;   // [TS][SX]
-  float* z1;   // [TS][SZ]   z1, later dZ1
-  float* z2;   // [TS][SZ]   z2, later dZ2
-  __bf16* h1;  // [TS][SH]   gelu(z1), later dZ1 (bf16)
-  __bf16* h2;  // [TS][SH]   gelu(z2), later dZ2 (bf16)
-  float* y;    // [TS][NOUT]
-  float* dy;   // [TS][NOUT]
-};
-
-__device__ Lds carve(char* base) {
-  Lds l;
-  l.x = (__bf16*)base;
-  base += (size_t)TS * SX * 2;
-  l.z1 = (float*)base;
-  base += (size_t)TS * SZ * 4;
-  l.z2 = (float*)base;
-  base += (size_t)TS * SZ * 4;
-  l.h1 = (__bf16*)base;
-  base += (size_t)TS * SH * 2;
-  l.h2 = (__bf16*)base;
-  base += (size_t)TS * SH * 2;
-  l.y = (float*)base;
-  base += (size_t)TS * NOUT * 4;
-  l.dy = (float*)base;
-  return l;
-}
-
-constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
-}
-__device__ __forceinline__ float gelu_grad(float z) {
-  const float t = tanhf(kGeluC * (z + 0.044715f * z * z * z));
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
-}
-
-// 8 consecutive bf16 of a row (16-byte aligned)
-__device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
-// 8 bf16 at stride `st` (column of a row-major array)
-__device__ __forceinline__ bf16x8 ld8s(const __bf16* p, int st) {
-  bf16x8 v;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = p[e * st];
-  return v;
-}
 
 // X tile of samples row0 .. row0+TS-1 (rows >= rows_valid zero) -> bf16 in LDS.
 __device__ void load_rows(const float* __restrict__ x, const int* __restrict__ index, long row0,
@@ -93,68 +41,6 @@ __device__ void load_rows(const float* __restrict__ x, const int* __restrict__ i
     }
     L.x[s * SX + c] = (__bf16)v;
   }
-}
-
-// Forward of the tile in L.x into LDS (z1, h1, z2, h2, y).
-__device__ void tile_forward(const float* __restrict__ P, const __bf16* __restrict__ W1b,
-                             const __bf16* __restrict__ W2b, const Lds& L) {
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  __syncthreads();
-  // Z1 = X W1^T + b1: 2 x 8 output tiles, wave w takes n-tiles 2w, 2w+1 for both m-tiles
-#pragma unroll
-  for (int mi = 0; mi < TS / 16; ++mi)
-#pragma unroll
-    for (int jj = 0; jj < 2; ++jj) {
-      const int nj = 2 * w + jj;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < NIN / 32; ++ks) {
-        const bf16x8 a = ld8(L.x + (16 * mi + (lane & 15)) * SX + 32 * ks + 8 * (lane >> 4));
-        const bf16x8 b = ld8(W1b + (16 * nj + (lane & 15)) * NIN + 32 * ks + 8 * (lane >> 4));
-        acc = mfma_bf16(a, b, acc);
-      }
-      const int col = 16 * nj + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * mi + 4 * (lane >> 4) + v;
-        const float z = acc[v] + P[OB1 + col];
-        L.z1[row * SZ + col] = z;
-        L.h1[row * SH + col] = (__bf16)gelu(z);
-      }
-    }
-  __syncthreads();
-  // Z2 = H1 W2^T + b2
-#pragma unroll
-  for (int mi = 0; mi < TS / 16; ++mi)
-#pragma unroll
-    for (int jj = 0; jj < 2; ++jj) {
-      const int nj = 2 * w + jj;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < NH / 32; ++ks) {
-        const bf16x8 a = ld8(L.h1 + (16 * mi + (lane & 15)) * SH + 32 * ks + 8 * (lane >> 4));
-        const bf16x8 b = ld8(W2b + (16 * nj + (lane & 15)) * NH + 32 * ks + 8 * (lane >> 4));
-        acc = mfma_bf16(a, b, acc);
-      }
-      const int col = 16 * nj + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = 16 * mi + 4 * (lane >> 4) + v;
-        const float z = acc[v] + P[OB2 + col];
-        L.z2[row * SZ + col] = z;
-        L.h2[row * SH + col] = (__bf16)gelu(z);
-      }
-    }
-  __syncthreads();
-  // Y = gelu(Z2) W3^T + b3 (fp32): thread (s, o) for the first 64 threads,
-  // others idle; the 128-term dot products read Z2 from LDS
-  if (tid < TS * NOUT) {
-    const int s = tid / NOUT, o = tid % NOUT;
-    float acc = P[OB3 + o];
-    for (int k = 0; k < NH; ++k) acc += gelu(L.z2[s * SZ + k]) * P[OW3 + o * NH + k];
-    L.y[s * NOUT + o] = acc;
-  }
-  __syncthreads();
 }
 
 __global__ __launch_bounds__(TPB) void mlp_prepare_kernel(const float* __restrict__ P,
@@ -182,7 +68,7 @@ __global__ __launch_bounds__(TPB) void mlp_tile_kernel(const float* __restrict__
                                                        float* __restrict__ grad,
                                                        float* __restrict__ loss) {
   extern __shared__ char smem_raw[];
-  const Lds L = carve(smem_raw);
+  const Lds L(smem_raw);
   const __bf16* W1b = work;
   const __bf16* W2b = W1b + NH * NIN;
   const __bf16* W2t = W2b + NH * NH;
@@ -190,7 +76,7 @@ __global__ __launch_bounds__(TPB) void mlp_tile_kernel(const float* __restrict__
   const int rows = (int)((n - row0) < TS ? (n - row0) : TS);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   load_rows(x, index, row0, rows, x_scale, L);
-  tile_forward(P, W1b, W2b, L);
+  mlp::tile_forward<TS>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
   if constexpr (!TRAIN) {
     if (tid < rows * NOUT) y[(row0 + tid / NOUT) * NOUT + tid % NOUT] = L.y[tid];
     return;
@@ -358,7 +244,7 @@ __global__ __launch_bounds__(TPB) void mlp_centroid_kernel(
     const float* __restrict__ ref, int slope_order, float* __restrict__ slopes,
     float* __restrict__ intens, int* __restrict__ fallback) {
   extern __shared__ char smem_raw[];
-  const Lds L = carve(smem_raw);
+  const Lds L(smem_raw);
   const __bf16* W1b = work;
   const __bf16* W2b = W1b + NH * NIN;
   const long row0 = (long)blockIdx.x * TS;
@@ -373,33 +259,15 @@ __global__ __launch_bounds__(TPB) void mlp_centroid_kernel(
     const int sl = i / NIN, c = i % NIN;
     L.x[sl * SX + c] = (__bf16)(sl < rows ? pixel(row0 + sl, c) * x_scale : 0.0f);
   }
-  tile_forward(P, W1b, W2b, L);
+  mlp::tile_forward<TS>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
   // flux and CoG from the raw pixels: one wave per 8 sub-apertures
   for (int k = 0; k < TS / 4; ++k) {
     const int sl = w * (TS / 4) + k;
     if (sl >= rows) break;  // uniform over the wave
     const long s = row0 + sl;
-    const float b = pixel(s, lane);
-    const float I = wave_sum(b);
-    const float Sx = wave_sum(b * (float)(lane & 7));
-    const float Sy = wave_sum(b * (float)(lane >> 3));
-    if (lane == 0) {
-      const bool ood = !(I >= flux_lo && I <= flux_hi);
-      const int ix = slope_order == 0 ? (int)s : 2 * (int)s;
-      const int iy = slope_order == 0 ? (int)s + nvalid : 2 * (int)s + 1;
-      float cx, cy;
-      if (ood) {
-        cx = ((float)((double)Sx / ((double)I + 1.e-6)) - offset) * scale;
-        cy = ((float)((double)Sy / ((double)I + 1.e-6)) - offset) * scale;
-      } else {
-        cx = L.y[sl * NOUT + 0] * y_scale;
-        cy = L.y[sl * NOUT + 1] * y_scale;
-      }
-      slopes[ix] = cx - (ref ? ref[ix] : 0.0f);
-      slopes[iy] = cy - (ref ? ref[iy] : 0.0f);
-      if (intens) intens[s] = I;
-      if (fallback) fallback[s] = ood ? 1 : 0;
-    }
+    mlp::spot_slopes(pixel(s, lane), lane, (int)s, nvalid, L.y[sl * NOUT + 0], L.y[sl * NOUT + 1],
+                     flux_lo, flux_hi, offset, scale, y_scale, ref, slope_order, slopes, intens,
+                     fallback);
   }
 }
 

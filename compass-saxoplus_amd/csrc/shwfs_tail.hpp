@@ -9,6 +9,7 @@
 
 #include "shwfs_device.hpp"
 #include "shwfs_internal.hpp"
+#include "shwfs_raytrace.hpp"
 
 namespace shwfs {
 
@@ -45,10 +46,12 @@ __device__ __forceinline__ void field_sincos(float ph, float scale, float turn_s
 // PLANAR: Er at E[y*pe + x], Ei at E[plane + y*pe + x] (floats); otherwise
 // interleaved float2 at E2[y*pe + x].  All global loads of a chunk are issued
 // unconditionally (addresses clamped into the window) before the first use,
-// so their latencies overlap.
-template <int LANES, int EPL, bool PLANAR, bool FAST = false, int PD = 0>
+// so their latencies overlap.  RAY: the phase of each window point is traced
+// through the layers of *ray (ray_phase) instead of read from P.phase.
+template <int LANES, int EPL, bool PLANAR, bool FAST = false, int PD = 0, bool RAY = false>
 __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool valid, int lane,
-                                            float* E, int pe, int plane) {
+                                            float* E, int pe, int plane,
+                                            const RayParams* ray = nullptr) {
   const int pdiam = PD > 0 ? PD : P.pdiam;  // PD: pdiam known at compile time
   const int np2 = pdiam * pdiam;
   if (lane >= LANES) __builtin_unreachable();
@@ -74,7 +77,10 @@ __device__ __forceinline__ void load_fields(const ImageParams& P, int s, bool va
         const int yc = in ? y : pdiam - 1;
         const int xc = in ? x : pdiam - 1;
         gs[e] = (unsigned)(xc + yc * npup);
-        ph[e] = phase[gs[e]];
+        if constexpr (RAY)
+          ph[e] = ray_phase(*ray, P.istart[s] + xc, P.jstart[s] + yc);
+        else
+          ph[e] = phase[gs[e]];
         of[e] = P.offsets[(unsigned)(in ? idx : np2 - 1)];
         x += dx;
         y += dy;
@@ -219,9 +225,9 @@ struct Moments {
 // and binimg.  KIND (centroider) and FP64 (float64 moments) are fixed per
 // instantiation so the pixel loops carry no per-pixel tests of them.
 template <bool NOISE, int KIND, bool FP64>
-__device__ __forceinline__ void tail_pixel(const ImageParams& P, unsigned e, float b, float fx,
-                                           float fy, unsigned img, unsigned call_add,
-                                           Moments& m) {
+__device__ __forceinline__ float tail_pixel(const ImageParams& P, unsigned e, float b, float fx,
+                                            float fy, unsigned img, unsigned call_add,
+                                            Moments& m) {
   // no fused multiply-add anywhere in the pixel: the thresholded value is
   // taken from the stored (rounded) pixel, and the moment sums depend only on
   // the order of their terms
@@ -264,14 +270,17 @@ __device__ __forceinline__ void tail_pixel(const ImageParams& P, unsigned e, flo
     }
   }
   if (P.binimg) store_at(P.binimg, img, out);
+  return b;  // the spot-cube value
 }
 
 // The pixels p = lane, lane + LANES, ... of sub-aperture s (tail_pixel), in
 // that order.  NPIX > 0: npix == NPIX is known and, LANES being a multiple
 // of it, the lane's pixel column is fixed and its row advances by
 // LANES / NPIX from one pixel to the next (no per-pixel index work).
-template <int LANES, bool NOISE, int KIND, bool FP64, int NPIX = 0>
-__device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lane, const float* bin,
+// KEEP: pixel p's spot-cube value replaces bin[p] (the spot stays in LDS for
+// a centroider that runs after the tail).
+template <int LANES, bool NOISE, int KIND, bool FP64, int NPIX = 0, bool KEEP = false>
+__device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lane, float* bin,
                                             float fnorm, int vx, int vy, unsigned call_add,
                                             Moments& m) {
   if constexpr (NPIX > 0 && LANES % NPIX == 0) {
@@ -285,19 +294,24 @@ __device__ __forceinline__ void tail_pixels(const ImageParams& P, int s, int lan
     const unsigned dimg = __umul24(DY, P.nsub);
 #pragma unroll
     for (int j = 0; j * LANES < NP2; ++j)
-      if (lane + j * LANES < NP2)
-        tail_pixel<NOISE, KIND, FP64>(P, e0 + j * LANES, bin[lane + j * LANES] * fnorm,
-                                      (float)px, (float)(py0 + j * DY), img0 + j * dimg,
-                                      call_add, m);
+      if (lane + j * LANES < NP2) {
+        const float v = tail_pixel<NOISE, KIND, FP64>(P, e0 + j * LANES,
+                                                      bin[lane + j * LANES] * fnorm, (float)px,
+                                                      (float)(py0 + j * DY), img0 + j * dimg,
+                                                      call_add, m);
+        if constexpr (KEEP) bin[lane + j * LANES] = v;
+      }
   } else {
     const int npix = P.npix;
     const int npix2 = npix * npix;
     const float inv_npix = __builtin_amdgcn_rcpf((float)npix);
     for (int p = lane; p < npix2; p += LANES) {
       const int py = pixel_row(p, inv_npix), px = p - (int)__umul24(py, npix);
-      tail_pixel<NOISE, KIND, FP64>(P, (unsigned)(p + s * npix2), bin[p] * fnorm, (float)px,
-                                    (float)py, (unsigned)(vx + px) + __umul24(vy + py, P.nsub),
-                                    call_add, m);
+      const float v = tail_pixel<NOISE, KIND, FP64>(P, (unsigned)(p + s * npix2), bin[p] * fnorm,
+                                                    (float)px, (float)py,
+                                                    (unsigned)(vx + px) + __umul24(vy + py, P.nsub),
+                                                    call_add, m);
+      if constexpr (KEEP) bin[p] = v;
     }
   }
 }
@@ -329,12 +343,30 @@ __device__ __forceinline__ float spot_sum(const float* sp, int npix2, bool valid
 // red: for LANES == 64, LDS scratch of 192 doubles for the moment totals
 // (wave_sum3; must not overlap bin), or nullptr for the DPP reductions.
 // pre: the sub-aperture's scalars loaded beforehand, or nullptr.
-template <int LANES, bool NOISE>
+// KEEP: no centroider here; the spot-cube values replace bin (for the
+// learned centroider, which runs after the tail over several spots).
+template <int LANES, bool NOISE, bool KEEP = false>
 __device__ __forceinline__ void finish_spot(const ImageParams& P, int s, bool valid, int lane,
-                                            const float* bin, float I, float inten_scale,
+                                            float* bin, float I, float inten_scale,
                                             unsigned call_add = 0u, void* red = nullptr,
                                             const SubapScalars* pre = nullptr) {
   Moments m;
+  if constexpr (KEEP) {
+    if (valid) {
+      const float fl = pre ? pre->fl : P.flux[s];
+      if (lane == 0) P.intensities[s] = I * inten_scale;
+      const float fnorm = (I != 0.0f) ? fl * __builtin_amdgcn_rcpf(I) * P.nphot : 1.0f;
+      const int vx = pre ? pre->vx : P.validsubsx[s];
+      const int vy = pre ? pre->vy : P.validsubsy[s];
+      if (!NOISE && P.npix == 8)
+        tail_pixels<LANES, NOISE, SHWFS_CENTROID_NONE, false, NOISE ? 0 : 8, true>(
+            P, s, lane, bin, fnorm, vx, vy, call_add, m);
+      else
+        tail_pixels<LANES, NOISE, SHWFS_CENTROID_NONE, false, 0, true>(P, s, lane, bin, fnorm, vx,
+                                                                       vy, call_add, m);
+    }
+    return;
+  }
   if (valid) {
     const float fl = pre ? pre->fl : P.flux[s];
     const int vx = pre ? pre->vx : P.validsubsx[s];
@@ -437,7 +469,7 @@ __device__ __forceinline__ float bin_gather(const ImageParams& P, bool valid, in
 // hr: the NF x NF |E|^2 image of this sub-aperture in LDS, flat index u + v*NF,
 // holding inten_scale^-1 times the true intensities (a power of two).
 // bin: npix^2 floats of LDS private to this sub-aperture.
-template <int LANES, int NF, bool NOISE>
+template <int LANES, int NF, bool NOISE, bool KEEP = false>
 __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool valid, int lane,
                                           const float* hr, float* bin, const int* s_binmap,
                                           float inten_scale, unsigned call_add = 0u,
@@ -445,7 +477,7 @@ __device__ __forceinline__ void spot_tail(const ImageParams& P, int s, bool vali
                                           const SubapScalars* pre = nullptr) {
   const float part = bin_gather<LANES, NF>(P, valid, lane, hr, bin, s_binmap);
   const float I = lane_group_sum<LANES>(part);
-  finish_spot<LANES, NOISE>(P, s, valid, lane, bin, I, inten_scale, call_add, red, pre);
+  finish_spot<LANES, NOISE, KEEP>(P, s, valid, lane, bin, I, inten_scale, call_add, red, pre);
 }
 
 }  // namespace shwfs

@@ -75,15 +75,131 @@ SutraSource::SutraSource(CarmaContext* context, float xpos, float ypos, float la
       thetaML(0.f),
       dx(0.f),
       dy(0.f),
-      type(type_) {
+      type(type_),
+      lazy_raytrace(true) {
   context->set_active_device(device, 1);
   d_phase = new SutraPhase(context, size);
 }
 
 SutraSource::~SutraSource() { del(d_phase); }
 
-int32_t SutraSource::set_phase(float* data) { return d_phase->d_screen->host2device(data); }
-int32_t SutraSource::reset_phase() { return d_phase->d_screen->reset(); }
+int32_t SutraSource::set_phase(float* data) {
+  pending.clear();
+  return d_phase->d_screen->host2device(data);
+}
+int32_t SutraSource::reset_phase() {
+  pending.clear();
+  return d_phase->d_screen->reset();
+}
+
+// reference: libsutra/src.cpp/sutra_source.cpp:206-220
+int32_t SutraSource::add_layer(string ltype, int32_t idx, float mxoff, float myoff) {
+  current_context->set_active_device(device, 1);
+  xoff[make_pair(ltype, idx)] = mxoff;
+  yoff[make_pair(ltype, idx)] = myoff;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSource::remove_layer(string ltype, int32_t idx) {
+  current_context->set_active_device(device, 1);
+  xoff.erase(make_pair(ltype, idx));
+  yoff.erase(make_pair(ltype, idx));
+  return EXIT_SUCCESS;
+}
+
+// reference: libsutra/src.cpp/sutra_source.cpp:222-267.  The layers are the
+// "atmos" entries of the offset maps, in map order, each at its offset plus
+// the layer's accumulated shift; G / thetaML / dx / dy are the source's.
+int32_t SutraSource::raytrace(SutraAtmos* atmos, bool async) {
+  (void)async;  // one launch either way
+  if (atmos == nullptr) throw std::invalid_argument("raytrace: atmos is None");
+  if (lgs) throw std::runtime_error("raytrace: LGS cone compression is not part of this build");
+  current_context->set_active_device(device, 1);
+  vector<shwfs_layer> layers;
+  for (auto& kv : xoff) {
+    if (kv.first.first.find("atmos") != 0) continue;
+    const int32_t idx = kv.first.second;
+    if (idx < 0 || idx >= (int32_t)atmos->d_screens.size())
+      throw std::out_of_range("raytrace: no atmosphere layer of that index");
+    SutraTurbuScreen* ps = atmos->d_screens[idx];
+    CarmaObj<float>* scr = ps->d_tscreen->d_screen;
+    shwfs_layer l{};
+    l.d_screen = scr->get_data();
+    l.dim = (int32_t)scr->get_dims(1);
+    l.xoff = kv.second + ps->accumx;
+    l.yoff = yoff[kv.first] + ps->accumy;
+    l.G = G;
+    l.thetaML = thetaML;
+    l.dx = dx;
+    l.dy = dy;
+    l.delta = 1.0f;
+    layers.push_back(l);
+  }
+  if ((int32_t)layers.size() > SHWFS_MAX_LAYERS)
+    throw std::runtime_error("raytrace: more layers than SHWFS_MAX_LAYERS");
+  pending = layers;
+  if (!lazy_raytrace || pending.empty()) {
+    if (pending.empty()) return d_phase->d_screen->reset();  // no layer: the screen is zero
+    return render_layers();
+  }
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraSource::render_layers(hipStream_t stream) {
+  if (pending.empty()) return EXIT_SUCCESS;
+  current_context->set_active_device(device, 1);
+  CarmaObj<float>* scr = d_phase->d_screen;
+  shwfs_throw_if(shwfs_raytrace(scr->get_data(), (int32_t)scr->get_dims(1),
+                                (int32_t)scr->get_dims(2), pending.data(),
+                                (int32_t)pending.size(), 0, stream),
+                 "raytrace");
+  pending.clear();
+  return EXIT_SUCCESS;
+}
+
+// ============================================================ atmosphere layers
+// reference: libsutra/src.cpp/sutra_tscreen.cpp:25-80 (the holder part)
+SutraTurbuScreen::SutraTurbuScreen(CarmaContext* context, int64_t size, int64_t size2,
+                                   float amplitude_, float altitude_, float windspeed_,
+                                   float winddir_, float deltax_, float deltay_, int32_t device_)
+    : device(device_),
+      screen_size(size),
+      r0(0.f),
+      amplitude(amplitude_),
+      altitude(altitude_),
+      windspeed(windspeed_),
+      winddir(winddir_),
+      deltax(deltax_),
+      deltay(deltay_),
+      accumx(0.f),
+      accumy(0.f),
+      current_context(context) {
+  (void)size2;  // stencil size of the extrusion, not used here
+  if (size < 2) throw std::invalid_argument("TurbuScreen: size must be at least 2");
+  current_context->set_active_device(device, 1);
+  d_tscreen = new SutraPhase(context, size);
+  d_tscreen->d_screen->reset();
+}
+SutraTurbuScreen::~SutraTurbuScreen() { del(d_tscreen); }
+
+// reference: libsutra/src.cpp/sutra_atmos.cpp:24-58
+SutraAtmos::SutraAtmos(CarmaContext* context, int32_t nscreens_, float global_r0,
+                       float* r0_per_layer, int64_t* size1, int64_t* size2, float* altitude,
+                       float* windspeed, float* winddir, float* deltax, float* deltay,
+                       int32_t device)
+    : nscreens(nscreens_), r0(global_r0), current_context(context) {
+  for (int32_t i = 0; i < nscreens; ++i) {
+    // amplitude r0^(-5/6) of the layer (r0 in pixels), as the reference
+    const float amp = std::pow(r0_per_layer[i], -5.0f / 6.0f);
+    d_screens.push_back(new SutraTurbuScreen(context, size1[i], size2[i], amp, altitude[i],
+                                             windspeed[i], winddir[i], deltax[i], deltay[i],
+                                             device));
+    d_screens.back()->r0 = r0_per_layer[i];
+  }
+}
+SutraAtmos::~SutraAtmos() {
+  for (auto* s : d_screens) delete s;
+}
 
 // ============================================================ SutraWfs
 // reference: libsutra/src.cpp/sutra_wfs.cpp:25-96
@@ -263,6 +379,7 @@ int32_t SutraWfs::set_max_pix_value(int32_t v) {
 int32_t SutraWfs::slopes_geom(float* slopes, int32_t type_) {
   current_context->set_active_device(device, 1);
   if (!d_phasemap) throw std::runtime_error("slopes_geom: arrays not loaded");
+  d_gs->render_layers(stream);
   float alpha = float(0.206265 / subapd * nphase / (nphase - 1));
   shwfs_throw_if(shwfs_slopes_geom(type_, d_gs->d_phase->d_screen->get_data(), d_pupil->get_data(),
                                    d_fluxPerSub->get_data(), slopes, d_phasemap->get_data(),
@@ -496,7 +613,17 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
   f.d_intensities = d_intensities->get_data();
   f.centroid = nullptr;
   f.d_pupil_unit = d_pupil_unit ? d_pupil_unit->get_data() : nullptr;
+  if (layers_in_field_stage()) {
+    f.layers = d_gs->pending.data();
+    f.nlayers = (int32_t)d_gs->pending.size();
+  }
   return f;
+}
+
+bool SutraWfsSH::layers_in_field_stage() const {
+  const int32_t wg = get_wg_size();
+  return d_gs != nullptr && !d_gs->pending.empty() && get_method() == SHWFS_METHOD_FFT_FP32 &&
+         (wg == 0 || wg == 64);
 }
 
 // One fused launch: fillcamplipup + FFT + abs2c + fillbincube + subap_reduce +
@@ -516,6 +643,7 @@ int32_t SutraWfsSH::comp_generic() {
   if (d_gs->d_phase->d_screen->get_dims(1) != d_pupil->get_dims(1))
     throw std::runtime_error("comp_image: phase screen and pupil sizes differ");
   current_context->set_active_device(device, 1);
+  if (!layers_in_field_stage()) d_gs->render_layers(stream);
   shwfs_sh_geom g = geom();
   shwfs_frame_args f = frame_args(noise_this_call);
   if (roket && d_binimg_notnoisy != nullptr && d_binimg != nullptr) {

@@ -55,9 +55,52 @@ class SutraPhase {
   ~SutraPhase();
 };
 
+// One turbulent layer (reference: libsutra/include/sutra_tscreen.hpp:29-80):
+// its phase screen d_tscreen->d_screen (µm, x fastest), altitude and
+// accumulated wind shift accumx / accumy (pixels), which raytrace adds to a
+// source's offsets.  The screen is written by the caller (host2device of
+// d_screen); the extrusion machinery of the reference (A / B matrices,
+// stencils, move_atmos) is not part of this build.
+class SutraTurbuScreen {
+ public:
+  int32_t device;
+  SutraPhase* d_tscreen;
+  int64_t screen_size;
+  float r0, amplitude, altitude, windspeed, winddir, deltax, deltay;
+  float accumx, accumy;
+  CarmaContext* current_context;
+
+  SutraTurbuScreen(CarmaContext* context, int64_t size, int64_t size2, float amplitude,
+                   float altitude, float windspeed, float winddir, float deltax, float deltay,
+                   int32_t device);
+  ~SutraTurbuScreen();
+};
+
+// The layers of the atmosphere (reference: libsutra/include/sutra_atmos.hpp,
+// constructor libsutra/src.cpp/sutra_atmos.cpp:24-58 with the same arguments).
+class SutraAtmos {
+ public:
+  int32_t nscreens;
+  vector<SutraTurbuScreen*> d_screens;
+  float r0;
+  CarmaContext* current_context;
+
+  SutraAtmos(CarmaContext* context, int32_t nscreens, float global_r0, float* r0_per_layer,
+             int64_t* size1, int64_t* size2, float* altitude, float* windspeed, float* winddir,
+             float* deltax, float* deltay, int32_t device);
+  ~SutraAtmos();
+};
+
 // Guide-star source: holds the phase seen by the WFS (µm) and scale = 2*pi/lambda
-// (reference: libsutra/src.cpp/sutra_source.cpp:25-100).  Raytracing through
-// atmosphere / DM layers is not part of this build.
+// (reference: libsutra/src.cpp/sutra_source.cpp:25-100), and the layers it
+// sees (add_layer) with their offsets.
+//
+// raytrace(atmos) keeps the layers as `pending` instead of rendering them:
+// the FFT spot kernel of the WFS then traces its windows through them in its
+// field stage (the npup^2 screen is neither written nor read).  Any other
+// reader of the screen renders it first (render_layers): the phase property,
+// slopes_geom, the MFMA methods, batches.  set_phase / reset_phase drop the
+// pending layers.  lazy_raytrace = false renders at once, as the reference.
 class SutraSource {
  public:
   CarmaContext* current_context;
@@ -69,12 +112,20 @@ class SutraSource {
   float G, thetaML, dx, dy;
   string type;
   SutraPhase* d_phase;
+  map<std::pair<string, int32_t>, float> xoff, yoff;  // (layer type, index) -> offset
+  vector<shwfs_layer> pending;  // layers of the last raytrace, not rendered yet
+  bool lazy_raytrace;
 
   SutraSource(CarmaContext* context, float xpos, float ypos, float lambda, float mag, float zerop,
               int64_t size, string type, int32_t device);
   ~SutraSource();
   int32_t set_phase(float* data);
   int32_t reset_phase();
+  int32_t add_layer(string type, int32_t idx, float xoff, float yoff);
+  int32_t remove_layer(string type, int32_t idx);
+  int32_t raytrace(SutraAtmos* atmos, bool async = false);
+  // writes the pending layers into d_phase->d_screen (nothing when none)
+  int32_t render_layers(hipStream_t stream = nullptr);
 };
 
 class SutraWfs {
@@ -201,6 +252,9 @@ class SutraWfsSH : public SutraWfs {
   int32_t get_method() const { return campli_plan ? campli_plan->method : SHWFS_METHOD_FFT_FP32; }
   int32_t set_wg_size(int32_t wg_size);
   int32_t get_wg_size() const { return campli_plan ? campli_plan->wg_size : 0; }
+  // the guide star's pending layers are traced in the field stage of the spot
+  // kernel (FFT method, 64-lane workgroups); otherwise they are rendered first
+  bool layers_in_field_stage() const;
   // Geometry descriptor handed to the C ABI.
   shwfs_sh_geom geom() const;
   // Frame arguments of the next comp_image (advances nothing).
@@ -539,6 +593,7 @@ class SutraWfsBatch {
   // per-WFS (method, split), launch grid (0 = one workgroup per tile), XCD remap
   void set_tiles(const vector<int32_t>& methods, const vector<int32_t>& splits, int32_t grid,
                  int32_t remap);
+  void render_sources();
   vector<int32_t> methods() const;
   vector<int32_t> splits() const;
   int32_t grid() const;

@@ -44,6 +44,8 @@ SutraWfsBatch::SutraWfsBatch(CarmaContext* context, vector<SutraWfsSH*> wfs_,
       cargs[i] = c->centroid_args(c->d_intensities->get_data(), d_slopes.back()->get_data(),
                                   (int32_t)w->nvalid, (int32_t)w->npix,
                                   (int32_t)w->d_binimg->get_dims(1));
+      // the learned centroider runs per WFS (get_cog), not inside a batch
+      if (cargs[i].kind == SHWFS_CENTROID_MLP) cargs[i].kind = SHWFS_CENTROID_NONE;
       f.centroid = &cargs[i];
     }
     frames.push_back(f);
@@ -95,8 +97,15 @@ int32_t SutraWfsBatch::remap() const {
   return r;
 }
 
+// Batches read rendered screens: pending raytrace layers are drawn first.
+void SutraWfsBatch::render_sources() {
+  for (auto* w : wfs)
+    if (w->d_gs) w->d_gs->render_layers(nullptr);
+}
+
 float SutraWfsBatch::autotune(uint32_t method_mask, int32_t reps) {
   current_context->set_active_device(device, 1);
+  render_sources();
   float us = 0.f;
   shwfs_throw_if(shwfs_batch_autotune(handle, method_mask, reps, nullptr, &us), "WfsBatch.autotune");
   return us;
@@ -114,11 +123,13 @@ uint32_t SutraWfsBatch::frame() {
 
 void SutraWfsBatch::launch() {
   current_context->set_active_device(device, 1);
+  render_sources();
   shwfs_throw_if(shwfs_batch_launch(handle, nullptr), "WfsBatch.launch");
 }
 
 void SutraWfsBatch::capture(int32_t frames) {
   current_context->set_active_device(device, 1);
+  render_sources();
   shwfs_throw_if(shwfs_batch_capture(handle, frames, nullptr), "WfsBatch.capture");
 }
 
@@ -141,11 +152,13 @@ static double timed(const std::function<void()>& body) {
 
 double SutraWfsBatch::replay(int32_t ngraphs) {
   current_context->set_active_device(device, 1);
+  render_sources();
   return timed([&] { shwfs_throw_if(shwfs_batch_replay(handle, ngraphs, nullptr), "WfsBatch.replay"); });
 }
 
 double SutraWfsBatch::run(int32_t nframes) {
   current_context->set_active_device(device, 1);
+  render_sources();
   return timed([&] {
     for (int32_t k = 0; k < nframes; ++k)
       shwfs_throw_if(shwfs_batch_launch(handle, nullptr), "WfsBatch.run");

@@ -586,17 +586,21 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
       throw std::runtime_error("comp_frame: WCoG weights not initialised for this WFS");
   }
   wfs->current_context->set_active_device(wfs->device, 1);
-  // not fused: spots, then centroids (roket keeps the image before the noise,
-  // which the fused launch does not write)
-  if (c->get_type() == "mlp" || c->get_type() == "bpcog" || wfs->roket) {
-    wfs->comp_image(noise);
-    return c->get_cog(intensities, slopes, noise);
-  }
-  shwfs_sh_geom g = wfs->geom();
-  shwfs_frame_args f = wfs->frame_args(noise);
   shwfs_centroid_args a =
       c->centroid_args(intensities, slopes, (int32_t)wfs->nvalid, (int32_t)wfs->npix,
                        (int32_t)wfs->d_binimg->get_dims(1));
+  // not fused: spots, then centroids (roket keeps the image before the noise,
+  // which the fused launch does not write; the learned centroider is fused
+  // only where centroid_args says so)
+  if (a.kind == SHWFS_CENTROID_NONE || c->get_type() == "bpcog" || wfs->roket) {
+    wfs->comp_image(noise);
+    return c->get_cog(intensities, slopes, noise);
+  }
+  // the fused learned centroider and the traced field stage are separate kernels
+  if (a.kind == SHWFS_CENTROID_MLP || !wfs->layers_in_field_stage())
+    wfs->d_gs->render_layers(wfs->stream);
+  shwfs_sh_geom g = wfs->geom();
+  shwfs_frame_args f = wfs->frame_args(noise);
   f.centroid = &a;
   shwfs_throw_if(shwfs_sh_comp_image(&g, &f, wfs->stream), "comp_frame");
   wfs->noise_call++;

@@ -176,11 +176,37 @@ int32_t SutraCentroiderMlp<Tin, T>::get_cog() {
   return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
 }
 
+// Fused after binning (kind SHWFS_CENTROID_MLP) when the attached WFS
+// computes its spots with the FFT method, 8x8-pixel spots and nfft 16 or 32;
+// otherwise NONE (comp_frame runs the spots, then get_cog).
 template <class Tin, class T>
-shwfs_centroid_args SutraCentroiderMlp<Tin, T>::centroid_args(float*, T*, int32_t, int32_t,
-                                                              int32_t) {
+shwfs_centroid_args SutraCentroiderMlp<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                              int32_t nvalid, int32_t npix,
+                                                              int32_t size) {
   shwfs_centroid_args a{};
   a.kind = SHWFS_CENTROID_NONE;
+  auto* sh = dynamic_cast<SutraWfsSH*>(this->wfs);
+  if (!sh || sh->get_method() != SHWFS_METHOD_FFT_FP32 || npix != 8 ||
+      (sh->nfft != 16 && sh->nfft != 32) || d_fallback->get_nb_elements() < nvalid)
+    return a;
+  a.kind = SHWFS_CENTROID_MLP;
+  a.nvalid = nvalid;
+  a.npix = npix;
+  a.size = size;
+  a.d_validx = this->d_validx ? this->d_validx->get_data() : nullptr;
+  a.d_validy = this->d_validy ? this->d_validy->get_data() : nullptr;
+  a.offset = this->offset;
+  a.scale = this->scale;
+  a.d_ref = this->d_centroids_ref->get_data();
+  a.d_slopes = slopes;
+  a.d_intensities = intensities;
+  a.slope_order = (int32_t)this->slope_order;
+  a.d_mlp_params = d_params->get_data();
+  a.mlp_x_scale = x_scale;
+  a.mlp_y_scale = this->scale;
+  a.mlp_flux_lo = flux_lo;
+  a.mlp_flux_hi = flux_hi;
+  a.d_mlp_fallback = d_fallback->get_data();
   return a;
 }
 
@@ -197,6 +223,7 @@ int64_t make_mlp_training_set(SutraWfsSH* wfs, CarmaObj<float>* base, float r0_b
   if (X->get_nb_elements() < n * 64 || T->get_nb_elements() < n * 2)
     throw std::invalid_argument("make_mlp_training_set: X / T too small");
   CarmaObj<float>* scr = wfs->d_gs->d_phase->d_screen;
+  wfs->d_gs->pending.clear();  // the screen is written here, frame by frame
   const int32_t npup = (int32_t)scr->get_dims(1);
   const int32_t side = (int32_t)base->get_dims(1);
   if (base->get_dims(2) != side || side < npup)

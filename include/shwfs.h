@@ -42,7 +42,8 @@ enum {
   SHWFS_CENTROID_NONE = 0,
   SHWFS_CENTROID_COG = 1,
   SHWFS_CENTROID_TCOG = 2,
-  SHWFS_CENTROID_WCOG = 3
+  SHWFS_CENTROID_WCOG = 3,
+  SHWFS_CENTROID_MLP = 4  /* learned centroider (shwfs_mlp_*), fused after binning */
 };
 
 /* Geometry of one SH WFS: sizes + device tables (all owned by the caller).
@@ -88,7 +89,32 @@ typedef struct shwfs_centroid_args {
   float* d_intensities;    /* [nvalid] centroider flux */
   int32_t slope_order;     /* 0 untied (x..., y...), 1 interlaced (x, y, x, y...) */
   int32_t fp64_moments;    /* 1: flux and first moments summed in float64 */
+  /* SHWFS_CENTROID_MLP only (fused into the spot kernel of the FFT method;
+   * 8x8-pixel spots, nfft 16 or 32): as shwfs_mlp_centroids over the spot
+   * cube, in the same launch as the spots.  Other kinds ignore these. */
+  const float* d_mlp_params; /* [SHWFS_MLP_NPARAM] */
+  float mlp_x_scale;         /* input scale of the spot pixels */
+  float mlp_y_scale;         /* network output (pixels) -> slope units */
+  float mlp_flux_lo;         /* spots with flux outside [lo, hi] take their CoG */
+  float mlp_flux_hi;
+  int32_t* d_mlp_fallback;   /* [nvalid] 1 where the CoG was used, or NULL */
 } shwfs_centroid_args;
+
+/* One layer seen by a source: the phase screen of an atmospheric layer (or
+ * a DM shape) and where the source's footprint lies in it.  Reference: the
+ * (type, index) -> (xoff, yoff) maps of SutraSource filled by add_layer
+ * (libsutra/src.cpp/sutra_source.cpp:206-213), read by SutraSource::raytrace
+ * (222-267) with the layer's accumulated shift (accumx, accumy) added. */
+#define SHWFS_MAX_LAYERS 16
+typedef struct shwfs_layer {
+  const float* d_screen; /* [dim * dim] layer phase (x fastest), micrometres */
+  int32_t dim;           /* side of the layer screen (>= 2) */
+  float xoff, yoff;      /* footprint offset in the layer, pixels (accum included) */
+  float G;               /* magnification (1: none) */
+  float thetaML;         /* rotation, radians */
+  float dx, dy;          /* extra shift, pixels */
+  float delta;           /* cone compression 1 - altitude / LGS height (1: NGS) */
+} shwfs_layer;
 
 /* One frame of one SH WFS. */
 typedef struct shwfs_frame_args {
@@ -109,6 +135,12 @@ typedef struct shwfs_frame_args {
   const int32_t* d_pupil_unit; /* NULL, or [nvalid] 1 where the pupil window of the
                                   sub-aperture is all ones (shwfs_sh_unit_pupil of
                                   this d_pupil): the kernels skip reading it */
+  const shwfs_layer* layers;   /* host array [nlayers], or NULL */
+  int32_t nlayers;             /* > 0: the phase is the raytrace of these layers
+                                  over an npup x npup source screen, computed in
+                                  the field stage (d_phase is not read and may be
+                                  NULL): the values shwfs_raytrace would write
+                                  there, bit for bit.  FFT method only. */
 } shwfs_frame_args;
 
 /* ---- spot image (SutraWfsSH::comp_generic + fill_binimage, fused) ---------
@@ -338,6 +370,21 @@ int32_t shwfs_mlp_centroids(const float* d_params, void* d_work, const float* d_
                             float flux_hi, float offset, float scale, float y_scale,
                             const float* d_ref, int32_t slope_order, float* d_slopes,
                             float* d_intensities, int32_t* d_fallback, void* stream);
+
+/* ---- guide-star raytrace -----------------------------------------------------------
+ * d_out (nx x ny, x fastest) = (accumulate ? d_out : 0) + sum over the layers,
+ * in order, of each layer's bilinear value at the traced point:
+ *   x1 = (x - nx/2) G, y1 = (y - ny/2) G,
+ *   xref = ((cos t x1 - sin t y1) + nx/2 + xoff/delta) delta + dx (y alike),
+ *   (1-u)(1-v) L[i,j] + u(1-v) L[i+1,j] + (1-u) v L[i,j+1] + u v L[i+1,j+1]
+ * with i = trunc(xref), u = xref - i; a point whose square leaves the layer
+ * adds nothing (the reference adds nothing past the far edge and reads
+ * before the layer for a negative i or j; here those add nothing too).
+ * Replaces SutraSource::raytrace(SutraAtmos*) (libsutra/src.cpp/
+ * sutra_source.cpp:222-267): cudaMemset + target_raytrace / raytrace_krnl
+ * per layer (libsutra/src.cu/sutra_target.cu:25-98), in one launch. */
+int32_t shwfs_raytrace(float* d_out, int32_t nx, int32_t ny, const shwfs_layer* layers,
+                       int32_t nlayers, int32_t accumulate, void* stream);
 
 /* ---- phase window (synthetic inputs) ---------------------------------------------
  * d_dst[x + y*n] = gain * d_src[(ox + x) + (oy + y)*src_side] for x, y < n:

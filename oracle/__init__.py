@@ -19,6 +19,8 @@ loops for the small cases, the algorithm of the reference path:
 * ``turbulence``- von Karman phase structure function (closed form used by the
                   reference's ``rodconan``, shesha/shesha/util/iterkolmo.py:313-351)
 * ``mlp``       - float64 forward/backward of the per-subaperture MLP centroider
+* ``raytrace``  - guide-star raytrace through layers (libsutra/src.cpp/
+                  sutra_source.cpp:222-267, libsutra/src.cu/sutra_target.cu:25-77)
 
 Pinning status of every function is listed in ``PINNING`` below; each entry
 names the tests that pin it.  Any function marked "parity unpinned" has no
@@ -38,4 +40,8 @@ PINNING = {
     "turbulence.structure_function": "pinned: Kolmogorov limit 6.88 (r/r0)^(5/3) and "
                                      "saturation 2*variance for r >> L0",
     "mlp": "pinned: central finite differences of the float64 loss",
+    "raytrace.raytrace": "pinned: integer shifts copy the layer exactly, constant layer and its "
+                         "edge, linear layers reproduced at the traced coordinates (rotation, "
+                         "magnification, compression, shifts), layers add "
+                         "(tests/test_oracle_raytrace.py)",
 }

@@ -14,7 +14,7 @@ SHWFS_ERR_HIP = -3
 SHWFS_ERR_STATE = -4
 
 METHOD_FFT_FP32, METHOD_DFT_BF16X3, METHOD_DFT_FP16 = 0, 1, 2
-CENTROID_NONE, CENTROID_COG, CENTROID_TCOG, CENTROID_WCOG = 0, 1, 2, 3
+CENTROID_NONE, CENTROID_COG, CENTROID_TCOG, CENTROID_WCOG, CENTROID_MLP = 0, 1, 2, 3, 4
 
 i32, u32, u64, f32, vp = C.c_int32, C.c_uint32, C.c_uint64, C.c_float, C.c_void_p
 
@@ -31,14 +31,22 @@ class CentroidArgs(C.Structure):
     _fields_ = [("kind", i32), ("nvalid", i32), ("npix", i32), ("size", i32),
                 ("d_validx", vp), ("d_validy", vp), ("offset", f32), ("scale", f32),
                 ("threshold", f32), ("d_weights", vp), ("d_ref", vp), ("d_slopes", vp),
-                ("d_intensities", vp), ("slope_order", i32), ("fp64_moments", i32)]
+                ("d_intensities", vp), ("slope_order", i32), ("fp64_moments", i32),
+                ("d_mlp_params", vp), ("mlp_x_scale", f32), ("mlp_y_scale", f32),
+                ("mlp_flux_lo", f32), ("mlp_flux_hi", f32), ("d_mlp_fallback", vp)]
+
+
+class Layer(C.Structure):
+    _fields_ = [("d_screen", vp), ("dim", i32), ("xoff", f32), ("yoff", f32), ("G", f32),
+                ("thetaML", f32), ("dx", f32), ("dy", f32), ("delta", f32)]
 
 
 class FrameArgs(C.Structure):
     _fields_ = [("d_phase", vp), ("d_pupil", vp), ("scale", f32), ("nphot", f32), ("noise", f32),
                 ("noise_background", f32), ("seed", u64), ("call", u32), ("method", i32),
                 ("wg_size", i32), ("d_bincube", vp), ("d_binimg", vp), ("d_intensities", vp),
-                ("centroid", C.POINTER(CentroidArgs)), ("d_pupil_unit", vp)]
+                ("centroid", C.POINTER(CentroidArgs)), ("d_pupil_unit", vp),
+                ("layers", C.POINTER(Layer)), ("nlayers", i32)]
 
 
 class BatchTile(C.Structure):
@@ -70,6 +78,7 @@ def load():
     lib.shwfs_tt_filter.argtypes = [vp, vp, vp, vp, i32, vp]
     lib.shwfs_fill_binimg.argtypes = [vp, vp, vp, vp, i32, i32, i32, f32, u64, u32, u32, vp]
     lib.shwfs_upload.argtypes = [vp, C.c_int64, vp, C.c_int64, i32, vp]
+    lib.shwfs_raytrace.argtypes = [vp, i32, i32, C.POINTER(Layer), i32, i32, vp]
     lib.shwfs_mfma_probe.argtypes = [vp, vp, vp, i32, vp]
     lib.shwfs_batch_create.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), i32,
                                        C.POINTER(vp)]
