@@ -474,6 +474,41 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
   return SHWFS_OK;
 }
 
+// ---- device scratch of shwfs_sh_comp_image_noalloc --------------------------
+// The MFMA DFT operand table of the frame's shape (the largest, nfft 64 with
+// a 32-point padded window in split-BF16x3, is 32 KiB), rebuilt into this
+// array by every call of that entry point.  Each device holds its own copy
+// for the life of the process.  Host code uses the address that
+// hipGetSymbolAddress returns per device (scratch_address below), never the
+// array's name.
+constexpr size_t kScratchBytes = SHWFS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current; NULL with the error set on failure.
+static void* scratch_address() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("comp_image_noalloc: no current device (or device index above %d)", kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("comp_image_noalloc: hipGetSymbolAddress of the scratch array failed: %s",
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    addr[dev] = ptr;
+  }
+  return addr[dev];
+}
+
 }  // namespace shwfs
 
 using namespace shwfs;
@@ -559,6 +594,45 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, v
   }
   if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
   return launch_image_dft(p, g->nfft, f->method, st);
+}
+
+int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* g, const shwfs_frame_args* f,
+                                    void* stream) {
+  ImageParams p;
+  int32_t rc = image_params(g, f, &p);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const void* wtab = nullptr;
+  if (f->method != SHWFS_METHOD_FFT_FP32) {
+    void* buf = scratch_address();
+    if (!buf) return SHWFS_ERR_HIP;
+    // the operand table of this shape, written into the scratch array ahead
+    // of the spot kernel on the same stream (a size beyond the array is
+    // rejected before any launch)
+    size_t bytes = 0;
+    rc = build_dft_operand_table(g->nfft, g->nphase, f->method, buf, kScratchBytes, st, &bytes);
+    if (rc) return rc;
+    wtab = buf;
+  }
+  if (f->d_binimg && f->noise_background > 0.0f) {
+    const long n = (long)p.nsub * p.nsub;
+    rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
+    if (rc) return rc;
+  }
+  if (f->nlayers > 0) {
+    RayParams r;
+    rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
+    if (rc) return rc;
+    return launch_image_fft_ray(p, r, g->nfft, st);
+  }
+  if (p.ckind == SHWFS_CENTROID_MLP) {
+    const shwfs_centroid_args* c = f->centroid;
+    const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
+                      c->mlp_flux_lo, c->mlp_flux_hi};
+    return launch_image_fft_mlp(p, m, g->nfft, st);
+  }
+  if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
+  return launch_image_dft(p, g->nfft, f->method, st, wtab);
 }
 
 int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_centroid_args* a,

@@ -153,6 +153,21 @@ typedef struct shwfs_frame_args {
  * plus an optional fused get_centroids. */
 int32_t shwfs_sh_comp_image(const shwfs_sh_geom* geom, const shwfs_frame_args* frame, void* stream);
 
+/* The same frame with no device allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  shwfs_sh_comp_image builds the MFMA DFT
+ * operand table of a shape in memory it allocates on first use; this entry
+ * instead rebuilds that table, on `stream`, into a static device array of
+ * SHWFS_NOALLOC_SCRATCH_BYTES held by each device until the process ends
+ * (every DFT shape fits: nfft <= 64, nphase <= 32), then runs the same
+ * kernels.  Results equal shwfs_sh_comp_image's bit for bit.  All calls on
+ * one device share that array: calls that overlap on one device (other
+ * streams, other host threads) race and must be ordered by the caller.
+ * The FFT method uses no scratch. */
+#define SHWFS_NOALLOC_SCRATCH_BYTES 32768
+int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* geom, const shwfs_frame_args* frame,
+                                    void* stream);
+
 /* d_flags[s] = 1 if every pupil value of the window of sub-aperture s is
  * exactly 1, else 0 (for shwfs_frame_args.d_pupil_unit).  No counterpart in
  * the reference: fully illuminated sub-apertures then skip the pupil read. */

@@ -62,6 +62,7 @@ def declared_functions():
 def load():
     lib = C.CDLL(LIB_PATH)
     lib.shwfs_sh_comp_image.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), vp]
+    lib.shwfs_sh_comp_image_noalloc.argtypes = [C.POINTER(ShGeom), C.POINTER(FrameArgs), vp]
     lib.shwfs_sh_check_geom.argtypes = [C.POINTER(ShGeom), i32]
     lib.shwfs_sh_unit_pupil.argtypes = [C.POINTER(ShGeom), vp, vp, vp]
     lib.shwfs_binmap_block_origin.argtypes = [vp, i32, i32, i32, C.POINTER(i32)]
