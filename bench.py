@@ -225,6 +225,34 @@ def measure_uneven(method_mask=3, graphs=10, frames_per_graph=100):
                 nvalid=[int(s.wfs.nvalid) for s in bs.systems])
 
 
+def measure_uneven_fixed(method, graphs=4, frames_per_graph=50):
+    """The eight-WFS frame with every WFS on one method (one workgroup per
+    tile, no autotuner), and each WFS alone (its own one-frame kernel,
+    graph-replayed): where the uneven frame's time goes."""
+    import sutra
+    from shwfs import configs, system
+    cfgs = configs.batch()
+    bs = system.BatchSystem(cfgs)
+    bs.set_phases([system.make_screens(c, 1)[0] for c in cfgs])
+    n = len(cfgs)
+    bs.batch.set_tiles([method] * n, [0] * n)
+    bs.batch.capture(frames_per_graph)
+    bs.batch.replay(1)
+    sync()
+    ms = bs.batch.replay(graphs)
+    sync()
+    frames = graphs * frames_per_graph
+    per_wfs = []
+    for s in bs.systems:
+        s.wfs.set_method(method)
+        slopes = s.rtc.d_control[0].d_centroids
+        wms = sutra.run_frames_graph(s.wfs, s.centro, slopes, frames_per_graph, graphs)
+        per_wfs.append(round(1e3 * wms / frames, 3))
+    return dict(method=METHOD_NAMES[method], us_per_frame=round(1e3 * ms / frames, 3),
+                alone_us_per_frame=per_wfs, alone_sum_us=round(sum(per_wfs), 3),
+                nvalid=[int(s.wfs.nvalid) for s in bs.systems])
+
+
 def oracle_frame(s, phase, thr):
     from oracle import centroid as oc
     from oracle import wfs as ow
@@ -295,6 +323,9 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the eight-WFS batch")
     ap.add_argument("--uneven-only", action="store_true",
                     help="run only a short eight-WFS batch (for kernel profiles)")
+    ap.add_argument("--uneven-fixed", type=str, default="",
+                    help="with --uneven-only: comma-separated methods, each run on all eight WFS "
+                         "without the autotuner, with the time of each WFS alone")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run a device copy of this many MiB first (HBM counter calibration)")
     a = ap.parse_args()
@@ -322,6 +353,10 @@ def main():
         sync()
         del src, dst
 
+    if a.uneven_only and a.uneven_fixed:
+        out = [measure_uneven_fixed(int(m)) for m in a.uneven_fixed.split(",") if m.strip()]
+        print(json.dumps(dict(uneven_fixed=out)), flush=True)
+        return
     if a.uneven_only:
         # a short run of the eight-WFS batch alone (kernel profiles of it)
         out = {}

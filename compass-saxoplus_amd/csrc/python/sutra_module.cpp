@@ -131,6 +131,13 @@ static void declare_wfs(py::module& m) {
       .def_property_readonly("device", [](SutraWfs& w) { return w.device; })
       .def_property_readonly("type", [](SutraWfs& w) { return w.type; })
       .def_property_readonly("is_low_order", [](SutraWfs& w) { return w.is_low_order; })
+      .def_property_readonly("kernconv", [](SutraWfs& w) { return w.kernconv; })
+      .def("set_kernconv", [](SutraWfs& w, bool k) { w.kernconv = k; }, py::arg("kernconv"),
+           "convolve the HR image with d_ftkernel before binning (the rtc sets it for imats)")
+      .def_property_readonly("d_hrmap", [](SutraWfs& w) { return w.d_hrmap; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_ftkernel", [](SutraWfs& w) { return w.d_ftkernel; },
+                             py::return_value_policy::reference_internal)
       .def_property_readonly("nxsub", [](SutraWfs& w) { return w.nxsub; })
       .def_property_readonly("nvalid", [](SutraWfs& w) { return w.nvalid; })
       .def_property_readonly("npix", [](SutraWfs& w) { return w.npix; })
@@ -220,7 +227,8 @@ static void declare_wfs_sh(py::module& m) {
              if (phasemap.size() < np2 * w.nvalid || binmap.size() != w.nrebin * w.nrebin * w.npix * w.npix ||
                  offsets.size() != np2 || fluxPerSub.size() < nv || validsubsx.size() < nv ||
                  validsubsy.size() < nv || istart.size() < w.nvalid || jstart.size() < w.nvalid ||
-                 ttprojmat.size() != np2 * 2 * w.nvalid || kernel.size() != w.ntot * w.ntot)
+                 ttprojmat.size() != np2 * 2 * w.nvalid || kernel.size() != w.ntot * w.ntot ||
+                 (w.ntot != w.nfft && hrmap.size() != w.nfft * w.nfft))
                throw std::runtime_error("load_arrays: array sizes do not match the WFS");
              return w.load_arrays(phasemap.mutable_data(), hrmap.mutable_data(),
                                   binmap.mutable_data(), offsets.mutable_data(),

@@ -314,6 +314,10 @@ int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* f
       rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: the learned centroider is not fused in batches");
       break;
     }
+    if (geoms[i].ntot != geoms[i].nfft || geoms[i].kernconv) {
+      rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: ntot != nfft / kernconv run per WFS");
+      break;
+    }
     if (frames[i].nlayers > 0) {
       rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: batches read rendered phase screens (no layers)");
       break;

@@ -371,8 +371,18 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
     return fail(SHWFS_ERR_ARG, "geometry sizes must be positive");
   if (!(g->nfft == 8 || g->nfft == 16 || g->nfft == 32 || g->nfft == 64))
     return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
-  if (g->ntot != g->nfft)
-    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft (enlarged HR field of view) is not implemented");
+  const bool ext = g->ntot != g->nfft || g->kernconv;
+  if (g->ntot != g->nfft) {
+    if (!(g->ntot == 16 || g->ntot == 32 || g->ntot == 64) || g->ntot < g->nfft || g->nfft < 16)
+      return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft: needs nfft in {16, 32, 64} and ntot a "
+                                         "power of two in (nfft, 64]");
+    if (!g->d_hrmap) return fail(SHWFS_ERR_STATE, "ntot != nfft: hrmap not loaded");
+  }
+  if (g->kernconv && !g->d_ftkernel) return fail(SHWFS_ERR_STATE, "kernconv: ftkernel not loaded");
+  if (ext && method != SHWFS_METHOD_FFT_FP32)
+    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv: FFT method only");
+  if (ext && g->nfft < 16)
+    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv: needs nfft in {16, 32, 64}");
   if (g->nphase > g->nfft) return fail(SHWFS_ERR_ARG, "nphase must be <= nfft");
   if (g->npix * g->nrebin > g->ntot) return fail(SHWFS_ERR_ARG, "npix*nrebin must be <= ntot");
   // the kernels index the spot cube, the detector image and the screen with
@@ -398,6 +408,10 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
   if (!f || (!f->d_phase && f->nlayers <= 0) || !f->d_pupil || !f->d_bincube ||
       !f->d_intensities)
     return fail(SHWFS_ERR_ARG, "frame: phase (or layers), pupil, bincube and intensities are required");
+  if ((g->ntot != g->nfft || g->kernconv) &&
+      (f->nlayers > 0 || (f->centroid && f->centroid->kind == SHWFS_CENTROID_MLP)))
+    return fail(SHWFS_ERR_UNSUPPORTED,
+                "ntot != nfft / kernconv: no layers and no fused learned centroider");
   if (f->nlayers > 0) {
     if (f->method != SHWFS_METHOD_FFT_FP32)
       return fail(SHWFS_ERR_UNSUPPORTED, "frame: the raytrace in the field stage runs in the FFT method only");
@@ -580,6 +594,12 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, v
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
     if (rc) return rc;
   }
+  if (g->ntot != g->nfft || g->kernconv) {
+    const ExtParams x{g->ntot != g->nfft ? g->d_hrmap : nullptr,
+                      g->kernconv ? reinterpret_cast<const float2*>(g->d_ftkernel) : nullptr,
+                      g->kernconv ? 1 : 0};
+    return launch_image_ext(p, x, g->nfft, g->ntot, st);
+  }
   if (f->nlayers > 0) {
     RayParams r;
     rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
@@ -618,6 +638,12 @@ int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* g, const shwfs_frame_ar
     const long n = (long)p.nsub * p.nsub;
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
     if (rc) return rc;
+  }
+  if (g->ntot != g->nfft || g->kernconv) {
+    const ExtParams x{g->ntot != g->nfft ? g->d_hrmap : nullptr,
+                      g->kernconv ? reinterpret_cast<const float2*>(g->d_ftkernel) : nullptr,
+                      g->kernconv ? 1 : 0};
+    return launch_image_ext(p, x, g->nfft, g->ntot, st);
   }
   if (f->nlayers > 0) {
     RayParams r;

@@ -161,6 +161,15 @@ int32_t check_geom(const shwfs_sh_geom* g, int method);
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
 
 int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
+// The enlarged HR grid (ntot != nfft) and the kernel convolution: one
+// workgroup per sub-aperture (shwfs_image_ext.hip).
+struct ExtParams {
+  const int* hrmap;        // [nfft^2] or NULL when ntot == nfft
+  const float2* ftkernel;  // [ntot^2] or NULL without convolution
+  int conv;
+};
+int32_t launch_image_ext(const ImageParams& p, const ExtParams& x, int nfft, int ntot,
+                         hipStream_t s);
 // The FFT spot kernel tracing its window phases through layers (RayParams,
 // shwfs_raytrace.hpp) instead of reading p.phase.
 struct RayParams;

@@ -437,8 +437,7 @@ int32_t SutraWfsSH::define_mpi_rank(int32_t rank_, int32_t size) {
 int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_plans) {
   (void)fttotim_plans;
   current_context->set_active_device(device, 1);
-  if (ntot != nfft)
-    throw std::runtime_error("Ntot != Nfft (enlarged HR field of view) is not part of this build");
+  if (ntot < nfft) throw std::runtime_error("allocate_buffers: ntot must be >= nfft");
   int64_t d[5];
   if (rank == 0) {
     d_binimg = new CarmaObj<float>(current_context, dims2(d, npix * nxsub, npix * nxsub));
@@ -466,6 +465,9 @@ int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_pla
   d_bincube = new CarmaObj<float>(current_context, dims3(d, npix, npix, nvalid_tot));
   d_slopes = new CarmaObj<float>(current_context, dims1(d, 2 * nvalid));
   d_ftkernel = new CarmaObj<hipFloatComplex>(current_context, dims2(d, ntot, ntot));
+  // enlarged HR grid: the ntot-grid index of each nfft-grid element
+  // (reference: libsutra/src.cpp/sutra_wfs_sh.cpp:196-204)
+  if (ntot != nfft) d_hrmap = new CarmaObj<int32_t>(current_context, dims1(d, nfft * nfft));
   d_offsets = new CarmaObj<float>(current_context, dims2(d, nphase, nphase));
   d_binmap = new CarmaObj<int32_t>(current_context, dims2(d, nrebin * nrebin, npix * npix));
   d_intensities = new CarmaObj<float>(current_context, dims1(d, nvalid_tot));
@@ -517,7 +519,6 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
                                 float* offsets, float* fluxPerSub, int32_t* validsubsx,
                                 int32_t* validsubsy, int32_t* istart, int32_t* jstart,
                                 float* ttprojmat, hipFloatComplex* kernel) {
-  (void)hrmap;
   if (d_bincube == nullptr)
     throw std::runtime_error("ERROR : d_bincube not initialized, did you do the allocate_buffers?");
   current_context->set_active_device(device, 1);
@@ -534,8 +535,14 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
       throw std::runtime_error("load_arrays: spot window outside the detector image");
   }
   for (int64_t i = 0; i < nrebin * nrebin * npix * npix; ++i)
-    if (binmap[i] < 0 || binmap[i] >= nfft * nfft)
+    if (binmap[i] < 0 || binmap[i] >= ntot * ntot)
       throw std::runtime_error("load_arrays: binmap entry outside the HR grid");
+  if (ntot != nfft) {
+    for (int64_t i = 0; i < nfft * nfft; ++i)
+      if (hrmap[i] < 0 || hrmap[i] >= ntot * ntot)
+        throw std::runtime_error("load_arrays: hrmap entry outside the enlarged HR grid");
+    d_hrmap->host2device(hrmap);  // reference: sutra_wfs_sh.cpp:323
+  }
   for (int64_t i = 0; i < nphase * nphase * nvalid; ++i)
     if (phasemap[offset * nphase * nphase + i] < 0 || phasemap[offset * nphase * nphase + i] >= npup * npup)
       throw std::runtime_error("load_arrays: phasemap entry outside the phase screen");
@@ -549,9 +556,11 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
   d_validpuppixx->host2device(istart);
   d_validpuppixy->host2device(jstart);
   d_ftkernel->host2device(kernel);
-  int32_t origin = 0;
-  int32_t rc = shwfs_binmap_block_origin(binmap, (int32_t)nfft, (int32_t)npix, (int32_t)nrebin, &origin);
-  if (rc < 0) shwfs_throw_if(rc, "load_arrays");
+  int32_t origin = 0, rc = 0;
+  if (ntot == nfft) {  // the block form is a property of the nfft grid
+    rc = shwfs_binmap_block_origin(binmap, (int32_t)nfft, (int32_t)npix, (int32_t)nrebin, &origin);
+    if (rc < 0) shwfs_throw_if(rc, "load_arrays");
+  }
   bin_block = rc;
   bin_origin = rc ? origin : 0;
   arrays_loaded = true;
@@ -592,6 +601,9 @@ shwfs_sh_geom SutraWfsSH::geom() const {
     g.d_binmap = d_binmap->get_data();
     g.binmap_block = bin_block;
     g.bin_origin = bin_origin;
+    g.d_hrmap = d_hrmap ? d_hrmap->get_data() : nullptr;
+    g.d_ftkernel = reinterpret_cast<const float*>(d_ftkernel->get_data());
+    g.kernconv = kernconv ? 1 : 0;
   }
   return g;
 }

@@ -15,6 +15,8 @@ quantum pixels), the phase screens and the centroider.
           own pupil, screen and r0 in 0.10-0.20 m
   lowflux D=8 m, 40x40 SH, pdiam 16, Nfft 32, 8x8 px, 50 ph/subap Poisson +
           3 e- readout
+  wide    D=4 m, 8x8 SH, pdiam 8, Nfft 16, 3x3 binning to 8x8 px: the 24-point
+          detector window needs the enlarged 32-point HR grid (Ntot 32)
 All: annular pupil (central obstruction 0.12), fraction threshold 0.5,
 lambda 0.5 um, L0 25 m.
 """
@@ -68,7 +70,11 @@ def batch():
     return out
 
 
-WORKLOADS = {"small": small, "noisy": noisy, "large": large, "lowflux": lowflux}
+def wide():
+    return sh_config(4.0, 8, 8, 8, 16, 3, r0=0.16, seeds=(1,), name="wide")
+
+
+WORKLOADS = {"small": small, "noisy": noisy, "large": large, "lowflux": lowflux, "wide": wide}
 
 
 def get(name):

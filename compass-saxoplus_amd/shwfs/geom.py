@@ -214,6 +214,21 @@ def _binmap(p_wfs):
     return out
 
 
+def _hrmap(p_wfs):
+    """Ntot-grid flat index (u + v*Ntot) of each Nfft-grid HR element
+    (u + v*Nfft), geom_init.py:847-866: the Nfft block at (Ntot - Nfft)/2,
+    rolled by Ntot//2 on both axes, its cells in increasing flat index; a
+    (2, 2) placeholder when Ntot == Nfft (the reference uploads it unused)."""
+    Ntot, Nfft = p_wfs._Ntot, p_wfs._Nfft
+    if Ntot == Nfft:
+        return np.zeros((2, 2), dtype=np.int32)
+    indi = int((Ntot - Nfft) / 2.0)
+    block = np.zeros((Ntot, Ntot), dtype=bool)
+    block[indi:indi + Nfft, indi:indi + Nfft] = True
+    block = np.roll(np.roll(block, Ntot // 2, axis=0), Ntot // 2, axis=1)
+    return np.flatnonzero(block.ravel()).astype(np.int32).reshape(-1, 1)
+
+
 def init_sh_geom(p_wfs, p_tel, p_geom, ittime=1.0, r0=None):
     pd = p_wfs._pdiam
     starts, flux = _valid_subaps(p_wfs, p_geom)
@@ -244,9 +259,7 @@ def init_sh_geom(p_wfs, p_tel, p_geom, ittime=1.0, r0=None):
         h = np.linspace(0, 2 * np.pi, p_wfs._Nfft + 1)[0:pd] / 2.0
         p_wfs._halfxy = (h[None, :] + h[:, None]).astype(np.float32)
 
-    if p_wfs._Ntot != p_wfs._Nfft:
-        raise NotImplementedError("Ntot != Nfft (enlarged HR field of view) is not part of this build")
-    p_wfs._hrmap = np.zeros((2, 2), dtype=np.int32)
+    p_wfs._hrmap = _hrmap(p_wfs)
     p_wfs._binmap = _binmap(p_wfs)
 
     # seeing + detector blur kernel (only read with kernel convolution; kept

@@ -54,7 +54,8 @@ typedef struct shwfs_sh_geom {
   int32_t nphase;  /* phase points per sub-aperture side (pdiam) */
   int32_t npup;    /* side of the phase screen / pupil arrays */
   int32_t nfft;    /* FFT support side (8, 16, 32 or 64) */
-  int32_t ntot;    /* HR support side; must equal nfft here */
+  int32_t ntot;    /* HR grid side: nfft, or (FFT method) a power of two in (nfft, 64]
+                      onto which the |E|^2 image is placed through d_hrmap */
   int32_t npix;    /* detector pixels per sub-aperture side */
   int32_t nrebin;  /* HR pixels per detector pixel side */
   int32_t nxsub;   /* sub-apertures across the pupil; binimg side = npix*nxsub */
@@ -69,6 +70,13 @@ typedef struct shwfs_sh_geom {
                                   shwfs_binmap_block_origin (kernels compute the
                                   indices instead of reading the table) */
   int32_t bin_origin;          /* origin returned by shwfs_binmap_block_origin */
+  /* enlarged HR grid and kernel convolution (SutraWfsSH::comp_generic,
+   * libsutra/src.cpp/sutra_wfs_sh.cpp:409-481, 514-525); FFT method only */
+  const int32_t* d_hrmap;      /* [nfft^2] ntot-grid flat index of HR element u + v*nfft
+                                  (indexfill); read when ntot != nfft */
+  const float* d_ftkernel;     /* [2 * ntot^2] complex (re, im) FFT of the convolution
+                                  kernel / ntot^2; read when kernconv */
+  int32_t kernconv;            /* 1: HR image convolved with the kernel before binning */
 } shwfs_sh_geom;
 
 /* A centroider applied to a detector image (or fused after the spot stage).

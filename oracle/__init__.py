@@ -33,6 +33,12 @@ PINNING = {
                             "binmap injection/fftshift invariants, brute-force recount",
     "wfs.comp_image": "pinned: brute-force DFT, Parseval, flux normalisation, Fourier-shift "
                       "theorem slopes, flat-wavefront centring (tests/test_oracle_wfs.py)",
+    "wfs.extend_grid / geometry.hrmap_table": "pinned: frequency-preserving zero padding "
+                                              "(fftshift, centred pad, ifftshift), flux kept "
+                                              "(tests/test_oracle_wfs_ext.py)",
+    "wfs.convolve_kernel": "pinned: Dirac kernel is the identity, Gaussian kernel equals the "
+                           "circular convolution summed from its definition, flux kept "
+                           "(tests/test_oracle_wfs_ext.py)",
     "centroid.cog/wcog/tcog": "pinned: scipy.ndimage.center_of_mass (as the reference tests do), "
                               "threshold/epsilon special cases (tests/test_oracle_centroid.py)",
     "philox.philox4x32": "pinned: Random123 known-answer vectors (tests/test_oracle_philox.py)",

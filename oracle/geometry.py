@@ -123,6 +123,22 @@ def binmap_table(Ntot, npix, nrebin, shift=True):
     return out
 
 
+def hrmap_table(Nfft, Ntot):
+    """hrmap[k]: the Ntot-grid flat index (u + v*Ntot) receiving element k
+    (u + v*Nfft) of an Nfft x Nfft HR image, geom_init.py:847-866: the
+    Nfft x Nfft block at (Ntot - Nfft)/2 of an Ntot grid, rolled by Ntot//2 on
+    both axes; its cells in increasing flat index, the k-th receiving element
+    k (indexfill_krnl, libsutra/src.cu/sutra_wfs.cu:462-479)."""
+    indi = int((Ntot - Nfft) / 2.0)
+    cells = []
+    for r in range(Ntot):
+        for c in range(Ntot):
+            rr, cc = (r - Ntot // 2) % Ntot, (c - Ntot // 2) % Ntot  # before the roll
+            if indi <= rr < indi + Nfft and indi <= cc < indi + Nfft:
+                cells.append(c + r * Ntot)
+    return np.array(sorted(cells), dtype=np.int32).reshape(-1, 1)
+
+
 def halfxy_table(pdiam, Nfft, npix, nrebin):
     """Half-pixel tip-tilt phase, geom_init.py:833-845: pi*(a+b)/Nfft (zero if
     npix and nrebin are both odd)."""

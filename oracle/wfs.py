@@ -1,7 +1,8 @@
 """Oracle for SutraWfsSH::comp_image in float64 (test infrastructure only).
 
-Restates libsutra/src.cpp/sutra_wfs_sh.cpp:337-643 for the NGS, Ntot == Nfft,
-no-field-stop, no-kernel-convolution case, kernel by kernel:
+Restates libsutra/src.cpp/sutra_wfs_sh.cpp:337-643 for the NGS, no-field-stop
+case, kernel by kernel (the enlarged grid Ntot != Nfft and the kernel
+convolution kernconv included):
 
 * camplipup_krnl   libsutra/src.cu/sutra_wfs.cu:27-53
       E[x + y*Nfft + s*Nfft^2] = mask[p] * exp(i*(phase[p]*scale - offset[x + y*pdiam]))
@@ -9,6 +10,11 @@ no-field-stop, no-kernel-convolution case, kernel by kernel:
 * CarmaFFT(dir=1)  libcarma/src.cpp/carma_fft.cpp:164-180 -> forward, unnormalised,
       exp(-2*pi*i*k*n/N)  (numpy.fft.fft2 convention).
 * abs2c_krnl       libsutra/src.cu/sutra_utils.cu:144-168
+* indexfill_krnl   libsutra/src.cu/sutra_wfs.cu:462-479 (Ntot != Nfft)
+      hr_tot[hrmap[k] + s*Ntot^2] = hr[k + s*Nfft^2] on a zero Ntot^2 grid
+* kernconv         libsutra/src.cpp/sutra_wfs_sh.cpp:453-464, 514-525:
+      forward FFT, conv_krnl (sutra_wfs.cu:491-513: product with ftkernel),
+      inverse FFT (CarmaFFT dir -1: unnormalised), real part read by binning
 * fillbincube_krnl libsutra/src.cu/sutra_wfs.cu:369-414
       bincube[p + s*npix^2] = sum_i hr[binmap[i + p*nrebin^2] + s*Nfft^2]
 * reduce2          libsutra/src.cu/sutra_wfs.cu:556-590  (per-subap flux)
@@ -44,6 +50,27 @@ def hr_images(E):
     return (F.real ** 2 + F.imag ** 2)
 
 
+def extend_grid(hr, hrmap, Ntot):
+    """indexfill: each Nfft x Nfft image onto a zero Ntot x Ntot grid, element
+    k = u + v*Nfft (F-order flat) at flat index hrmap[k]; returns [s, u, v]."""
+    nvalid, N, _ = hr.shape
+    flat = hr.transpose(0, 2, 1).reshape(nvalid, N * N)
+    out = np.zeros((nvalid, Ntot * Ntot))
+    out[:, np.asarray(hrmap).ravel()] = flat
+    return out.reshape(nvalid, Ntot, Ntot).transpose(0, 2, 1)
+
+
+def convolve_kernel(hr, ftkernel):
+    """kernconv: real part of the unnormalised inverse FFT of FFT(hr) * ftkernel;
+    ftkernel [u, v] (F-order view of the device array) already holds the
+    1/N^2 of the inverse transform."""
+    N = hr.shape[1]
+    K = np.asarray(ftkernel, dtype=np.complex128).reshape(N, N, order="F") \
+        if np.ndim(ftkernel) == 1 else np.asarray(ftkernel, dtype=np.complex128)
+    F = np.fft.fft2(hr, axes=(1, 2)) * K[None]
+    return (np.fft.ifft2(F, axes=(1, 2)) * (N * N)).real
+
+
 def bin_images(hr, binmap, npix):
     """Gather-sum of the HR pixels listed by binmap, returns [px, py, s]."""
     nvalid, N, _ = hr.shape
@@ -75,14 +102,20 @@ def binimg_from_cube(bincube, validsubsx, validsubsy, nsub, base=None):
 
 
 def comp_image(phase, pupil, scale, offsets, istart, jstart, validsubsx, validsubsy,
-               binmap, fluxPerSub, nphot, pdiam, Nfft, npix, nxsub):
+               binmap, fluxPerSub, nphot, pdiam, Nfft, npix, nxsub, Ntot=None, hrmap=None,
+               ftkernel=None):
     """Noise-free SH image: returns (bincube [px,py,s], intensities_raw, binimg [x,y]).
 
     ``intensities_raw`` is what the reference leaves in d_intensities: the
-    per-subap sum *before* normalisation.
+    per-subap sum *before* normalisation.  Ntot / hrmap: the enlarged grid
+    (binmap then indexes it); ftkernel: the kernel convolution (kernconv).
     """
     E = fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft)
     hr = hr_images(E)
+    if Ntot is not None and Ntot != Nfft:
+        hr = extend_grid(hr, hrmap, Ntot)
+    if ftkernel is not None:
+        hr = convolve_kernel(hr, ftkernel)
     b = bin_images(hr, binmap, npix)
     bn, inten = normalise(b, fluxPerSub, nphot)
     img = binimg_from_cube(bn, validsubsx, validsubsy, npix * nxsub)

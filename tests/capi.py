@@ -24,7 +24,8 @@ class ShGeom(C.Structure):
                 ("npix", i32), ("nrebin", i32), ("nxsub", i32),
                 ("d_istart", vp), ("d_jstart", vp), ("d_validsubsx", vp), ("d_validsubsy", vp),
                 ("d_offsets", vp), ("d_fluxPerSub", vp), ("d_binmap", vp),
-                ("binmap_block", i32), ("bin_origin", i32)]
+                ("binmap_block", i32), ("bin_origin", i32), ("d_hrmap", vp),
+                ("d_ftkernel", vp), ("kernconv", i32)]
 
 
 class CentroidArgs(C.Structure):

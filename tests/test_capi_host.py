@@ -30,7 +30,10 @@ def test_version_and_error_reporting(lib):
     assert "nfft" in capi.last_error(lib)
     g.nfft = g.ntot = 16
     assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_STATE   # tables not loaded
-    g.ntot = 32
+    g.ntot = 32   # enlarged HR grid: needs its hrmap
+    assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_STATE
+    assert "hrmap" in capi.last_error(lib)
+    g.ntot = 128  # beyond the enlarged-grid kernel
     assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_UNSUPPORTED
     assert lib.shwfs_sh_check_geom(None, 0) == capi.SHWFS_ERR_ARG
 

@@ -87,3 +87,17 @@ def test_sizing_rule_matches_oracle():
         o = og.wfs_size(diam, nxsub, pdiam, npix, pix, 0.5)
         assert (w._Nfft, w._Ntot, w._nrebin) == (o["Nfft"], o["Ntot"], o["nrebin"])
         assert w.pixsize == pytest.approx(o["pixsize"], rel=1e-12)
+
+
+def test_product_hrmap_equals_oracle():
+    """The enlarged-grid map of the host geometry (shwfs/geom.py) equals the
+    oracle's restatement (itself pinned as frequency-preserving zero padding,
+    tests/test_oracle_wfs_ext.py)."""
+    import numpy as np
+    from oracle import geometry as og
+    from shwfs import configs, system
+    c = system.geometry(configs.get("wide"))
+    w = c["p_wfs"]
+    assert (w._Nfft, w._Ntot) == (16, 32)
+    assert np.array_equal(w._hrmap, og.hrmap_table(16, 32))
+    assert w._binmap.max() < 32 * 32
