@@ -31,6 +31,10 @@ struct shwfs_batch {
   // same-kind kernel (method >= 0) instead of the per-kind dispatch kernel
   int uni_method = -1, uni_nfft = 0, uni_pdiam = 0, uni_nr = 0;
   bool uni_full = false;
+  // stage groups: the tasks fall into three stage kinds for which a group
+  // kernel is compiled (group_kernel >= 0, shwfs_batch_groups.hip)
+  int group_kernel = -1;
+  GroupArgs gargs{};
   size_t lds = 0;  // bytes of LDS per wave
   int waves = 1;   // waves per workgroup of the mixed kernel (kBatchWaves when the slots fit)
   uint32_t* d_frame = nullptr;
@@ -122,6 +126,108 @@ int32_t fill_task(shwfs_batch* b, int i, const shwfs_batch_tile& tile, hipStream
   return SHWFS_OK;
 }
 
+// Stage code of a task (stage_code): the FFT with its row split, or the MFMA
+// stage with its window class, full-window flag and register rebin code.
+int task_stage_code(const BatchTask& t, const shwfs_sh_geom& g, int method) {
+  const int n = g.nfft;
+  if (method == SHWFS_METHOD_FFT_FP32) {
+    const int split = (t.kind == BK_FFT16_2 || t.kind == BK_FFT32_2 || t.kind == BK_FFT64_2) ? 2 : 1;
+    return stage_code(-1, n, split, false, 0);
+  }
+  const int q = g.nphase <= 16 ? 1 : 2;
+  int nr = dft_register_rebin(t.P);
+  if (nr == 4 && dft_half_grid(t.P, n)) nr |= kHalfGrid;
+  return stage_code(method, n, q, g.nphase == 16 * q, nr);
+}
+
+// Group mode: the tasks' stage codes are three distinct codes in the order of
+// a compiled group kernel (at most kTaskArgs tasks).  Reorders `tasks` by
+// group (heaviest tiles first inside a group), sets per-tile counts, the
+// group starts and operand tables, and the LDS slot; returns false when no
+// group kernel covers the batch.
+bool setup_groups(shwfs_batch* b, std::vector<BatchTask>& tasks, std::vector<double>& cost,
+                  std::vector<int>& order, size_t* lds, hipStream_t st) {
+  const int n = (int)tasks.size();
+  if (n > kTaskArgs) return false;
+  std::vector<int> code(n);
+  std::vector<int> distinct;
+  for (int i = 0; i < n; ++i) {
+    code[i] = task_stage_code(tasks[i], b->geoms[i], b->tiles[i].method);
+    if (std::find(distinct.begin(), distinct.end(), code[i]) == distinct.end())
+      distinct.push_back(code[i]);
+  }
+  if (distinct.size() != 3) return false;
+  std::sort(distinct.begin(), distinct.end());
+  int index = -1;
+  do {
+    index = group_kernel_index(distinct.data());
+  } while (index < 0 && std::next_permutation(distinct.begin(), distinct.end()));
+  if (index < 0) return false;
+  const int* gc = group_kernel_codes(index);
+  std::vector<int> group(n);
+  size_t wlds = 0;
+  for (int i = 0; i < n; ++i) {
+    group[i] = (int)(std::find(gc, gc + 3, code[i]) - gc);
+    const shwfs_sh_geom& g = b->geoms[i];
+    BatchTask& t = tasks[i];
+    size_t l = 0;
+    if (b->tiles[i].method == SHWFS_METHOD_FFT_FP32) {
+      t.per_tile = 64 / g.nfft;
+      const int split = (t.kind == BK_FFT16_2 || t.kind == BK_FFT32_2 || t.kind == BK_FFT64_2) ? 2 : 1;
+      int f = 0;
+      switch (g.nfft * 4 + split) {
+        case 16 * 4 + 1: f = FftLayout<16, 1>::slot_floats(g.nphase, g.npix); break;
+        case 16 * 4 + 2: f = FftLayout<16, 2>::slot_floats(g.nphase, g.npix); break;
+        case 32 * 4 + 1: f = FftLayout<32, 1>::slot_floats(g.nphase, g.npix); break;
+        case 32 * 4 + 2: f = FftLayout<32, 2>::slot_floats(g.nphase, g.npix); break;
+        case 64 * 4 + 1: f = FftLayout<64, 1>::slot_floats(g.nphase, g.npix); break;
+        default: f = FftLayout<64, 2>::slot_floats(g.nphase, g.npix); break;
+      }
+      l = sizeof(float) * (size_t)t.per_tile * f;
+    } else {
+      t.per_tile = dft_stage_subaps(g.nfft);
+      const bool gather = t.nr == 0;
+      const int q = g.nphase <= 16 ? 1 : 2;
+      int f = 0;
+      switch (g.nfft * 4 + q) {
+        case 16 * 4 + 1: f = subaps_floats<16, 1>(g.npix, gather, kDftTaskSubaps); break;
+        case 32 * 4 + 1: f = subaps_floats<32, 1>(g.npix, gather, kDftTaskSubaps); break;
+        case 32 * 4 + 2: f = subaps_floats<32, 2>(g.npix, gather, kDftTaskSubaps); break;
+        // one sub-aperture per wave (dft_subap): the single-frame kernel's slot
+        case 64 * 4 + 1: f = DftLayout<64, 1>::wave_floats(g.npix, gather); break;
+        default: f = DftLayout<64, 2>::wave_floats(g.npix, gather); break;
+      }
+      l = sizeof(float) * (size_t)f;
+    }
+    t.ntiles = (g.nvalid + t.per_tile - 1) / t.per_tile;
+    cost[i] = tile_cost(t, g.nfft);
+    wlds = std::max(wlds, l);
+  }
+  if (wlds * kDftWaves > max_lds_per_block()) return false;
+  // group order, heaviest tiles first inside each group
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    return group[a] != group[c] ? group[a] < group[c] : cost[a] > cost[c];
+  });
+  GroupArgs ga{};
+  int first = 0;
+  for (int gi = 0; gi < 3; ++gi) {
+    ga.first[gi] = first;
+    ga.wtab[gi] = nullptr;
+    for (int k = 0; k < n; ++k) {
+      const int i = order[k];
+      if (group[i] != gi) continue;
+      first += tasks[i].ntiles;
+      if (b->tiles[i].method != SHWFS_METHOD_FFT_FP32) ga.wtab[gi] = tasks[i].wtab;
+    }
+  }
+  (void)st;
+  b->group_kernel = index;
+  b->gargs = ga;
+  *lds = wlds;
+  return true;
+}
+
 int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
   const int n = (int)b->geoms.size();
   // the table may still be read by frames in flight on any stream
@@ -171,15 +277,24 @@ int32_t build_tasks(shwfs_batch* b, hipStream_t st) {
       }
     }
   }
+  std::vector<int> order(n);
+  b->group_kernel = -1;
+  bool grouped = false;
+  if (!uni) {
+    size_t glds = 0;
+    grouped = setup_groups(b, tasks, cost, order, &glds, st);
+    if (grouped) lds = glds;
+  }
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "batch: LDS footprint exceeds the per-workgroup limit");
   // waves per workgroup of the mixed kernel: kBatchWaves slots if they fit
   int waves = kBatchWaves;
   while (waves > 1 && lds * waves > max_lds_per_block()) waves /= 2;
-  // heaviest tiles first (they start first and the light ones fill in)
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  if (!grouped) {
+    // heaviest tiles first (they start first and the light ones fill in)
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  }
   std::vector<BatchTask> sorted(n);
   int first = 0;
   for (int k = 0; k < n; ++k) {
@@ -245,6 +360,9 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
                             b->noisy, b->d_tasks, b->starts, b->args, b->ntasks, b->total_tiles, grid,
                             b->lds,
                             b->d_frame, frame_off, b->remap, st);
+  if (b->group_kernel >= 0)
+    return launch_batch_groups(b->group_kernel, b->noisy, b->starts, b->args, b->ntasks,
+                               b->total_tiles, b->gargs, b->lds, b->d_frame, frame_off, st);
   if (b->noisy)
     return launch_batch_noisy(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
                               b->waves, b->lds, b->d_frame, frame_off, b->remap, st);
@@ -464,28 +582,44 @@ int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps,
   float best = 0.f;
   int32_t rc = time_config(b, reps, st, &best);
   if (rc) return rc;
-  // every WFS on one method (where its geometry allows): one kind set, so the
-  // batch kernel carries the registers of that set only; changing one WFS at
-  // a time from a uniform start mixes the sets and would not reach these
+  // one method per geometry class (WFS of equal nfft, nphase, npix, nrebin
+  // move together), every combination over at most three classes, else every
+  // WFS on one method: these reach the same-kind and the stage-group kernels,
+  // which changing one WFS at a time from a uniform start would not
   {
+    std::vector<int> cls(n, -1);
+    int ncls = 0;
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < i && cls[i] < 0; ++j) {
+        const shwfs_sh_geom &a = b->geoms[i], &c = b->geoms[j];
+        if (a.nfft == c.nfft && a.nphase == c.nphase && a.npix == c.npix && a.nrebin == c.nrebin)
+          cls[i] = cls[j];
+      }
+      if (cls[i] < 0) cls[i] = ncls++;
+    }
     const std::vector<shwfs_batch_tile> start = b->tiles;
     std::vector<shwfs_batch_tile> keep = start;
-    for (int m = 0; m < 3; ++m) {
-      if (!((method_mask >> m) & 1u)) continue;
+    const int per_class = ncls <= 3;
+    int combos = 1;
+    for (int c = 0; c < (per_class ? ncls : 1); ++c) combos *= 3;
+    for (int code = 0; code < combos; ++code) {
       b->tiles = start;
-      bool changed = false;
-      for (int i = 0; i < n; ++i) {
+      bool ok = true, changed = false;
+      for (int i = 0; i < n && ok; ++i) {
+        int m = code;
+        for (int c = 0; c < (per_class ? cls[i] : 0); ++c) m /= 3;
+        m %= 3;
         const shwfs_sh_geom& g = b->geoms[i];
         int k;
-        const bool ok = check_geom(&g, m) == SHWFS_OK &&
-                        (m == SHWFS_METHOD_FFT_FP32 || dft_kind(g.nfft, g.nphase, m, &k));
+        ok = ((method_mask >> m) & 1u) && check_geom(&g, m) == SHWFS_OK &&
+             (m == SHWFS_METHOD_FFT_FP32 || dft_kind(g.nfft, g.nphase, m, &k));
         if (ok && b->tiles[i].method != m) {
           b->tiles[i] = shwfs_batch_tile{m, 0};
           changed = true;
         }
       }
       float us = 0.f;
-      if (changed && time_config(b, reps, st, &us) == SHWFS_OK && us < best) {
+      if (ok && changed && time_config(b, reps, st, &us) == SHWFS_OK && us < best) {
         best = us;
         keep = b->tiles;
       }

@@ -160,6 +160,30 @@ void sh_batch_kernel(const BatchTask* __restrict__ tasks, const TaskStarts start
   }
 }
 
+// ---- stage-group kernels (shwfs_batch_groups.hip) ---------------------------
+// Code of a stage: the FFT (prec < 0, qr = row split) or an MFMA stage (prec
+// = method, qr = padded-window class, full window, register rebin code).
+__host__ __device__ constexpr int stage_code(int prec, int n, int qr, bool full, int nr) {
+  return ((((prec + 1) * 128 + n) * 4 + qr) * 2 + (full ? 1 : 0)) * 32 + nr;
+}
+// Sub-apertures per wave of an MFMA stage in a group kernel: one for the
+// 64-point transform, kDftTaskSubaps otherwise.
+__host__ __device__ constexpr int dft_stage_subaps(int n) { return n == 64 ? 1 : kDftTaskSubaps; }
+// Groups of one launch: group g covers the tiles [first[g], first[g + 1])
+// (first[3] implied by the tile count), its MFMA operand table wtab[g].
+struct GroupArgs {
+  int first[3];
+  const void* wtab[3];
+};
+// Index of the group kernel compiled for the three stage codes (in group
+// order), or -1; the codes of a kernel by index.
+int group_kernel_index(const int codes[3]);
+const int* group_kernel_codes(int index);
+// lds: bytes of one wave's slot; one wave per tile, kDftWaves per workgroup.
+int32_t launch_batch_groups(int index, bool noisy, const TaskStarts& starts, const TaskArgs& args,
+                            int ntasks, int total, const GroupArgs& ga, size_t lds,
+                            const uint32_t* frame_base, uint32_t frame_off, hipStream_t st);
+
 // Launchers of the instantiations (noise-free / noisy in translation units of
 // their own); `set` is a BatchKindSet.
 // grid: workgroups of `waves` waves; lds: bytes of one wave's slot.
