@@ -253,6 +253,75 @@ def measure_uneven_fixed(method, graphs=4, frames_per_graph=50):
                 nvalid=[int(s.wfs.nvalid) for s in bs.systems])
 
 
+def _spot_slopes_np(sp, kind, thr=0.0, weights=None):
+    """CoG-family slopes (pixels from the spot centre) of spots [n, 8, 8]
+    (indexed [s, y, x]) on the host: kind 'cog', 'tcog' (pixels above thr
+    keep v - thr, the others 1e-6, as the reference) or 'wcog' (weights
+    [8, 8])."""
+    sp = sp.astype(np.float64)
+    if kind == "tcog":
+        sp = np.where(sp > thr, sp - thr, 1e-6)
+    elif kind == "wcog":
+        sp = sp * weights[None]
+    I = sp.sum((1, 2))
+    xs = np.arange(8.0)
+    sx = (sp * xs[None, None, :]).sum((1, 2)) / (I + 1e-6) - 3.5
+    sy = (sp * xs[None, :, None]).sum((1, 2)) / (I + 1e-6) - 3.5
+    return np.stack([sx, sy], 1)
+
+
+def measure_learned(graph_frames=20, reps=10):
+    """The learned centroider at 50 photons per sub-aperture + 3 e- readout
+    (40x40 SH, 8x8-pixel spots): training on 2.0 M spot / true-gradient pairs
+    made by the WFS itself (1600 frames of shifted, r0-rescaled screen
+    windows), the held-out slope RMS error against the true gradient for the
+    network and the CoG family on the same spots, and the frame rate of the
+    fused spots + network launch (noisy frames, hipGraph)."""
+    import sutra
+    from shwfs import configs, learned, system
+    s = system.ShSystem(configs.get("lowflux"))
+    lc = learned.LearnedCentroider(s, seed=3)
+    lc.mlp.init_params(11)
+    lc.mlp.set_input_scale(1.0 / 50.0)
+    r0b = 0.16
+    t0 = time.perf_counter()
+    X, T, n = lc.make_data(lc.base_screen(2048, r0b, seed=101), r0b, nframes=1600, seed=7)
+    sync()
+    gen_s = time.perf_counter() - t0
+    lo, hi = lc.set_flux_range_from(X, n)
+    t0 = time.perf_counter()
+    loss = lc.train(X, T, n, epochs=4, batch=4096, lr=2e-3, seed=5)
+    sync()
+    train_s = time.perf_counter() - t0
+    del X, T
+    Xe, Te, ne = lc.make_data(lc.base_screen(2048, r0b, seed=202), r0b, nframes=100, seed=8)
+    t = np.array(Te).reshape(2, ne, order="F").T
+    sp = np.array(Xe).reshape(64, ne, order="F").T.reshape(ne, 8, 8)
+    pix = lc.pixsize
+    pred = lc.predict(Xe, ne)
+    flux = sp.sum((1, 2))
+    g = np.exp(-((np.arange(8) - 3.5)[:, None] ** 2 + (np.arange(8) - 3.5)[None, :] ** 2)
+               / (2 * (4.0 / 2.355) ** 2))  # Gaussian weights, FWHM 4 px
+    est = dict(mlp=pred, cog=_spot_slopes_np(sp, "cog") * pix,
+               tcog=_spot_slopes_np(sp, "tcog", thr=9.0) * pix,
+               wcog=_spot_slopes_np(sp, "wcog", weights=g) * pix)
+    errs = {}
+    for key, keep in (("all_spots", np.ones(ne, bool)), ("flux_ge_25ph", flux >= 25.0)):
+        errs[key] = {k: round(float(np.sqrt(np.mean((v - t)[keep] ** 2))), 4) for k, v in est.items()}
+    # fused spots + network, noisy frames
+    slopes = s.rtc.d_control[0].d_centroids  # a 2 * nvalid slope vector
+    ms = sutra.run_frames_graph(s.wfs, lc.mlp, slopes, graph_frames, reps, True)
+    frames = graph_frames * reps
+    return dict(workload="40x40 SH (1256 sub-apertures), pdiam 16, Nfft 32, 8x8 px, 50 ph/subap "
+                         "Poisson + 3 e- readout; MLP 64-128-128-2 GELU, bf16 MFMA",
+                pairs=int(n), generate_s=round(gen_s, 2), train_s=round(train_s, 2),
+                final_epoch_loss_px2=round(float(loss), 4), flux_range=[round(lo, 2), round(hi, 2)],
+                heldout_pairs=int(ne), slope_rms_err_arcsec=errs,
+                tcog_threshold_e=9.0, wcog_weights="Gaussian FWHM 4 px",
+                fused_frame_us=round(1e3 * ms / frames, 3),
+                fused_frames_per_s=round(frames / (ms * 1e-3), 1))
+
+
 def oracle_frame(s, phase, thr):
     from oracle import centroid as oc
     from oracle import wfs as ow
@@ -321,6 +390,9 @@ def main():
                     help="comma-separated frames of the 80x80 WFS per launch in the batched "
                          "form (0 = skip)")
     ap.add_argument("--no-uneven", action="store_true", help="skip the eight-WFS batch")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned centroider")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="run only the learned-centroider leg (for kernel profiles)")
     ap.add_argument("--uneven-only", action="store_true",
                     help="run only a short eight-WFS batch (for kernel profiles)")
     ap.add_argument("--uneven-fixed", type=str, default="",
@@ -353,6 +425,9 @@ def main():
         sync()
         del src, dst
 
+    if a.learned_only:
+        print(json.dumps(dict(learned_centroider=measure_learned())), flush=True)
+        return
     if a.uneven_only and a.uneven_fixed:
         out = [measure_uneven_fixed(int(m)) for m in a.uneven_fixed.split(",") if m.strip()]
         print(json.dumps(dict(uneven_fixed=out)), flush=True)
@@ -441,6 +516,11 @@ def main():
             print("[bench] uneven 8-WFS batch (%s): %.0f frames/s (%.2f us/frame), methods %s"
                   % (key, r["frames_per_s"], r["us_per_frame"], r["methods"]),
                   file=sys.stderr, flush=True)
+    learned_cen = None
+    if not a.no_learned:
+        learned_cen = measure_learned()
+        print("[bench] learned centroider: %s" % json.dumps(learned_cen["slope_rms_err_arcsec"]),
+              file=sys.stderr, flush=True)
     mfma = mfma_utilisation(w, results, batched)
     cpu = None
     if not a.no_cpu:
@@ -500,6 +580,7 @@ def main():
                                    "one launch per frame, autotuned tiles, 1000 hipGraph frames; "
                                    "fp32_accurate: methods FFT/BF16x3, any_method: FP16 too",
                           **uneven) if uneven else None,
+        learned_centroider=learned_cen,
     )
     print(json.dumps(line), flush=True)
 

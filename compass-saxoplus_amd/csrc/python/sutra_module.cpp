@@ -437,6 +437,43 @@ static void declare_centroiders(py::module& m, const std::string& sfx) {
       .def("set_nmax", &Bpcog::set_nmax, py::arg("nmax"),
            "number of brightest pixels kept per sub-aperture");
 
+  using Corr = SutraCentroiderCorr<Tin, float>;
+  py::class_<Corr, Centro>(m, ("CentroiderCORR" + sfx).c_str())
+      .def_property_readonly("npix", [](Corr& c) { return c.npix; })
+      .def_property_readonly("interp_sizex", [](Corr& c) { return c.interp_sizex; })
+      .def_property_readonly("interp_sizey", [](Corr& c) { return c.interp_sizey; })
+      .def_property_readonly("d_corrfnct", [](Corr& c) { return c.d_corrfnct; },
+                             py::return_value_policy::reference_internal,
+                             "reference function(s) in pixels [npix, npix, nvalid]")
+      .def_property_readonly("d_corrspot", [](Corr& c) { return c.d_corrspot; },
+                             py::return_value_policy::reference_internal,
+                             "None: the correlation is summed without spot transforms")
+      .def_property_readonly("d_corrnorm", [](Corr& c) { return c.d_corrnorm; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_corrmax", [](Corr& c) { return c.d_corrmax; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_corr", [](Corr& c) { return c.d_corr; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_interpmat", [](Corr& c) { return c.d_interpmat; },
+                             py::return_value_policy::reference_internal)
+      .def("init_corr",
+           [](Corr& c, int32_t isizex, int32_t isizey, ArrayFStyle<float>& interpmat) {
+             if (interpmat.size() != (py::ssize_t)isizex * isizey * 6)
+               throw std::runtime_error("init_corr: interpmat must hold isizex * isizey * 6 values");
+             return c.init_corr(isizex, isizey, interpmat.mutable_data());
+           },
+           py::arg("isizex"), py::arg("isizey"), py::arg("interpmat"))
+      .def("load_corr",
+           [](Corr& c, ArrayFStyle<float>& corr, ArrayFStyle<float>& corr_norm, int32_t ndim) {
+             const py::ssize_t need = (py::ssize_t)c.npix * c.npix * (ndim == 3 ? c.nvalid : 1);
+             const py::ssize_t nn = (py::ssize_t)(2 * c.npix - 1) * (2 * c.npix - 1);
+             if (corr.size() != need) throw std::runtime_error("load_corr: wrong corr size");
+             if (corr_norm.size() < nn)
+               throw std::runtime_error("load_corr: corr_norm needs (2 npix - 1)^2 values");
+             return c.load_corr(corr.mutable_data(), corr_norm.mutable_data(), ndim);
+           },
+           py::arg("corr"), py::arg("corr_norm"), py::arg("ndim"));
+
   if constexpr (std::is_same<Tin, float>::value) {  // the learned centroider takes float images
     using Mlp = SutraCentroiderMlp<float, float>;
     py::class_<Mlp, Centro>(m, "CentroiderMLP_FF")
@@ -492,6 +529,8 @@ static void declare_rtc(py::module& m, const char* name) {
             for (auto* c : r.d_centro) {
               if (auto* b = dynamic_cast<SutraCentroiderBpcog<Tin, float>*>(c))
                 l.append(py::cast(b, py::return_value_policy::reference));
+              else if (auto* k = dynamic_cast<SutraCentroiderCorr<Tin, float>*>(c))
+                l.append(py::cast(k, py::return_value_policy::reference));
               else if (auto* t = dynamic_cast<SutraCentroiderTcog<Tin, float>*>(c))
                 l.append(py::cast(t, py::return_value_policy::reference));
               else if (auto* w = dynamic_cast<SutraCentroiderWcog<Tin, float>*>(c))
@@ -585,15 +624,15 @@ PYBIND11_MODULE(sutra, m) {
         py::arg("noise") = true, py::arg("fused") = true,
         "issues nframes frames back to back; returns hipEvent-timed milliseconds");
   m.def("run_frames_graph",
-        [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, int32_t n, int32_t reps) {
+        [](SutraWfsSH& w, Centro& c, CarmaObj<float>& slopes, int32_t n, int32_t reps, bool noise) {
           if (slopes.get_nb_elements() < 2 * w.nvalid) throw std::runtime_error("run_frames_graph: slope vector too short");
           py::gil_scoped_release nogil;
-          return run_frames_graph(&w, &c, slopes.get_data(), n, reps);
+          return run_frames_graph(&w, &c, slopes.get_data(), n, reps, noise);
         },
         py::arg("wfs"), py::arg("centroider"), py::arg("slopes"), py::arg("nframes"),
-        py::arg("reps"),
-        "noise-free fused frames captured in a hipGraph of nframes launches, replayed reps "
-        "times; returns hipEvent-timed milliseconds of the reps replays");
+        py::arg("reps"), py::arg("noise") = false,
+        "fused frames (noise-free by default) captured in a hipGraph of nframes launches, "
+        "replayed reps times; returns hipEvent-timed milliseconds of the reps replays");
   py::class_<SutraAcquisim>(m, "Acquisim")
       .def(py::init([](SutraSensors& sensors, int32_t wfs_num) {
              return new SutraAcquisim(&sensors, wfs_num);

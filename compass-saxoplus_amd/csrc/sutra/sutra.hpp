@@ -444,6 +444,42 @@ class SutraCentroiderBpcog : public SutraCentroider<Tin, T> {
                                     int32_t size) override;
 };
 
+// Correlation centroider (reference: libsutra/include/sutra_centroider_corr.hpp,
+// src.cpp/sutra_centroider_corr.cpp): per sub-aperture the correlation of the
+// spot with a reference function, its maximum and a parabola fitted around
+// it.  The correlation is summed directly on the lags -(npix-1) .. npix-1
+// (the reference's zero-padded FFT gives the same lags), so d_corrfnct holds
+// the reference function in pixels and there is no spot-transform buffer
+// (d_corrspot stays NULL).  Taken from the detector image after the frame.
+template <class Tin, class T>
+class SutraCentroiderCorr : public SutraCentroider<Tin, T> {
+ public:
+  int32_t interp_sizex, interp_sizey;
+  CarmaObj<T>* d_corrfnct;   // [npix, npix, 1 or nvalid] reference function(s)
+  CarmaObj<T>* d_corrspot;   // NULL (no transform of the spots)
+  CarmaObj<T>* d_corrnorm;   // [2 npix - 1, 2 npix - 1]
+  CarmaObj<int32_t>* d_corrmax;  // [nvalid] flat index of the maximum in each map
+  CarmaObj<T>* d_corr;       // [2 npix - 1, 2 npix - 1, nvalid] normalised maps
+  CarmaObj<T>* d_interpmat;  // [sizex * sizey, 6] parabola fit matrix
+  SutraCentroiderCorr(CarmaContext* context, SutraWfs* wfs, int64_t nvalid, float offset,
+                      float scale, bool filter_TT, int32_t device);
+  ~SutraCentroiderCorr();
+  string get_type() override { return "corr"; }
+  int32_t init_corr(int32_t isizex, int32_t isizey, T* interpmat);
+  // corr: [npix, npix] (ndim 2, every sub-aperture) or [npix, npix, nvalid]
+  // (ndim 3); corr_norm: the first (2 npix - 1)^2 values are used
+  int32_t load_corr(T* corr, T* corr_norm, int32_t ndim);
+  int32_t get_cog(float* img, float* intensities, T* centroids, int32_t nvalid, int32_t npix,
+                  int32_t ntot, hipStream_t stream = nullptr) override;
+  int32_t get_cog(float* intensities, T* slopes, bool noise) override;
+  int32_t get_cog() override;
+  shwfs_centroid_args centroid_args(float* intensities, T* slopes, int32_t nvalid, int32_t npix,
+                                    int32_t size) override;
+
+ private:
+  bool ref_per_subap = false;
+};
+
 template <class Tin, class T>
 class SutraCentroiderTcog : public SutraCentroider<Tin, T> {
  public:
@@ -568,7 +604,8 @@ double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slop
 // returns the elapsed ms of the timed replays between hipEvents on that
 // stream (frames issued by the device back to back, without host launches).
 double run_frames_graph(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
-                        int32_t nframes, int32_t reps);
+                        int32_t nframes, int32_t reps,
+                        bool noise = false);
 
 // Several SH WFS (uneven sizes) computed by one launch per frame, with their
 // fused centroiders; replaces the per-WFS loop over SutraSensors::d_wfs.

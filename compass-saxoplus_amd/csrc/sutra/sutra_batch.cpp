@@ -33,8 +33,9 @@ SutraWfsBatch::SutraWfsBatch(CarmaContext* context, vector<SutraWfsSH*> wfs_,
     SutraCentroider<float, float>* c = centro[i];
     if (c) {
       if (c->wfs != w) throw std::invalid_argument("WfsBatch: centroider not attached to its WFS");
-      if (c->get_type() == "bpcog")
-        throw std::invalid_argument("WfsBatch: bpcog centroiders run after the frame (get_cog)");
+      if (c->get_type() == "bpcog" || c->get_type() == "corr")
+        throw std::invalid_argument("WfsBatch: " + c->get_type() +
+                                    " centroiders run after the frame (get_cog)");
       if (c->get_type() == "wcog") {
         auto* wc = dynamic_cast<SutraCentroiderWcog<float, float>*>(c);
         if (!wc || !wc->d_weights ||

@@ -432,6 +432,141 @@ int32_t SutraCentroiderBpcog<Tin, T>::get_cog() {
   return EXIT_FAILURE;
 }
 
+// ============================================================ Corr
+// reference: libsutra/src.cpp/sutra_centroider_corr.cpp:26-285
+template <class Tin, class T>
+SutraCentroiderCorr<Tin, T>::SutraCentroiderCorr(CarmaContext* context, SutraWfs* wfs,
+                                                 int64_t nvalid, float offset, float scale,
+                                                 bool filter_TT, int32_t device)
+    : SutraCentroider<Tin, T>(context, wfs, nvalid, offset, scale, filter_TT, device),
+      interp_sizex(0),
+      interp_sizey(0),
+      d_corrfnct(nullptr),
+      d_corrspot(nullptr),
+      d_corrnorm(nullptr),
+      d_corrmax(nullptr),
+      d_corr(nullptr),
+      d_interpmat(nullptr) {
+  if (wfs != nullptr) {
+    this->npix = (int32_t)wfs->npix;
+    this->nxsub = (int32_t)wfs->nxsub;
+  }
+}
+
+template <class Tin, class T>
+SutraCentroiderCorr<Tin, T>::~SutraCentroiderCorr() {
+  del(d_corrfnct);
+  del(d_corrnorm);
+  del(d_corrmax);
+  del(d_corr);
+  del(d_interpmat);
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCorr<Tin, T>::init_corr(int32_t isizex, int32_t isizey, T* interpmat) {
+  const int32_t n = this->npix;
+  if (n <= 0) throw std::runtime_error("init_corr: set_npix has not been called");
+  if (isizex <= 0 || isizey <= 0 || isizex > 2 * n - 1 || isizey > 2 * n - 1)
+    throw std::runtime_error("init_corr: interpolation patch larger than the correlation map");
+  this->current_context->set_active_device(this->device, 1);
+  del(d_corrfnct);
+  del(d_corrnorm);
+  del(d_corrmax);
+  del(d_corr);
+  del(d_interpmat);
+  const int64_t N = 2 * n - 1;
+  int64_t d3[4] = {3, n, n, this->nvalid};
+  d_corrfnct = new CarmaObj<T>(this->current_context, d3);
+  int64_t dc[4] = {3, N, N, this->nvalid};
+  d_corr = new CarmaObj<T>(this->current_context, dc);
+  int64_t dn[3] = {2, N, N};
+  d_corrnorm = new CarmaObj<T>(this->current_context, dn);
+  std::vector<T> ones((size_t)(N * N), T(1));
+  d_corrnorm->host2device(ones.data());
+  int64_t dm[2] = {1, this->nvalid};
+  d_corrmax = new CarmaObj<int32_t>(this->current_context, dm);
+  interp_sizex = isizex;
+  interp_sizey = isizey;
+  int64_t di[3] = {2, (int64_t)isizex * isizey, 6};
+  d_interpmat = new CarmaObj<T>(this->current_context, di);
+  d_interpmat->host2device(interpmat);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCorr<Tin, T>::load_corr(T* corr, T* corr_norm, int32_t ndim) {
+  if (d_corrfnct == nullptr) throw std::runtime_error("load_corr: init_corr has not been called");
+  if (ndim != 2 && ndim != 3) throw std::runtime_error("load_corr: ndim must be 2 or 3");
+  this->current_context->set_active_device(this->device, 1);
+  d_corrnorm->host2device(corr_norm);
+  const size_t n2 = (size_t)this->npix * this->npix;
+  ref_per_subap = ndim == 3;
+  carma_safe_call(hipMemcpy(d_corrfnct->get_data(), corr,
+                            sizeof(T) * n2 * (ref_per_subap ? (size_t)this->nvalid : 1),
+                            hipMemcpyHostToDevice));
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+shwfs_centroid_args SutraCentroiderCorr<Tin, T>::centroid_args(float* intensities, T* slopes,
+                                                               int32_t nvalid, int32_t npix,
+                                                               int32_t size) {
+  // the CoG kinds do not apply; the fused paths run this centroider after the frame
+  return base_args(this, SHWFS_CENTROID_NONE, intensities, slopes, nvalid, npix, size);
+}
+
+// The spots are read with the row stride npix * nxsub (the detector image
+// side), as fill_bincube does in the reference; ntot is not used.
+template <class Tin, class T>
+int32_t SutraCentroiderCorr<Tin, T>::get_cog(float* img, float* intensities, T* centroids,
+                                             int32_t nvalid, int32_t npix, int32_t ntot,
+                                             hipStream_t stream) {
+  (void)intensities;
+  (void)ntot;
+  this->current_context->set_active_device(this->device, 1);
+  if (d_corrfnct == nullptr) throw std::runtime_error("get_cog (corr): init_corr has not been called");
+  if (this->d_validx == nullptr)
+    throw std::runtime_error("get_cog: load_validpos has not been called");
+  if (this->nxsub <= 0) throw std::runtime_error("get_cog (corr): nxsub not set for this centroider");
+  if (npix != this->npix || nvalid != this->nvalid)
+    throw std::runtime_error("get_cog (corr): npix / nvalid differ from init_corr's");
+  if (this->d_validx->get_nb_elements() < nvalid)
+    throw std::runtime_error("get_cog: fewer valid positions loaded than sub-apertures");
+  shwfs_throw_if(
+      shwfs_centroids_corr(img, npix * this->nxsub, this->d_validx->get_data(),
+                           this->d_validy->get_data(), nvalid, npix, d_corrfnct->get_data(),
+                           ref_per_subap ? 1 : 0, d_corrnorm->get_data(),
+                           d_interpmat->get_data(), interp_sizex, interp_sizey, this->offset,
+                           this->scale, this->d_centroids_ref->get_data(),
+                           (int32_t)this->slope_order, centroids, d_corr->get_data(),
+                           d_corrmax->get_data(), stream),
+      "get_cog (corr)");
+  if (this->filter_TT) this->apply_TT_filter(centroids);
+  return EXIT_SUCCESS;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCorr<Tin, T>::get_cog(float* intensities, T* slopes, bool noise) {
+  if (this->wfs != nullptr) {
+    CarmaObj<float>* img = (noise || !this->wfs->roket) ? this->wfs->d_binimg
+                                                        : this->wfs->d_binimg_notnoisy;
+    if (img->get_dims(1) != (int64_t)this->npix * this->nxsub)
+      throw std::runtime_error("get_cog (corr): detector image side is not npix * nxsub");
+    return get_cog(img->get_data(), intensities, slopes, (int32_t)this->wfs->nvalid,
+                   (int32_t)this->wfs->npix, (int32_t)img->get_dims(1), this->wfs->stream);
+  }
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
+template <class Tin, class T>
+int32_t SutraCentroiderCorr<Tin, T>::get_cog() {
+  if (this->wfs != nullptr)
+    return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
+  std::cerr << "this->wfs was not initialized" << std::endl;
+  return EXIT_FAILURE;
+}
+
 // ============================================================ Tcog
 // reference: libsutra/src.cpp/sutra_centroider_tcog.cpp:26-108
 template <class Tin, class T>
@@ -632,11 +767,11 @@ double run_frames(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slop
 }
 
 double run_frames_graph(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* slopes,
-                        int32_t nframes, int32_t reps) {
+                        int32_t nframes, int32_t reps, bool noise) {
   if (nframes <= 0 || reps <= 0) throw std::runtime_error("run_frames_graph: counts must be positive");
   wfs->current_context->set_active_device(wfs->device, 1);
   // one frame outside the capture: first-use work (device tables) happens here
-  comp_frame(wfs, c, c->d_intensities->get_data(), slopes, false);
+  comp_frame(wfs, c, c->d_intensities->get_data(), slopes, noise);
   carma_safe_call(hipDeviceSynchronize());
   hipStream_t st = nullptr;
   carma_safe_call(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -647,7 +782,9 @@ double run_frames_graph(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float
   double ms_total = -1.0;
   try {
     carma_safe_call(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int32_t k = 0; k < nframes; ++k) comp_frame(wfs, c, c->d_intensities->get_data(), slopes, false);
+    // with noise, each captured frame keeps the Philox call counter it was
+    // captured with: replays repeat the same draws (the work is the same)
+    for (int32_t k = 0; k < nframes; ++k) comp_frame(wfs, c, c->d_intensities->get_data(), slopes, noise);
     carma_safe_call(hipStreamEndCapture(st, &g));
     carma_safe_call(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
     carma_safe_call(hipGraphLaunch(ex, st));
@@ -694,6 +831,8 @@ template class SutraCentroiderTcog<uint16_t, float>;
 template class SutraCentroiderWcog<uint16_t, float>;
 template class SutraCentroiderBpcog<float, float>;
 template class SutraCentroiderBpcog<uint16_t, float>;
+template class SutraCentroiderCorr<float, float>;
+template class SutraCentroiderCorr<uint16_t, float>;
 
 // ============================================================ controller / rtc
 template <class T, class Tout>
@@ -746,6 +885,9 @@ int32_t SutraRtc<Tin, T, Tout>::add_centroider(CarmaContext* context, int64_t nv
   if (typec == "bpcog")  // ten brightest pixels, as the reference's factory
     d_centro.push_back(new SutraCentroiderBpcog<Tin, T>(context, wfs, nvalid, offset, scale,
                                                         filter_TT, (int32_t)device, 10));
+  else if (typec == "corr")
+    d_centro.push_back(new SutraCentroiderCorr<Tin, T>(context, wfs, nvalid, offset, scale,
+                                                       filter_TT, (int32_t)device));
   else if (typec == "cog")
     d_centro.push_back(new SutraCentroiderCog<Tin, T>(context, wfs, nvalid, offset, scale,
                                                       filter_TT, (int32_t)device));

@@ -80,9 +80,13 @@ class ParamWfs:
 
 
 class ParamCentroider:
-    def __init__(self, type="cog", thresh=0.0, width=None, type_fct="gauss", nwfs=0):
+    def __init__(self, type="cog", thresh=0.0, width=None, type_fct="gauss", nwfs=0, nmax=10):
         self.type = type
         self.thresh = float(thresh)
+        self.nmax = int(nmax)
+        self.sizex = 3
+        self.sizey = 3
+        self.interpmat = None
         self.width = width
         self.type_fct = type_fct
         self.nwfs = nwfs

@@ -217,6 +217,33 @@ int32_t shwfs_fill_weights(float* d_out, const float* d_in, int32_t npix, int32_
 int32_t shwfs_centroids_bpcog(float* d_img, int32_t nbpix, const shwfs_centroid_args* a,
                               void* stream);
 
+/* Correlation centroider over the image d_img (row stride `size`, the image
+ * side npix * nxsub): per sub-aperture s the spot b[x, y] = d_img[(validx[s] +
+ * x) + (validy[s] + y) * size] is correlated with the reference function r
+ * (d_ref_fn [npix^2], x fastest; with ref_per_subap [npix^2 * nvalid]):
+ * C[kx, ky] = sum_m b[m + k] r[m] for lags -(npix-1) .. npix-1, divided by
+ * d_corrnorm [(2 npix - 1)^2] (NULL: 1).  The maximum over the central nbmax^2
+ * window (nbmax = npix if 2 npix - 1 > 20, else 2 npix - 1; the lowest flat
+ * index on ties) centres a sizex x sizey patch (moved inside the map when the
+ * maximum sits on its border); d_interpmat [sizex*sizey, 6] (column-major:
+ * element (t, c) at t + c*sizex*sizey, patch sample t = col + row*sizex) fits
+ * a x^2 + b y^2 + c xy + d x + e y + f, and the vertex plus the patch position
+ * gives ((v + m - 0.5 - npix/2) - offset) * scale, minus d_ref (may be NULL;
+ * in the order of the slopes: slope_order 0 x then y, 1 interlaced).
+ * d_corr [(2 npix - 1)^2 * nvalid] and d_corrmax [nvalid] (flat index of the
+ * maximum in its map) may be NULL.  Replaces SutraCentroiderCorr::get_cog
+ * (libsutra/src.cpp/sutra_centroider_corr.cpp:190-262) and its kernels
+ * fillcorr, correl, roll2real, corr_norm, sortmaxi and interp_parab
+ * (libsutra/src.cu/sutra_centroider_corr.cu).  npix <= 16; 6 <= sizex*sizey
+ * <= 49, sizex and sizey <= 2 npix - 1. */
+int32_t shwfs_centroids_corr(const float* d_img, int32_t size, const int32_t* d_validx,
+                             const int32_t* d_validy, int32_t nvalid, int32_t npix,
+                             const float* d_ref_fn, int32_t ref_per_subap,
+                             const float* d_corrnorm, const float* d_interpmat, int32_t sizex,
+                             int32_t sizey, float offset, float scale, const float* d_ref,
+                             int32_t slope_order, float* d_slopes, float* d_corr,
+                             int32_t* d_corrmax, void* stream);
+
 /* ---- counter-based noise on a device array ----------------------------------
  * Replaces CarmaObj<float>::prng (libcarma/src.cpp/carma_obj.cpp:490-527,
  * src.cu/carma_rng.cu:98-141,287-303) with Philox4x32-10 streams:

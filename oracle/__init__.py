@@ -21,6 +21,9 @@ loops for the small cases, the algorithm of the reference path:
 * ``mlp``       - float64 forward/backward of the per-subaperture MLP centroider
 * ``raytrace``  - guide-star raytrace through layers (libsutra/src.cpp/
                   sutra_source.cpp:222-267, libsutra/src.cu/sutra_target.cu:25-77)
+* ``corr``      - correlation centroider (libsutra/src.cpp/
+                  sutra_centroider_corr.cpp:190-262) and the acquisim gather
+                  (libsutra/src.cu/sutra_acquisim.cu:65-87)
 
 Pinning status of every function is listed in ``PINNING`` below; each entry
 names the tests that pin it.  Any function marked "parity unpinned" has no
@@ -50,4 +53,10 @@ PINNING = {
                          "edge, linear layers reproduced at the traced coordinates (rotation, "
                          "magnification, compression, shifts), layers add "
                          "(tests/test_oracle_raytrace.py)",
+    "corr.corr_map": "pinned: equals (2n)^2 times the linear cross-correlation summed from its "
+                     "definition on random data (tests/test_oracle_corr.py)",
+    "corr.interp_mat / corr.centroids": "pinned: the fit recovers the vertex of an exact "
+                                        "quadratic, whole-pixel shifts of a pattern are measured "
+                                        "exactly through the full centroider",
+    "corr.acquisim_bincube": "pinned: known-answer gather of listed windows",
 }

@@ -127,3 +127,24 @@ def test_batch_arguments_rejected_without_launch(lib):
                lib.shwfs_batch_autotune(None, 1, 10, None, None)):
         assert rc == capi.SHWFS_ERR_ARG
     assert "batch" in capi.last_error(lib)
+
+
+def test_corr_arguments_rejected_without_launch(lib):
+    dev = C.c_void_p(16)  # never dereferenced: every call below fails its checks first
+
+    def call(img=dev, size=64, npix=8, nvalid=4, ref_fn=dev, interp=dev, sx=3, sy=3, slopes=dev):
+        return lib.shwfs_centroids_corr(img, size, dev, dev, nvalid, npix, ref_fn, 0, None, interp,
+                                        sx, sy, 3.5, 1.0, None, 0, slopes, None, None, None)
+
+    assert call(img=None) == capi.SHWFS_ERR_ARG
+    assert call(ref_fn=None) == capi.SHWFS_ERR_ARG
+    assert call(interp=None) == capi.SHWFS_ERR_ARG
+    assert call(slopes=None) == capi.SHWFS_ERR_ARG
+    assert call(nvalid=0) == capi.SHWFS_ERR_ARG
+    assert call(size=4) == capi.SHWFS_ERR_ARG  # the image is smaller than one spot
+    assert call(npix=17, size=200) == capi.SHWFS_ERR_UNSUPPORTED
+    # the parabola patch: at least 6 points, at most 49, inside the 2 npix - 1 map
+    assert call(sx=2, sy=2) == capi.SHWFS_ERR_ARG
+    assert call(sx=7, sy=9) == capi.SHWFS_ERR_ARG
+    assert call(npix=2, size=8, sx=5, sy=3) == capi.SHWFS_ERR_ARG
+    assert "centroids_corr" in capi.last_error(lib)
