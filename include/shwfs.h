@@ -353,7 +353,9 @@ typedef struct shwfs_batch_tile {
   int32_t split;  /* FFT stage: 1, or 2 (row transforms split in two, needs
                      2*nphase <= nfft); 0 = 2 when allowed.  Ignored by DFT. */
 } shwfs_batch_tile;
-/* frames[i].method is the initial method of WFS i (wg_size is ignored). */
+/* frames[i].method is the initial method of WFS i (wg_size is ignored).
+ * Refused (SHWFS_ERR_UNSUPPORTED): ntot != nfft, kernconv, layers, and the
+ * SHWFS_CENTROID_MLP kind, which run per WFS through shwfs_sh_comp_image. */
 int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* frames,
                            int32_t nwfs, shwfs_batch** out);
 int32_t shwfs_batch_destroy(shwfs_batch* batch);
