@@ -321,6 +321,8 @@ static void declare_sensors(py::module& m) {
       .def("set_noise", &SutraSensors::set_noise, py::arg("nwfs"), py::arg("noise"), py::arg("seed"))
       .def("set_field_stop",
            [](SutraSensors& s, int32_t nwfs, ArrayFStyle<float>& fs, int32_t N) {
+             if (fs.size() != (py::ssize_t)N * N)
+               throw std::runtime_error("set_field_stop: the mask must hold N * N values");
              return s.set_field_stop(nwfs, fs.mutable_data(), N);
            },
            py::arg("nwfs"), py::arg("field_stop"), py::arg("N"));

@@ -407,6 +407,8 @@ SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
       d_validpuppixy(nullptr),
       d_fsamplipup(nullptr),
       d_fsamplifoc(nullptr),
+      d_fsphase(nullptr),
+      d_fsampl(nullptr),
       fsampli_plan(nullptr),
       noise_seed(1234),
       noise_call(0),
@@ -506,6 +508,8 @@ SutraWfsSH::~SutraWfsSH() {
   del(d_submask);
   del(d_fsamplipup);
   del(d_fsamplifoc);
+  del(d_fsphase);
+  del(d_fsampl);
   del(d_camimg);
   del(d_dark);
   del(d_flat);
@@ -625,6 +629,11 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
   f.d_intensities = d_intensities->get_data();
   f.centroid = nullptr;
   f.d_pupil_unit = d_pupil_unit ? d_pupil_unit->get_data() : nullptr;
+  if (d_submask != nullptr) {  // the stopped field (field_stop_stage) as amplitude and phase
+    f.d_phase = d_fsphase->get_data();
+    f.d_pupil = d_fsampl->get_data();
+    f.d_pupil_unit = nullptr;
+  }
   if (layers_in_field_stage()) {
     f.layers = d_gs->pending.data();
     f.nlayers = (int32_t)d_gs->pending.size();
@@ -634,8 +643,21 @@ shwfs_frame_args SutraWfsSH::frame_args(bool with_noise) const {
 
 bool SutraWfsSH::layers_in_field_stage() const {
   const int32_t wg = get_wg_size();
-  return d_gs != nullptr && !d_gs->pending.empty() && get_method() == SHWFS_METHOD_FFT_FP32 &&
-         (wg == 0 || wg == 64);
+  return d_gs != nullptr && !d_gs->pending.empty() && d_submask == nullptr &&
+         get_method() == SHWFS_METHOD_FFT_FP32 && (wg == 0 || wg == 64);
+}
+
+// reference: the field-stop branch of comp_generic
+// (libsutra/src.cpp/sutra_wfs_sh.cpp:366-391)
+void SutraWfsSH::field_stop_stage(hipStream_t st) {
+  if (d_submask == nullptr) return;
+  d_gs->render_layers(st);
+  shwfs_throw_if(shwfs_field_stop(d_gs->d_phase->d_screen->get_data(), d_pupil->get_data(),
+                                  (int32_t)d_pupil->get_dims(1), d_gs->scale,
+                                  d_submask->get_data(), (int32_t)d_submask->get_dims(1),
+                                  d_fsamplipup->get_data(), d_fsphase->get_data(),
+                                  d_fsampl->get_data(), st),
+                 "field stop");
 }
 
 // One fused launch: fillcamplipup + FFT + abs2c + fillbincube + subap_reduce +
@@ -656,6 +678,7 @@ int32_t SutraWfsSH::comp_generic() {
     throw std::runtime_error("comp_image: phase screen and pupil sizes differ");
   current_context->set_active_device(device, 1);
   if (!layers_in_field_stage()) d_gs->render_layers(stream);
+  field_stop_stage(stream);
   shwfs_sh_geom g = geom();
   shwfs_frame_args f = frame_args(noise_this_call);
   if (roket && d_binimg_notnoisy != nullptr && d_binimg != nullptr) {
@@ -741,11 +764,29 @@ int32_t SutraWfsSH::set_bincube(float* bincube, int32_t nElem) {
   return EXIT_SUCCESS;
 }
 
+// reference: libsutra/src.cpp/sutra_wfs_sh.cpp:674-705.  The transform
+// pair needs no plan (the library's own kernels), so campli_plans is not
+// touched; the stop applies to every spot method.
 int32_t SutraWfsSH::set_field_stop(PlanMap& campli_plans, float* field_stop, int32_t N) {
   (void)campli_plans;
-  (void)field_stop;
-  (void)N;
-  throw std::runtime_error("field stops are not part of this build");
+  const int64_t np = d_pupil->get_dims(1);
+  if (N < 16 || N > 4096 || (N & (N - 1)) != 0)
+    throw std::runtime_error("set_field_stop: N must be a power of two in [16, 4096]");
+  if (N < np) throw std::runtime_error("set_field_stop: N smaller than the pupil support");
+  if (field_stop == nullptr) throw std::runtime_error("set_field_stop: no mask");
+  current_context->set_active_device(device, 1);
+  del(d_submask);
+  del(d_fsamplipup);
+  del(d_fsphase);
+  del(d_fsampl);
+  int64_t d2[3] = {2, N, N};
+  d_submask = new CarmaObj<float>(current_context, d2, field_stop);
+  int64_t dw[3] = {2, N, np};  // rows y < npup of the padded field
+  d_fsamplipup = new CarmaObj<hipFloatComplex>(current_context, dw);
+  int64_t dp[3] = {2, np, np};
+  d_fsphase = new CarmaObj<float>(current_context, dp);
+  d_fsampl = new CarmaObj<float>(current_context, dp);
+  return EXIT_SUCCESS;
 }
 
 int32_t SutraWfsSH::set_wg_size(int32_t wg_size) {

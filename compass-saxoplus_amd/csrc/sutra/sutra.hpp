@@ -214,8 +214,14 @@ class SutraWfsSH : public SutraWfs {
   CarmaObj<int32_t>* d_binmap;
   CarmaObj<int32_t>* d_validpuppixx;
   CarmaObj<int32_t>* d_validpuppixy;
+  // field stop (set_field_stop): d_fsamplipup is the n x npup work array of
+  // the transforms; the spot kernels read the stopped field as the amplitude
+  // d_fsampl and the phase d_fsphase ([npup^2] each).  d_fsamplifoc stays
+  // NULL: the focal-plane field is never stored whole.
   CarmaObj<hipFloatComplex>* d_fsamplipup;
   CarmaObj<hipFloatComplex>* d_fsamplifoc;
+  CarmaObj<float>* d_fsphase;
+  CarmaObj<float>* d_fsampl;
   ShwfsPlan* fsampli_plan;
 
   // Philox state of the spot / image noise: (key, call counter).
@@ -255,6 +261,9 @@ class SutraWfsSH : public SutraWfs {
   // the guide star's pending layers are traced in the field stage of the spot
   // kernel (FFT method, 64-lane workgroups); otherwise they are rendered first
   bool layers_in_field_stage() const;
+  // with a field stop: render the guide star's layers, then the stopped
+  // field into d_fsphase / d_fsampl (read by frame_args)
+  void field_stop_stage(hipStream_t st);
   // Geometry descriptor handed to the C ABI.
   shwfs_sh_geom geom() const;
   // Frame arguments of the next comp_image (advances nothing).

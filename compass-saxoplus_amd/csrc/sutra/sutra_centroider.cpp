@@ -734,6 +734,7 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
   // the fused learned centroider and the traced field stage are separate kernels
   if (a.kind == SHWFS_CENTROID_MLP || !wfs->layers_in_field_stage())
     wfs->d_gs->render_layers(wfs->stream);
+  wfs->field_stop_stage(wfs->stream);
   shwfs_sh_geom g = wfs->geom();
   shwfs_frame_args f = wfs->frame_args(noise);
   f.centroid = &a;

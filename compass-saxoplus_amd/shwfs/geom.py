@@ -280,6 +280,30 @@ def init_sh_geom(p_wfs, p_tel, p_geom, ittime=1.0, r0=None):
 
     p_wfs._nphotons = compute_nphotons(ittime, p_wfs.optthroughput, p_tel.diam, p_wfs.nxsub,
                                        zerop=p_wfs.zerop, gsmag=p_wfs.gsmag)
+    p_wfs._submask = field_stop_mask(p_wfs, p_tel, p_geom)
+
+
+def field_stop_mask(p_wfs, p_tel, p_geom):
+    """Focal-plane stop of the whole pupil's transform, fftshifted
+    (shesha/shesha/init/geom_init.py:973-995): on the fft_goodsize of the
+    padded pupil, pixels of fspixsize = pupdiam lambda / D / fftsize arcsec,
+    a disc (round) or square of side fssize, centred between the middle
+    pixels.  None when fssize is 0."""
+    if p_wfs.fssize == 0:
+        return None
+    fftsize = fft_goodsize(p_geom._mpupil.shape[0])
+    fspixsize = (p_geom.pupdiam * (p_wfs.Lambda * 1.0e-6) / p_tel.diam * RAD2ARCSEC) / fftsize
+    r = int(p_wfs.fssize / fspixsize / 2.0)
+    if p_wfs.fstop == "round":
+        focmask = dist(fftsize, xc=fftsize / 2.0 - 0.5, yc=fftsize / 2.0 - 0.5) < r
+    elif p_wfs.fstop == "square":
+        X = np.indices((fftsize, fftsize)) + 1
+        x = X[1] - (fftsize + 1.0) / 2.0
+        y = X[0] - (fftsize + 1.0) / 2.0
+        focmask = (np.abs(x) <= r) * (np.abs(y) <= r)
+    else:
+        raise ValueError("wfs fstop must be 'round' or 'square'")
+    return np.fft.fftshift(focmask * 1.0).astype(np.float32)
 
 
 def flux_per_valid_subap(p_wfs):

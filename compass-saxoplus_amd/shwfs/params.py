@@ -56,6 +56,9 @@ class ParamWfs:
         self.max_pix_value = 0
         self.is_low_order = False
         self.nPupils = 1
+        self.fssize = 0.0       # field stop size (arcsec), 0: none
+        self.fstop = "round"    # "round" or "square"
+        self._submask = None
         # derived (geom)
         self._pdiam = 0
         self._Nfft = 0

@@ -47,4 +47,6 @@ def wfs_init(context, telescope, p_wfss, p_tel, p_geom, seeds=None, roket=False)
         sensors.d_wfs[i].load_arrays(w._phasemap, w._hrmap, w._binmap, w._halfxy,
                                      geom.flux_per_valid_subap(w), w._validsubsx, w._validsubsy,
                                      w._validpuppixx, w._validpuppixy, w._ttprojmat, w._ftkernel)
+        if w._submask is not None:  # shesha/shesha/init/wfs_init.py:234-235
+            sensors.set_field_stop(i, w._submask, w._submask.shape[0])
     return sensors

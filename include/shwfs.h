@@ -244,6 +244,24 @@ int32_t shwfs_centroids_corr(const float* d_img, int32_t size, const int32_t* d_
                              int32_t slope_order, float* d_slopes, float* d_corr,
                              int32_t* d_corrmax, void* stream);
 
+/* ---- field stop ----------------------------------------------------------------
+ * The pupil field pupil * exp(i scale phase) ([npup^2] each, x fastest) zero-
+ * padded to n x n, forward transform (unnormalised), product with d_submask
+ * [n^2] (element kx + ky n: the stop in the focal plane, fftshifted as
+ * geom_init builds it), inverse transform (unnormalised), divided by n^2:
+ * the filtered field F on the npup x npup support, returned as amplitude
+ * d_ampl_out = |F| and phase d_phase_out = arg(F) / scale ([npup^2] each).
+ * Passed as d_pupil / d_phase of shwfs_sh_comp_image (with d_pupil_unit NULL)
+ * they give the spot field F * exp(-i offset).  d_work: [2 * n * npup] floats
+ * (complex n x npup rows).  Replaces the field-stop branch of
+ * SutraWfsSH::comp_generic (libsutra/src.cpp/sutra_wfs_sh.cpp:366-391:
+ * fsamplipup_krnl, CarmaFFT, submask_krnl, CarmaFFT, fillcamplipup from the
+ * focal field, libsutra/src.cu/sutra_wfs.cu:87-181, 1868-1888).
+ * n a power of two in [16, 4096], npup <= n, scale finite and non-zero. */
+int32_t shwfs_field_stop(const float* d_phase, const float* d_pupil, int32_t npup, float scale,
+                         const float* d_submask, int32_t n, void* d_work, float* d_phase_out,
+                         float* d_ampl_out, void* stream);
+
 /* ---- counter-based noise on a device array ----------------------------------
  * Replaces CarmaObj<float>::prng (libcarma/src.cpp/carma_obj.cpp:490-527,
  * src.cu/carma_rng.cu:98-141,287-303) with Philox4x32-10 streams:

@@ -59,4 +59,9 @@ PINNING = {
                                         "quadratic, whole-pixel shifts of a pattern are measured "
                                         "exactly through the full centroider",
     "corr.acquisim_bincube": "pinned: known-answer gather of listed windows",
+    "wfs.field_stop": "pinned: an all-pass stop returns the pupil field, a zero-frequency stop "
+                      "returns the field's mean, brute-force DFT pair on a small grid "
+                      "(tests/test_oracle_field_stop.py)",
+    "geometry.field_stop_mask": "pinned: centred and symmetric under both flips and the "
+                                "transpose, centre pixel kept; equals the product's mask",
 }

@@ -192,3 +192,23 @@ def sh_geometry(pupdiam, nxsub, pdiam, npix, nrebin, Nfft, Ntot, fracsub, cobs):
         halfxy=halfxy_table(pdiam, Nfft, npix, nrebin),
         binmap=binmap_table(Ntot, npix, nrebin),
     )
+
+
+def field_stop_mask(mpupil_side, pupdiam, lam_um, diam, fssize, fstop):
+    """Field-stop mask of geom_init (shesha/shesha/init/geom_init.py:973-995):
+    fftsize = fft_goodsize(mpupil_side), pixel fspixsize = pupdiam lambda / D
+    / fftsize (arcsec), radius r = int(fssize / fspixsize / 2); round: the
+    distance to the centre (fftsize/2 - 0.5 on both axes) below r; square:
+    |x|, |y| <= r about (fftsize + 1) / 2 of the 1-based indices; fftshifted."""
+    n = fft_goodsize(mpupil_side)
+    fspix = pupdiam * lam_um * 1e-6 / diam * RAD2ARCSEC / n
+    r = int(fssize / fspix / 2.0)
+    i = np.arange(n, dtype=np.float64)
+    if fstop == "round":
+        c = n / 2.0 - 0.5
+        m = np.sqrt((i[:, None] - c) ** 2 + (i[None, :] - c) ** 2) < r
+    else:
+        c = (n + 1.0) / 2.0
+        a = np.abs(i + 1 - c)
+        m = (a[:, None] <= r) & (a[None, :] <= r)
+    return np.fft.fftshift(m.astype(np.float64))

@@ -22,19 +22,33 @@ convolution kernconv included):
       b = b * fluxPerSub[s] / I[s] * nphot, skipped when I[s] == 0
 * bimg_krnl        libsutra/src.cu/sutra_wfs.cu:183-229 (alpha = 0)
       binimg[x + y*nsub + vx[s] + vy[s]*nsub] = bincube[x + y*npix + s*npix^2]
+* field stop       libsutra/src.cpp/sutra_wfs_sh.cpp:366-391:
+      fsamplipup_krnl (sutra_wfs.cu:130-152): f[x, y] = mask * exp(i phase*scale)
+      on the npup corner of an n x n zero grid; CarmaFFT forward; submask_krnl
+      (1868-1888): product with the fftshifted stop; CarmaFFT inverse
+      (unnormalised); camplipup_krnl's field-stop form (sutra_wfs.cu:87-128)
+      then takes E = F[p] * exp(-i offset) / n^2 (no pupil product: the mask
+      is already inside F).
 
 Arrays are indexed [x, y] (x = fast device axis).
 """
 import numpy as np
 
 
-def fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft):
-    """Zero-padded complex fields, shape (nvalid, Nfft, Nfft) indexed [s, x, y]."""
+def fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft, field=None):
+    """Zero-padded complex fields, shape (nvalid, Nfft, Nfft) indexed [s, x, y].
+    field: the complex pupil field [npup, npup] of a field stop (F / n^2,
+    from field_stop), which replaces pupil * exp(i phase * scale)."""
     nvalid = len(istart)
     E = np.zeros((nvalid, Nfft, Nfft), dtype=np.complex128)
+    off = np.asarray(offsets, dtype=np.float64)
+    if field is not None:
+        for s in range(nvalid):
+            i0, j0 = int(istart[s]), int(jstart[s])
+            E[s, :pdiam, :pdiam] = field[i0:i0 + pdiam, j0:j0 + pdiam] * np.exp(-1j * off)
+        return E
     ph = np.asarray(phase, dtype=np.float64)
     pu = np.asarray(pupil, dtype=np.float64)
-    off = np.asarray(offsets, dtype=np.float64)
     sc = float(np.float32(scale))
     for s in range(nvalid):
         i0, j0 = int(istart[s]), int(jstart[s])
@@ -42,6 +56,19 @@ def fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft):
         msk = pu[i0:i0 + pdiam, j0:j0 + pdiam]
         E[s, :pdiam, :pdiam] = msk * np.exp(1j * (sub * sc - off))
     return E
+
+
+def field_stop(phase, pupil, scale, submask):
+    """The field-stopped pupil field F / n^2 on the npup x npup support
+    (arrays indexed [x, y]; submask [n, n] indexed [kx, ky])."""
+    ph = np.asarray(phase, dtype=np.float64)
+    npup = ph.shape[0]
+    n = submask.shape[0]
+    f = np.zeros((n, n), dtype=np.complex128)
+    f[:npup, :npup] = np.asarray(pupil, dtype=np.float64) * np.exp(1j * ph * float(np.float32(scale)))
+    # np.fft.ifft2 divides by n^2: the unnormalised inverse over n^2
+    F = np.fft.ifft2(np.fft.fft2(f) * np.asarray(submask, dtype=np.float64))
+    return F[:npup, :npup]
 
 
 def hr_images(E):
@@ -103,14 +130,16 @@ def binimg_from_cube(bincube, validsubsx, validsubsy, nsub, base=None):
 
 def comp_image(phase, pupil, scale, offsets, istart, jstart, validsubsx, validsubsy,
                binmap, fluxPerSub, nphot, pdiam, Nfft, npix, nxsub, Ntot=None, hrmap=None,
-               ftkernel=None):
+               ftkernel=None, submask=None):
     """Noise-free SH image: returns (bincube [px,py,s], intensities_raw, binimg [x,y]).
 
     ``intensities_raw`` is what the reference leaves in d_intensities: the
     per-subap sum *before* normalisation.  Ntot / hrmap: the enlarged grid
-    (binmap then indexes it); ftkernel: the kernel convolution (kernconv).
+    (binmap then indexes it); ftkernel: the kernel convolution (kernconv);
+    submask: the field stop.
     """
-    E = fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft)
+    field = None if submask is None else field_stop(phase, pupil, scale, submask)
+    E = fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft, field)
     hr = hr_images(E)
     if Ntot is not None and Ntot != Nfft:
         hr = extend_grid(hr, hrmap, Ntot)

@@ -74,6 +74,7 @@ def load():
     lib.shwfs_calibrate_u16.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp]
     lib.shwfs_digitalize.argtypes = [vp, vp, vp, vp, i32, i32, C.c_int64, vp]
     lib.shwfs_centroids_bpcog.argtypes = [vp, i32, C.POINTER(CentroidArgs), vp]
+    lib.shwfs_field_stop.argtypes = [vp, vp, i32, f32, vp, i32, vp, vp, vp, vp]
     lib.shwfs_centroids_corr.argtypes = [vp, i32, vp, vp, i32, i32, vp, i32, vp, vp, i32, i32, f32,
                                          f32, vp, i32, vp, vp, vp, vp]
     lib.shwfs_fill_bincube.argtypes = [vp, vp, vp, vp, i32, i32, i32, vp]
