@@ -41,6 +41,76 @@ static void declare_telescope(py::module& m) {
                              py::return_value_policy::reference_internal);
 }
 
+static void declare_lgs(py::module& m) {
+  using C = hipFloatComplex;
+  py::class_<SutraLGS>(m, "LGS")
+      .def_property_readonly("device", [](SutraLGS& l) { return l.device; })
+      .def_property_readonly("nvalid", [](SutraLGS& l) { return l.nvalid; })
+      .def_property_readonly("npix", [](SutraLGS& l) { return l.npix; })
+      .def_property_readonly("nprof", [](SutraLGS& l) { return l.nprof; })
+      .def_property_readonly("nmaxhr", [](SutraLGS& l) { return l.nmaxhr; })
+      .def_property_readonly("hg", [](SutraLGS& l) { return l.hg; })
+      .def_property_readonly("h0", [](SutraLGS& l) { return l.h0; })
+      .def_property_readonly("deltah", [](SutraLGS& l) { return l.deltah; })
+      .def_property_readonly("pixsize", [](SutraLGS& l) { return l.pixsize; })
+      .def_property_readonly("d_doffaxis", [](SutraLGS& l) { return l.d_doffaxis; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_azimuth", [](SutraLGS& l) { return l.d_azimuth; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_prof1d", [](SutraLGS& l) { return l.d_prof1d; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_profcum", [](SutraLGS& l) { return l.d_profcum; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_prof2d", [](SutraLGS& l) { return l.d_prof2d; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_beam", [](SutraLGS& l) { return l.d_beam; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_ftbeam", [](SutraLGS& l) { return l.d_ftbeam; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_lgskern", [](SutraLGS& l) { return l.d_lgskern; },
+                             py::return_value_policy::reference_internal)
+      .def_property_readonly("d_ftlgskern", [](SutraLGS& l) { return l.d_ftlgskern; },
+                             py::return_value_policy::reference_internal)
+      .def("lgs_init",
+           [](SutraLGS& l, int32_t nprof, float hg, float h0, float deltah, float pixsize,
+              ArrayFStyle<float>& doffaxis, ArrayFStyle<float>& prof1d,
+              ArrayFStyle<float>& profcum, ArrayFStyle<float>& beam,
+              py::array_t<std::complex<float>, py::array::f_style | py::array::forcecast>& ftbeam,
+              ArrayFStyle<float>& azimuth) {
+             if (doffaxis.size() != l.nvalid || azimuth.size() != l.nvalid)
+               throw std::runtime_error("lgs_init: doffaxis and azimuth need nvalid values");
+             if (prof1d.size() < nprof || profcum.size() < nprof + 1)
+               throw std::runtime_error("lgs_init: prof1d needs nprof values, profcum nprof + 1");
+             if (beam.size() != l.npix || ftbeam.size() != l.npix)
+               throw std::runtime_error("lgs_init: beam and ftbeam need npix values");
+             return l.lgs_init(nprof, hg, h0, deltah, pixsize, doffaxis.mutable_data(),
+                               prof1d.mutable_data(), profcum.mutable_data(), beam.mutable_data(),
+                               reinterpret_cast<C*>(ftbeam.mutable_data()),
+                               azimuth.mutable_data());
+           },
+           py::arg("nprof"), py::arg("hg"), py::arg("h0"), py::arg("deltah"), py::arg("pixsize"),
+           py::arg("doffaxis"), py::arg("prof1d"), py::arg("profcum"), py::arg("beam"),
+           py::arg("ftbeam"), py::arg("azimuth"))
+      .def("load_prof",
+           [](SutraLGS& l, ArrayFStyle<float>& prof1d, ArrayFStyle<float>& profcum, float hg,
+              float h0, float deltah) {
+             if (prof1d.size() < l.nprof || profcum.size() < l.nprof + 1)
+               throw std::runtime_error("load_prof: prof1d needs nprof values, profcum nprof + 1");
+             return l.load_prof(prof1d.mutable_data(), profcum.mutable_data(), hg, h0, deltah);
+           },
+           py::arg("prof1d"), py::arg("profcum"), py::arg("hg"), py::arg("h0"), py::arg("deltah"))
+      .def("lgs_update", [](SutraLGS& l) { return l.lgs_update(); })
+      .def("lgs_makespot", [](SutraLGS& l, int32_t nin) { return l.lgs_makespot(nullptr, nin); },
+           py::arg("nin") = 0)
+      .def("load_kernels",
+           [](SutraLGS& l, ArrayFStyle<float>& k) {
+             if (k.size() != l.npix * l.npix * l.nvalid)
+               throw std::runtime_error("load_kernels: npix * npix * nvalid values needed");
+             return l.load_kernels(k.mutable_data());
+           },
+           py::arg("lgskern"));
+}
+
 static void declare_source(py::module& m) {
   py::class_<SutraSource>(m, "Source")
       .def_property_readonly("device", [](SutraSource& s) { return s.device; })
@@ -53,6 +123,8 @@ static void declare_source(py::module& m) {
       .def_property_readonly("zp", [](SutraSource& s) { return s.zp; })
       .def_property_readonly("scale", [](SutraSource& s) { return s.scale; })
       .def_property_readonly("lgs", [](SutraSource& s) { return s.lgs; })
+      .def_property_readonly("d_lgs", [](SutraSource& s) { return s.d_lgs; },
+                             py::return_value_policy::reference_internal)
       .def_property_readonly("d_phase",
                              [](SutraSource& s) {
                                s.render_layers();  // pending raytrace layers, if any
@@ -599,6 +671,7 @@ PYBIND11_MODULE(sutra, m) {
   py::module::import("carma");  // registers the CarmaObj / CarmaContext types
   declare_telescope(m);
   declare_atmos(m);
+  declare_lgs(m);
   declare_source(m);
   declare_wfs(m);
   declare_wfs_sh(m);

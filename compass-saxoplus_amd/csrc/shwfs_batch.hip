@@ -432,8 +432,8 @@ int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* f
       rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: the learned centroider is not fused in batches");
       break;
     }
-    if (geoms[i].ntot != geoms[i].nfft || geoms[i].kernconv) {
-      rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: ntot != nfft / kernconv run per WFS");
+    if (ext_needed(&geoms[i])) {
+      rc = fail(SHWFS_ERR_UNSUPPORTED, "batch_create: ntot != nfft / kernconv / LGS run per WFS");
       break;
     }
     if (frames[i].nlayers > 0) {

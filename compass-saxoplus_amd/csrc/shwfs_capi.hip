@@ -364,6 +364,27 @@ __global__ __launch_bounds__(256) void fill_binimg_kernel(const float* bcube, fl
   }
 }
 
+bool ext_needed(const shwfs_sh_geom* g) {
+  return g->ntot != g->nfft || g->kernconv || g->d_ftlgskern != nullptr;
+}
+
+// comp_generic's order (libsutra/src.cpp/sutra_wfs_sh.cpp:409-525): with
+// ntot != nfft the LGS kernels and then the kernel; with ntot == nfft the LGS
+// kernels alone, or the kernel alone.  Two unnormalised transform pairs
+// multiply the result by ntot^2 once more than one pair over the product of
+// both kernels: kscale keeps the reference's raw flux.
+ExtParams ext_params(const shwfs_sh_geom* g) {
+  const bool lgs = g->d_ftlgskern != nullptr;
+  const bool kern = g->kernconv && (g->ntot != g->nfft || !lgs);
+  ExtParams x{};
+  x.hrmap = g->ntot != g->nfft ? g->d_hrmap : nullptr;
+  x.ftkernel = kern ? reinterpret_cast<const float2*>(g->d_ftkernel) : nullptr;
+  x.ftlgs = lgs ? reinterpret_cast<const float2*>(g->d_ftlgskern) : nullptr;
+  x.conv = (kern || lgs) ? 1 : 0;
+  x.kscale = (kern && lgs) ? (float)g->ntot * (float)g->ntot : 1.0f;
+  return x;
+}
+
 int32_t check_geom(const shwfs_sh_geom* g, int method) {
   if (!g) return fail(SHWFS_ERR_ARG, "geometry is NULL");
   if (g->nvalid <= 0 || g->nphase <= 0 || g->npup <= 0 || g->npix <= 0 || g->nrebin <= 0 ||
@@ -371,7 +392,9 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
     return fail(SHWFS_ERR_ARG, "geometry sizes must be positive");
   if (!(g->nfft == 8 || g->nfft == 16 || g->nfft == 32 || g->nfft == 64))
     return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
-  const bool ext = g->ntot != g->nfft || g->kernconv;
+  const bool ext = ext_needed(g);
+  if (g->d_ftlgskern && !(g->ntot == 16 || g->ntot == 32 || g->ntot == 64))
+    return fail(SHWFS_ERR_UNSUPPORTED, "LGS kernels: needs ntot in {16, 32, 64}");
   if (g->ntot != g->nfft) {
     if (!(g->ntot == 16 || g->ntot == 32 || g->ntot == 64) || g->ntot < g->nfft || g->nfft < 16)
       return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft: needs nfft in {16, 32, 64} and ntot a "
@@ -380,9 +403,9 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
   }
   if (g->kernconv && !g->d_ftkernel) return fail(SHWFS_ERR_STATE, "kernconv: ftkernel not loaded");
   if (ext && method != SHWFS_METHOD_FFT_FP32)
-    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv: FFT method only");
+    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv / LGS: FFT method only");
   if (ext && g->nfft < 16)
-    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv: needs nfft in {16, 32, 64}");
+    return fail(SHWFS_ERR_UNSUPPORTED, "ntot != nfft / kernconv / LGS: needs nfft in {16, 32, 64}");
   if (g->nphase > g->nfft) return fail(SHWFS_ERR_ARG, "nphase must be <= nfft");
   if (g->npix * g->nrebin > g->ntot) return fail(SHWFS_ERR_ARG, "npix*nrebin must be <= ntot");
   // the kernels index the spot cube, the detector image and the screen with
@@ -408,10 +431,10 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
   if (!f || (!f->d_phase && f->nlayers <= 0) || !f->d_pupil || !f->d_bincube ||
       !f->d_intensities)
     return fail(SHWFS_ERR_ARG, "frame: phase (or layers), pupil, bincube and intensities are required");
-  if ((g->ntot != g->nfft || g->kernconv) &&
+  if (ext_needed(g) &&
       (f->nlayers > 0 || (f->centroid && f->centroid->kind == SHWFS_CENTROID_MLP)))
     return fail(SHWFS_ERR_UNSUPPORTED,
-                "ntot != nfft / kernconv: no layers and no fused learned centroider");
+                "ntot != nfft / kernconv / LGS: no layers and no fused learned centroider");
   if (f->nlayers > 0) {
     if (f->method != SHWFS_METHOD_FFT_FP32)
       return fail(SHWFS_ERR_UNSUPPORTED, "frame: the raytrace in the field stage runs in the FFT method only");
@@ -594,12 +617,7 @@ int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, v
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
     if (rc) return rc;
   }
-  if (g->ntot != g->nfft || g->kernconv) {
-    const ExtParams x{g->ntot != g->nfft ? g->d_hrmap : nullptr,
-                      g->kernconv ? reinterpret_cast<const float2*>(g->d_ftkernel) : nullptr,
-                      g->kernconv ? 1 : 0};
-    return launch_image_ext(p, x, g->nfft, g->ntot, st);
-  }
+  if (ext_needed(g)) return launch_image_ext(p, ext_params(g), g->nfft, g->ntot, st);
   if (f->nlayers > 0) {
     RayParams r;
     rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
@@ -639,12 +657,7 @@ int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* g, const shwfs_frame_ar
     rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
     if (rc) return rc;
   }
-  if (g->ntot != g->nfft || g->kernconv) {
-    const ExtParams x{g->ntot != g->nfft ? g->d_hrmap : nullptr,
-                      g->kernconv ? reinterpret_cast<const float2*>(g->d_ftkernel) : nullptr,
-                      g->kernconv ? 1 : 0};
-    return launch_image_ext(p, x, g->nfft, g->ntot, st);
-  }
+  if (ext_needed(g)) return launch_image_ext(p, ext_params(g), g->nfft, g->ntot, st);
   if (f->nlayers > 0) {
     RayParams r;
     rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);

@@ -9,8 +9,9 @@
 //      index hrmap[k] of a zero grid (indexfill_krnl,
 //      libsutra/src.cu/sutra_wfs.cu:462-479); the identity when NT == NF
 //   3. CONV: forward FFT on NT, product with ftkernel (conv_krnl,
-//      sutra_wfs.cu:491-513), unnormalised inverse FFT (CarmaFFT dir -1),
-//      real part
+//      sutra_wfs.cu:491-513) and / or with this sub-aperture's LGS kernel
+//      (convolve, sutra_utils.cu:553-564), unnormalised inverse FFT
+//      (CarmaFFT dir -1), real part
 //   4. binning through binmap (indices on the NT grid, fillbincube_krnl,
 //      sutra_wfs.cu:369-414), then the shared tail (flux, normalisation,
 //      noise, bincube / binimg, fused centroider) by the first wave.
@@ -88,9 +89,17 @@ __global__ __launch_bounds__(kExtThreads) void sh_image_ext_kernel(ImageParams P
   if constexpr (CONV) {
     lds_fft_lines<NT, true, false>(B, tid);
     lds_fft_lines<NT, false, false>(B, tid);
+    const float2* lk = X.ftlgs ? X.ftlgs + (size_t)s * NT * NT : nullptr;
     for (int k = tid; k < NT * NT; k += kExtThreads) {
       const int u = k % NT, v = k / NT;
-      const float2 a = B[v * ST + u], w = X.ftkernel[k];
+      const float2 a = B[v * ST + u];
+      float2 w;
+      if (lk && X.ftkernel) {  // both kernels: one product, times kscale
+        const float2 l = lk[k], c = X.ftkernel[k];
+        w = make_float2((l.x * c.x - l.y * c.y) * X.kscale, (l.x * c.y + l.y * c.x) * X.kscale);
+      } else {
+        w = lk ? lk[k] : X.ftkernel[k];
+      }
       B[v * ST + u] = make_float2(w.x * a.x - w.y * a.y, w.y * a.x + w.x * a.y);
     }
     __syncthreads();
@@ -143,6 +152,91 @@ int32_t launch_image_ext(const ImageParams& p, const ExtParams& x, int nfft, int
 #undef SHWFS_EXT
   return fail(SHWFS_ERR_UNSUPPORTED,
               "enlarged grid / kernel convolution: needs nfft in {16, 32, 64}, ntot in {nfft, ..., 64}");
+}
+
+// ---- LGS elongated kernels: image, rotation, forward transform -------------
+// One workgroup per sub-aperture (SutraLGS::lgs_makespot, libsutra/src.cpp/
+// sutra_lgs.cpp:180-212): rollbeamexp_krnl (sutra_lgs.cu:106-129) then
+// rotate_krnl (155-226) into an NT x NT image, 2-D forward transform.
+
+struct LgsKernArgs {
+  float2* ftlgs;         // [NT^2 * nvalid]
+  float* lgskern;        // [NT^2 * nvalid] or NULL
+  const float2* prof2d;  // [NT * nvalid] or NULL (transform lgskern as given)
+  const float* beam;     // [NT]
+  const float* azimuth;  // [nvalid]
+};
+
+// bilinear sample index and weight along one axis, as rotate_krnl clamps it
+__device__ __forceinline__ void lgs_axis(float t, int n, int& ref, float& w) {
+  if (t < 0.0f) {
+    ref = 0;
+    w = 0.0f;
+  } else if (t < (float)(n - 2)) {
+    ref = (int)t;
+    w = t - (float)ref;
+  } else {
+    ref = n - 2;
+    w = 1.0f;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(kExtThreads) void lgs_kernels_kernel(LgsKernArgs a) {
+  constexpr int ST = NT + 1;
+  __shared__ float img[NT * NT];
+  __shared__ float2 B[NT * ST];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const size_t base = (size_t)s * NT * NT;
+  if (a.prof2d) {
+    const float2* pr = a.prof2d + (size_t)s * NT;
+    for (int k = tid; k < NT * NT; k += kExtThreads) {
+      const int x = k % NT, y = k / NT;
+      img[k] = fabsf(pr[(x + NT / 2) % NT].x) * a.beam[y];
+    }
+    __syncthreads();
+    const float th = a.azimuth[s];
+    const float cs = cosf(th), sn = sinf(th);
+    const float cent = (float)(NT / 2);  // width / 2 - center, center 0
+    for (int k = tid; k < NT * NT; k += kExtThreads) {
+      const int x = k % NT, y = k / NT;
+      const float u = (float)x - cent, v = (float)y - cent;
+      const float tu = u * cs + v * sn + cent;
+      const float tv = -u * sn + v * cs + cent;
+      int ur, vr;
+      float wu, wv;
+      lgs_axis(tu, NT, ur, wu);
+      lgs_axis(tv, NT, vr, wv);
+      const float i1 = img[ur + vr * NT], i2 = img[ur + 1 + vr * NT];
+      const float i3 = img[ur + (vr + 1) * NT], i4 = img[ur + 1 + (vr + 1) * NT];
+      const float val = (i1 * (1 - wu) + i2 * wu) * (1 - wv) + (i3 * (1 - wu) + i4 * wu) * wv;
+      B[y * ST + x] = make_float2(val, 0.0f);
+      if (a.lgskern) a.lgskern[base + k] = val;
+    }
+  } else {
+    for (int k = tid; k < NT * NT; k += kExtThreads)
+      B[(k / NT) * ST + (k % NT)] = make_float2(a.lgskern[base + k], 0.0f);
+  }
+  __syncthreads();
+  lds_fft_lines<NT, true, false>(B, tid);
+  lds_fft_lines<NT, false, false>(B, tid);
+  for (int k = tid; k < NT * NT; k += kExtThreads) a.ftlgs[base + k] = B[(k / NT) * ST + (k % NT)];
+}
+
+int32_t launch_lgs_kernels(float2* ftlgs, float* lgskern, const float2* prof2d, const float* beam,
+                           const float* azimuth, int ntot, int nvalid, hipStream_t st) {
+  const LgsKernArgs a{ftlgs, lgskern, prof2d, beam, azimuth};
+  if (nvalid <= 0) return SHWFS_OK;
+  if (ntot == 16)
+    hipLaunchKernelGGL(lgs_kernels_kernel<16>, dim3(nvalid), dim3(kExtThreads), 0, st, a);
+  else if (ntot == 32)
+    hipLaunchKernelGGL(lgs_kernels_kernel<32>, dim3(nvalid), dim3(kExtThreads), 0, st, a);
+  else if (ntot == 64)
+    hipLaunchKernelGGL(lgs_kernels_kernel<64>, dim3(nvalid), dim3(kExtThreads), 0, st, a);
+  else
+    return fail(SHWFS_ERR_UNSUPPORTED, "lgs_kernels: ntot must be 16, 32 or 64");
+  SHWFS_LAUNCHED("lgs_kernels_kernel");
+  return SHWFS_OK;
 }
 
 }  // namespace shwfs

@@ -165,11 +165,20 @@ int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
 // workgroup per sub-aperture (shwfs_image_ext.hip).
 struct ExtParams {
   const int* hrmap;        // [nfft^2] or NULL when ntot == nfft
-  const float2* ftkernel;  // [ntot^2] or NULL without convolution
-  int conv;
+  const float2* ftkernel;  // [ntot^2] or NULL without kernel convolution
+  int conv;                // 1: convolve (kernel, LGS kernels or both)
+  const float2* ftlgs;     // [ntot^2 * nvalid] per-sub-aperture LGS kernels, or NULL
+  float kscale;            // factor of the product of both kernels (ntot^2, else 1)
 };
+// The parameters of the enlarged-grid / convolution kernel for a geometry,
+// and whether the geometry needs that kernel.
+bool ext_needed(const shwfs_sh_geom* g);
+ExtParams ext_params(const shwfs_sh_geom* g);
 int32_t launch_image_ext(const ImageParams& p, const ExtParams& x, int nfft, int ntot,
                          hipStream_t s);
+// The LGS kernels' rotation and 2-D transform (shwfs_image_ext.hip).
+int32_t launch_lgs_kernels(float2* ftlgs, float* lgskern, const float2* prof2d, const float* beam,
+                           const float* azimuth, int ntot, int nvalid, hipStream_t s);
 // The FFT spot kernel tracing its window phases through layers (RayParams,
 // shwfs_raytrace.hpp) instead of reading p.phase.
 struct RayParams;

@@ -79,9 +79,141 @@ SutraSource::SutraSource(CarmaContext* context, float xpos, float ypos, float la
       lazy_raytrace(true) {
   context->set_active_device(device, 1);
   d_phase = new SutraPhase(context, size);
+  d_lgs = nullptr;
 }
 
-SutraSource::~SutraSource() { del(d_phase); }
+SutraSource::~SutraSource() {
+  del(d_phase);
+  del(d_lgs);
+}
+
+// ============================================================ SutraLGS
+// reference: libsutra/src.cpp/sutra_lgs.cpp:26-250
+SutraLGS::SutraLGS(CarmaContext* context, CarmaObj<float>* lgskern,
+                   CarmaObj<hipFloatComplex>* ftlgskern, PlanMap& ftlgskern_plans,
+                   int64_t nvalid_, int64_t npix_, int64_t nmaxhr_)
+    : device(context->get_active_device()),
+      nvalid(nvalid_),
+      npix(npix_),
+      nmaxhr(nmaxhr_),
+      nprof(0),
+      hg(0.f),
+      h0(0.f),
+      deltah(0.f),
+      pixsize(0.f),
+      ready(false),
+      d_doffaxis(nullptr),
+      d_azimuth(nullptr),
+      d_prof1d(nullptr),
+      d_profcum(nullptr),
+      current_context(context) {
+  (void)lgskern;
+  (void)ftlgskern;
+  (void)ftlgskern_plans;
+  int64_t d1[2] = {1, npix};
+  d_beam = new CarmaObj<float>(context, d1);
+  d_ftbeam = new CarmaObj<hipFloatComplex>(context, d1);
+  int64_t dv[2] = {1, nvalid};
+  d_azimuth = new CarmaObj<float>(context, dv);
+  int64_t d2[3] = {2, npix, nvalid};
+  d_prof2d = new CarmaObj<hipFloatComplex>(context, d2);
+  int64_t d3[4] = {3, npix, npix, nvalid};
+  d_lgskern = new CarmaObj<float>(context, d3);
+  d_ftlgskern = new CarmaObj<hipFloatComplex>(context, d3);
+}
+
+SutraLGS::~SutraLGS() {
+  del(d_doffaxis);
+  del(d_azimuth);
+  del(d_prof1d);
+  del(d_profcum);
+  del(d_prof2d);
+  del(d_beam);
+  del(d_ftbeam);
+  del(d_lgskern);
+  del(d_ftlgskern);
+}
+
+int32_t SutraLGS::lgs_init(int32_t nprof_, float hg_, float h0_, float deltah_, float pixsize_,
+                           float* doffaxis, float* prof1d, float* profcum, float* beam,
+                           hipFloatComplex* ftbeam, float* azimuth) {
+  if (nprof_ < 2) throw std::runtime_error("lgs_init: the profile needs at least 2 samples");
+  current_context->set_active_device(device, 1);
+  nprof = nprof_;
+  hg = hg_;
+  h0 = h0_;
+  deltah = deltah_;
+  pixsize = pixsize_;
+  del(d_doffaxis);
+  del(d_prof1d);
+  del(d_profcum);
+  int64_t dv[2] = {1, nvalid};
+  d_doffaxis = new CarmaObj<float>(current_context, dv, doffaxis);
+  // nprof + 1 values as the reference allocates; the profile gives nprof
+  // (the last one is never read), the cumulative sum nprof + 1
+  int64_t dp[2] = {1, (int64_t)nprof + 1};
+  std::vector<float> p1(prof1d, prof1d + nprof);
+  p1.push_back(0.0f);
+  d_prof1d = new CarmaObj<float>(current_context, dp, p1.data());
+  d_profcum = new CarmaObj<float>(current_context, dp, profcum);
+  d_beam->host2device(beam);
+  d_ftbeam->host2device(ftbeam);
+  d_azimuth->host2device(azimuth);
+  return lgs_update();
+}
+
+int32_t SutraLGS::load_prof(float* prof1d, float* profcum, float hg_, float h0_, float deltah_) {
+  if (d_prof1d == nullptr) throw std::runtime_error("load_prof: lgs_init has not been called");
+  current_context->set_active_device(device, 1);
+  std::vector<float> p1(prof1d, prof1d + nprof);
+  p1.push_back(0.0f);
+  d_prof1d->host2device(p1.data());
+  d_profcum->host2device(profcum);
+  hg = hg_;
+  h0 = h0_;
+  deltah = deltah_;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraLGS::lgs_update(CarmaDevice* dev, hipStream_t stream) {
+  (void)dev;
+  if (d_prof1d == nullptr) throw std::runtime_error("lgs_update: lgs_init has not been called");
+  current_context->set_active_device(device, 1);
+  shwfs_throw_if(shwfs_lgs_prof2d(reinterpret_cast<float*>(d_prof2d->get_data()),
+                                  d_prof1d->get_data(), d_profcum->get_data(), nprof,
+                                  d_doffaxis->get_data(), hg, pixsize, h0, deltah,
+                                  reinterpret_cast<const float*>(d_ftbeam->get_data()),
+                                  (int32_t)npix, (int32_t)nvalid, stream),
+                 "lgs_update");
+  return lgs_makespot(nullptr, 0, stream);
+}
+
+int32_t SutraLGS::lgs_makespot(CarmaDevice* dev, int32_t nin, hipStream_t stream) {
+  (void)dev;
+  (void)nin;  // every sub-aperture at once
+  if (d_prof1d == nullptr) throw std::runtime_error("lgs_makespot: lgs_init has not been called");
+  current_context->set_active_device(device, 1);
+  shwfs_throw_if(shwfs_lgs_kernels(reinterpret_cast<float*>(d_ftlgskern->get_data()),
+                                   d_lgskern->get_data(),
+                                   reinterpret_cast<const float*>(d_prof2d->get_data()),
+                                   d_beam->get_data(), d_azimuth->get_data(), (int32_t)npix,
+                                   (int32_t)nvalid, stream),
+                 "lgs_makespot");
+  ready = true;
+  return EXIT_SUCCESS;
+}
+
+int32_t SutraLGS::load_kernels(float* h_lgskern, CarmaDevice* dev) {
+  (void)dev;
+  current_context->set_active_device(device, 1);
+  d_lgskern->host2device(h_lgskern);
+  shwfs_throw_if(shwfs_lgs_kernels(reinterpret_cast<float*>(d_ftlgskern->get_data()),
+                                   d_lgskern->get_data(), nullptr, nullptr, nullptr,
+                                   (int32_t)npix, (int32_t)nvalid, nullptr),
+                 "load_kernels");
+  ready = true;
+  return EXIT_SUCCESS;
+}
 
 int32_t SutraSource::set_phase(float* data) {
   pending.clear();
@@ -113,7 +245,8 @@ int32_t SutraSource::remove_layer(string ltype, int32_t idx) {
 int32_t SutraSource::raytrace(SutraAtmos* atmos, bool async) {
   (void)async;  // one launch either way
   if (atmos == nullptr) throw std::invalid_argument("raytrace: atmos is None");
-  if (lgs) throw std::runtime_error("raytrace: LGS cone compression is not part of this build");
+  if (lgs && (d_lgs == nullptr || d_lgs->hg == 0.0f))
+    throw std::runtime_error("raytrace: LGS source without lgs_init (its altitude is unknown)");
   current_context->set_active_device(device, 1);
   vector<shwfs_layer> layers;
   for (auto& kv : xoff) {
@@ -132,7 +265,8 @@ int32_t SutraSource::raytrace(SutraAtmos* atmos, bool async) {
     l.thetaML = thetaML;
     l.dx = dx;
     l.dy = dy;
-    l.delta = 1.0f;
+    // cone compression of the layer seen from the LGS (sutra_source.cpp:250-252)
+    l.delta = lgs ? 1.0f - ps->altitude / d_lgs->hg : 1.0f;
     layers.push_back(l);
   }
   if ((int32_t)layers.size() > SHWFS_MAX_LAYERS)
@@ -275,12 +409,18 @@ int32_t SutraWfs::wfs_initgs(float xpos, float ypos, float lambda, float mag, fl
                              int64_t size, float noise_, int64_t seed, float G, float thetaML,
                              float dx, float dy) {
   current_context->set_active_device(device, 1);
-  if (lgs) throw std::runtime_error("LGS wavefront sensors are not part of this build");
   d_gs = new SutraSource(current_context, xpos, ypos, lambda, mag, zerop, size, "wfs", device);
   d_gs->G = G;
   d_gs->thetaML = thetaML;
   d_gs->dx = dx;
   d_gs->dy = dy;
+  if (lgs) {  // reference: libsutra/src.cpp/sutra_wfs.cpp:114-119
+    if (ntot != 16 && ntot != 32 && ntot != 64)
+      throw std::runtime_error("LGS WFS: the HR support ntot must be 16, 32 or 64");
+    PlanMap none;
+    d_gs->d_lgs = new SutraLGS(current_context, nullptr, nullptr, none, nvalid, ntot, nmaxhr);
+    d_gs->lgs = true;
+  }
   return set_noise(noise_, seed);
 }
 
@@ -608,6 +748,10 @@ shwfs_sh_geom SutraWfsSH::geom() const {
     g.d_hrmap = d_hrmap ? d_hrmap->get_data() : nullptr;
     g.d_ftkernel = reinterpret_cast<const float*>(d_ftkernel->get_data());
     g.kernconv = kernconv ? 1 : 0;
+    // the LGS kernels once built (frames check lgs_ready first)
+    SutraLGS* l = (lgs && d_gs) ? d_gs->d_lgs : nullptr;
+    g.d_ftlgskern = (l && l->ready) ? reinterpret_cast<const float*>(l->d_ftlgskern->get_data())
+                                    : nullptr;
   }
   return g;
 }
@@ -647,6 +791,11 @@ bool SutraWfsSH::layers_in_field_stage() const {
          get_method() == SHWFS_METHOD_FFT_FP32 && (wg == 0 || wg == 64);
 }
 
+void SutraWfsSH::check_lgs_ready() const {
+  if (lgs && !(d_gs && d_gs->d_lgs && d_gs->d_lgs->ready))
+    throw std::runtime_error("LGS WFS: lgs_init (or load_kernels) has not been called");
+}
+
 // reference: the field-stop branch of comp_generic
 // (libsutra/src.cpp/sutra_wfs_sh.cpp:366-391)
 void SutraWfsSH::field_stop_stage(hipStream_t st) {
@@ -677,6 +826,7 @@ int32_t SutraWfsSH::comp_generic() {
   if (d_gs->d_phase->d_screen->get_dims(1) != d_pupil->get_dims(1))
     throw std::runtime_error("comp_image: phase screen and pupil sizes differ");
   current_context->set_active_device(device, 1);
+  check_lgs_ready();
   if (!layers_in_field_stage()) d_gs->render_layers(stream);
   field_stop_stage(stream);
   shwfs_sh_geom g = geom();

@@ -91,6 +91,53 @@ class SutraAtmos {
   ~SutraAtmos();
 };
 
+// Laser guide star of an SH WFS (reference: libsutra/include/sutra_lgs.hpp,
+// src.cpp/sutra_lgs.cpp): the sodium profile seen by every sub-aperture,
+// convolved with the beam (d_prof2d), and the elongated spot kernels with
+// their transforms (d_lgskern, d_ftlgskern), which the spot kernel of the WFS
+// multiplies into every frame's HR image.  npix is the HR support (ntot).
+// The kernels are built for all sub-apertures by lgs_init / lgs_update (and
+// kept until the next one); the reference rebuilds them for each chunk of
+// nmaxhr sub-apertures in every frame from the same profile, which gives the
+// same kernels.  Kernels given by load_kernels stay in place (in the reference
+// the next frame overwrites them from the profile).
+class SutraLGS {
+ public:
+  int32_t device;
+  int64_t nvalid, npix, nmaxhr;
+  int32_t nprof;
+  float hg, h0, deltah, pixsize;
+  bool ready;  // kernels built
+  CarmaObj<float>* d_doffaxis;
+  CarmaObj<float>* d_azimuth;
+  CarmaObj<float>* d_prof1d;
+  CarmaObj<float>* d_profcum;
+  CarmaObj<hipFloatComplex>* d_prof2d;  // [npix, nvalid]
+  CarmaObj<float>* d_beam;              // [npix]
+  CarmaObj<hipFloatComplex>* d_ftbeam;  // [npix]
+  CarmaObj<float>* d_lgskern;           // [npix, npix, nvalid]
+  CarmaObj<hipFloatComplex>* d_ftlgskern;
+  CarmaContext* current_context;
+
+  // d_lgskern / d_ftlgskern of the reference's shared buffers are not used:
+  // each LGS owns kernels for all its sub-apertures (pass NULL)
+  SutraLGS(CarmaContext* context, CarmaObj<float>* d_lgskern,
+           CarmaObj<hipFloatComplex>* d_ftlgskern, PlanMap& ftlgskern_plans, int64_t nvalid,
+           int64_t npix, int64_t nmaxhr);
+  ~SutraLGS();
+  int32_t lgs_init(int32_t nprof, float hg, float h0, float deltah, float pixsize,
+                   float* doffaxis, float* prof1d, float* profcum, float* beam,
+                   hipFloatComplex* ftbeam, float* azimuth);
+  int32_t load_prof(float* prof1d, float* profcum, float hg, float h0, float deltah);
+  // profile -> prof2d, then the kernels (lgs_makespot) for every sub-aperture
+  int32_t lgs_update(CarmaDevice* device = nullptr, hipStream_t stream = nullptr);
+  // kernels of the sub-apertures from nin / npix on (the reference's chunk
+  // offset nin counts profile elements); here always all of them
+  int32_t lgs_makespot(CarmaDevice* device = nullptr, int32_t nin = 0,
+                       hipStream_t stream = nullptr);
+  int32_t load_kernels(float* h_lgskern, CarmaDevice* device = nullptr);
+};
+
 // Guide-star source: holds the phase seen by the WFS (µm) and scale = 2*pi/lambda
 // (reference: libsutra/src.cpp/sutra_source.cpp:25-100), and the layers it
 // sees (add_layer) with their offsets.
@@ -112,6 +159,7 @@ class SutraSource {
   float G, thetaML, dx, dy;
   string type;
   SutraPhase* d_phase;
+  SutraLGS* d_lgs;  // laser guide star (lgs), else NULL
   map<std::pair<string, int32_t>, float> xoff, yoff;  // (layer type, index) -> offset
   vector<shwfs_layer> pending;  // layers of the last raytrace, not rendered yet
   bool lazy_raytrace;
@@ -264,6 +312,8 @@ class SutraWfsSH : public SutraWfs {
   // with a field stop: render the guide star's layers, then the stopped
   // field into d_fsphase / d_fsampl (read by frame_args)
   void field_stop_stage(hipStream_t st);
+  // an LGS WFS computes frames only once its kernels exist
+  void check_lgs_ready() const;
   // Geometry descriptor handed to the C ABI.
   shwfs_sh_geom geom() const;
   // Frame arguments of the next comp_image (advances nothing).

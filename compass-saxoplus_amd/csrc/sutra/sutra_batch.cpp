@@ -28,6 +28,7 @@ SutraWfsBatch::SutraWfsBatch(CarmaContext* context, vector<SutraWfsSH*> wfs_,
       throw std::invalid_argument("WfsBatch: roket WFS (image before the noise) are run through comp_image");
     if (w->d_submask != nullptr)
       throw std::invalid_argument("WfsBatch: WFS with a field stop are run through comp_image");
+    if (w->lgs) throw std::invalid_argument("WfsBatch: LGS WFS are run through comp_image");
     int64_t d[2] = {1, 2 * w->nvalid};
     d_slopes.push_back(new CarmaObj<float>(context, d));
     geoms.push_back(w->geom());

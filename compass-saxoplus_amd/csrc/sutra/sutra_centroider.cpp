@@ -732,6 +732,7 @@ int32_t comp_frame(SutraWfsSH* wfs, SutraCentroider<float, float>* c, float* int
     return c->get_cog(intensities, slopes, noise);
   }
   // the fused learned centroider and the traced field stage are separate kernels
+  wfs->check_lgs_ready();
   if (a.kind == SHWFS_CENTROID_MLP || !wfs->layers_in_field_stage())
     wfs->d_gs->render_layers(wfs->stream);
   wfs->field_stop_stage(wfs->stream);

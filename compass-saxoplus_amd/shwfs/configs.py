@@ -74,7 +74,18 @@ def wide():
     return sh_config(4.0, 8, 8, 8, 16, 3, r0=0.16, seeds=(1,), name="wide")
 
 
-WORKLOADS = {"small": small, "noisy": noisy, "large": large, "lowflux": lowflux, "wide": wide}
+def lgs(lltx=0.0, llty=0.0):
+    """8x8 SH on an 8 m telescope with a sodium laser guide star at 90 km
+    (one-Gaussian layer), beam FWHM 0.8 arcsec launched from (lltx, llty) m:
+    pdiam 16, Nfft = Ntot 32, 4x4 binning to 8x8 px."""
+    c = sh_config(8.0, 8, 16, 8, 32, 4, r0=0.16, seeds=(1,), name="lgs")
+    w = c["p_wfs"]
+    w.gsalt, w.beamsize, w.lltx, w.llty = 90000.0, 0.8, lltx, llty
+    return c
+
+
+WORKLOADS = {"small": small, "noisy": noisy, "large": large, "lowflux": lowflux, "wide": wide,
+             "lgs": lgs}
 
 
 def get(name):

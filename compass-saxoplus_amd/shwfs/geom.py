@@ -149,8 +149,13 @@ def init_wfs_geom(p_wfs, r0, p_tel, p_geom, ittime=1.0, nfft=None, nrebin=None):
     init_sh_geom(p_wfs, p_tel, p_geom, ittime=ittime, r0=r0)
 
 
-def compute_nphotons(ittime, optthroughput, diam, nxsub, zerop=0.0, gsmag=0.0):
-    """NGS photons per fully illuminated sub-aperture per frame."""
+def compute_nphotons(ittime, optthroughput, diam, nxsub, zerop=0.0, gsmag=0.0,
+                     lgsreturnperwatt=0.0, laserpower=0.0):
+    """Photons per fully illuminated sub-aperture per frame: from the guide
+    star magnitude, or for an LGS from its return per watt and laser power
+    (shesha/shesha/init/geom_init.py:396-458)."""
+    if laserpower > 0:
+        return lgsreturnperwatt * laserpower * optthroughput * (diam / nxsub) ** 2.0 * 1e4 * ittime
     if zerop == 0:
         zerop = 1e11
     return zerop * 10.0 ** (-0.4 * gsmag) * ittime * optthroughput * (diam / nxsub) ** 2.0
@@ -278,8 +283,11 @@ def init_sh_geom(p_wfs, p_tel, p_geom, ittime=1.0, r0=None):
     k = k / np.sum(k)
     p_wfs._ftkernel = (np.fft.fft2(k) / (Ntot * Ntot)).astype(np.complex64)
 
+    lgs = p_wfs.gsalt > 0
     p_wfs._nphotons = compute_nphotons(ittime, p_wfs.optthroughput, p_tel.diam, p_wfs.nxsub,
-                                       zerop=p_wfs.zerop, gsmag=p_wfs.gsmag)
+                                       zerop=p_wfs.zerop, gsmag=p_wfs.gsmag,
+                                       lgsreturnperwatt=p_wfs.lgsreturnperwatt if lgs else 0.0,
+                                       laserpower=p_wfs.laserpower if lgs else 0.0)
     p_wfs._submask = field_stop_mask(p_wfs, p_tel, p_geom)
 
 

@@ -56,6 +56,15 @@ class ParamWfs:
         self.max_pix_value = 0
         self.is_low_order = False
         self.nPupils = 1
+        # laser guide star (gsalt > 0): altitude (m), launch telescope position
+        # (m), beam FWHM (arcsec), return per watt and power, sodium profile
+        self.lltx = 0.0
+        self.llty = 0.0
+        self.beamsize = 0.0
+        self.lgsreturnperwatt = 0.0
+        self.laserpower = 0.0
+        self._altna = None
+        self._profna = None
         self.fssize = 0.0       # field stop size (arcsec), 0: none
         self.fstop = "round"    # "round" or "square"
         self._submask = None

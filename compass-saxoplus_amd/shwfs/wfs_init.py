@@ -7,7 +7,7 @@ built from per-WFS arrays, initgs attaches a guide-star source (noise seed
 """
 import numpy as np
 
-from . import geom, native
+from . import geom, lgs_init, native
 
 
 def tel_init(context, p_geom, p_tel, r0, p_wfss, ittime=1.0):
@@ -49,4 +49,6 @@ def wfs_init(context, telescope, p_wfss, p_tel, p_geom, seeds=None, roket=False)
                                      w._validpuppixx, w._validpuppixy, w._ttprojmat, w._ftkernel)
         if w._submask is not None:  # shesha/shesha/init/wfs_init.py:234-235
             sensors.set_field_stop(i, w._submask, w._submask.shape[0])
+        if w.gsalt > 0:  # shesha/shesha/init/wfs_init.py:237-242
+            lgs_init.prep_lgs_prof(w, i, p_tel, sensors)
     return sensors

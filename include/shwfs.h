@@ -77,6 +77,13 @@ typedef struct shwfs_sh_geom {
   const float* d_ftkernel;     /* [2 * ntot^2] complex (re, im) FFT of the convolution
                                   kernel / ntot^2; read when kernconv */
   int32_t kernconv;            /* 1: HR image convolved with the kernel before binning */
+  /* LGS elongation (SutraWfsSH::comp_generic with lgs, libsutra/src.cpp/
+   * sutra_wfs_sh.cpp:420-451, 483-512); FFT method, ntot in {16, 32, 64} */
+  const float* d_ftlgskern;    /* [2 * ntot^2 * nvalid] complex: per sub-aperture FFT of its
+                                  elongated spot kernel (shwfs_lgs_kernels), or NULL (NGS).
+                                  The HR image is convolved with it before binning; with
+                                  ntot != nfft and kernconv, then with ftkernel as well (with
+                                  ntot == nfft, kernconv is not applied, as the reference) */
 } shwfs_sh_geom;
 
 /* A centroider applied to a detector image (or fused after the spot stage).
@@ -262,6 +269,35 @@ int32_t shwfs_field_stop(const float* d_phase, const float* d_pupil, int32_t npu
                          const float* d_submask, int32_t n, void* d_work, float* d_phase_out,
                          float* d_ampl_out, void* stream);
 
+/* ---- LGS elongated spot kernels ---------------------------------------------
+ * Per sub-aperture s, the sodium profile seen along the elongation, sampled on
+ * the ntot HR pixels x (pixel size pixsize arcsec): with norm = 206265 *
+ * doffaxis[s] / hg^2 and t(x) = ((x - c) pixsize / norm + hg - h0) / deltah
+ * (c = ntot/2 + 0.5 for even ntot, ntot/2 for odd), linear interpolation of
+ * prof1d at t (index clamped to [0, nprof - 2]; 0 weight below 0, weight 1 at
+ * the top), or, when pixsize > deltah * norm, of profcum at t(x+1) minus at
+ * t(x); then convolved with the beam: unnormalised inverse DFT of DFT(p) *
+ * ftbeam.  d_prof2d [2 * ntot * nvalid] complex (x fastest).  Replaces
+ * SutraLGS::lgs_update (libsutra/src.cpp/sutra_lgs.cpp:160-178: iprof_krnl,
+ * CarmaFFT, tftbeam_krnl, CarmaFFT, libsutra/src.cu/sutra_lgs.cu:25-104).
+ * nprof >= 2; the first nprof values of prof1d and profcum are read. */
+int32_t shwfs_lgs_prof2d(float* d_prof2d, const float* d_prof1d, const float* d_profcum,
+                         int32_t nprof, const float* d_doffaxis, float hg, float pixsize,
+                         float h0, float deltah, const float* d_ftbeam, int32_t ntot,
+                         int32_t nvalid, void* stream);
+/* The elongated kernels and their transforms: image[x + y ntot] =
+ * |Re prof2d[(x + ntot/2) % ntot, s]| * beam[y], rotated by azimuth[s] about
+ * (ntot/2, ntot/2) with bilinear interpolation (clamped at the edges), stored
+ * in d_lgskern [ntot^2 * nvalid] (may be NULL), and its forward 2-D transform
+ * (unnormalised) in d_ftlgskern [2 * ntot^2 * nvalid].  With d_prof2d NULL the
+ * kernels are read from d_lgskern and only transformed (SutraLGS::
+ * load_kernels).  Replaces SutraLGS::lgs_makespot (sutra_lgs.cpp:180-212:
+ * rollbeamexp_krnl, rotate_krnl, CarmaFFT; sutra_lgs.cu:106-300).
+ * ntot in {16, 32, 64}. */
+int32_t shwfs_lgs_kernels(float* d_ftlgskern, float* d_lgskern, const float* d_prof2d,
+                          const float* d_beam, const float* d_azimuth, int32_t ntot,
+                          int32_t nvalid, void* stream);
+
 /* ---- counter-based noise on a device array ----------------------------------
  * Replaces CarmaObj<float>::prng (libcarma/src.cpp/carma_obj.cpp:490-527,
  * src.cu/carma_rng.cu:98-141,287-303) with Philox4x32-10 streams:
@@ -372,7 +408,7 @@ typedef struct shwfs_batch_tile {
                      2*nphase <= nfft); 0 = 2 when allowed.  Ignored by DFT. */
 } shwfs_batch_tile;
 /* frames[i].method is the initial method of WFS i (wg_size is ignored).
- * Refused (SHWFS_ERR_UNSUPPORTED): ntot != nfft, kernconv, layers, and the
+ * Refused (SHWFS_ERR_UNSUPPORTED): ntot != nfft, kernconv, LGS kernels, layers, and the
  * SHWFS_CENTROID_MLP kind, which run per WFS through shwfs_sh_comp_image. */
 int32_t shwfs_batch_create(const shwfs_sh_geom* geoms, const shwfs_frame_args* frames,
                            int32_t nwfs, shwfs_batch** out);

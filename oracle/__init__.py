@@ -21,6 +21,8 @@ loops for the small cases, the algorithm of the reference path:
 * ``mlp``       - float64 forward/backward of the per-subaperture MLP centroider
 * ``raytrace``  - guide-star raytrace through layers (libsutra/src.cpp/
                   sutra_source.cpp:222-267, libsutra/src.cu/sutra_target.cu:25-77)
+* ``lgs``       - LGS elongation (libsutra/src.cpp/sutra_lgs.cpp:160-212,
+                  libsutra/src.cu/sutra_lgs.cu)
 * ``corr``      - correlation centroider (libsutra/src.cpp/
                   sutra_centroider_corr.cpp:190-262) and the acquisim gather
                   (libsutra/src.cu/sutra_acquisim.cu:65-87)
@@ -62,6 +64,13 @@ PINNING = {
     "wfs.field_stop": "pinned: an all-pass stop returns the pupil field, a zero-frequency stop "
                       "returns the field's mean, brute-force DFT pair on a small grid "
                       "(tests/test_oracle_field_stop.py)",
+    "lgs.interp_transf / lgs.profiles": "pinned: exact on constant and linear profiles, clamps "
+                                        "at both ends (tests/test_oracle_lgs.py)",
+    "lgs.prof2d": "pinned: equals the circular convolution with the beam summed from its "
+                  "definition; a Dirac beam returns the profile",
+    "lgs.kernels": "pinned: a zero azimuth is the identity up to the clamp of the last two "
+                   "samples; the transform is numpy's fft2",
+    "lgs.convolve": "pinned: a Dirac kernel leaves the images unchanged (times n^2)",
     "geometry.field_stop_mask": "pinned: centred and symmetric under both flips and the "
                                 "transpose, centre pixel kept; equals the product's mask",
 }

@@ -130,20 +130,25 @@ def binimg_from_cube(bincube, validsubsx, validsubsy, nsub, base=None):
 
 def comp_image(phase, pupil, scale, offsets, istart, jstart, validsubsx, validsubsy,
                binmap, fluxPerSub, nphot, pdiam, Nfft, npix, nxsub, Ntot=None, hrmap=None,
-               ftkernel=None, submask=None):
+               ftkernel=None, submask=None, ftlgskern=None):
     """Noise-free SH image: returns (bincube [px,py,s], intensities_raw, binimg [x,y]).
 
     ``intensities_raw`` is what the reference leaves in d_intensities: the
     per-subap sum *before* normalisation.  Ntot / hrmap: the enlarged grid
     (binmap then indexes it); ftkernel: the kernel convolution (kernconv);
-    submask: the field stop.
+    submask: the field stop; ftlgskern [Ntot, Ntot, nvalid]: the LGS kernels
+    (with Ntot == Nfft the kernel convolution is then skipped, as the
+    reference's comp_generic does).
     """
     field = None if submask is None else field_stop(phase, pupil, scale, submask)
     E = fields(phase, pupil, scale, offsets, istart, jstart, pdiam, Nfft, field)
     hr = hr_images(E)
     if Ntot is not None and Ntot != Nfft:
         hr = extend_grid(hr, hrmap, Ntot)
-    if ftkernel is not None:
+    if ftlgskern is not None:
+        from oracle import lgs
+        hr = lgs.convolve(hr, ftlgskern)
+    if ftkernel is not None and (ftlgskern is None or (Ntot is not None and Ntot != Nfft)):
         hr = convolve_kernel(hr, ftkernel)
     b = bin_images(hr, binmap, npix)
     bn, inten = normalise(b, fluxPerSub, nphot)

@@ -67,13 +67,14 @@ def test_integer_shift_is_an_exact_copy():
     assert np.array_equal(got, L[7:75, 12:80])
 
 
-def _atmos(s, seeds=(5, 6)):
+def _atmos(s, seeds=(5, 6), altitude=None):
     n = s.p_geom._n
     dims = (n + 100, n + 160)
     carma, sutra = s.carma, s.sutra
     n = len(seeds)
+    alt = np.zeros(n, np.float32) if altitude is None else np.asarray(altitude, np.float32)
     at = sutra.Atmos(s.ctx, n, 0.16, np.full(n, 10.0, np.float32), np.array(dims, np.int64),
-                     np.full(n, 4, np.int64), np.zeros(n, np.float32), np.zeros(n, np.float32),
+                     np.full(n, 4, np.int64), alt, np.zeros(n, np.float32),
                      np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), 0)
     screens = []
     for k, (sd, d) in enumerate(zip(seeds, dims)):
@@ -113,3 +114,27 @@ def test_traced_field_stage_equals_rendered_screen(name):
     rendered = (s.bincube(), s.binimg(), s.slopes())
     for a, b in zip(traced, rendered):
         assert np.array_equal(a, b)
+
+
+def test_lgs_source_compresses_the_layers():
+    """An LGS WFS's source sees each layer through the cone: delta = 1 -
+    altitude / hg (sutra_source.cpp:250-252), hg the sodium layer's centre of
+    gravity set by lgs_init."""
+    s = system.ShSystem(configs.lgs())
+    alts = (5000.0, 12000.0)
+    at, screens = _atmos(s, altitude=alts)
+    gs = s.wfs.d_gs
+    hg = gs.d_lgs.hg
+    assert gs.lgs and abs(hg - 90000.0) < 1.0
+    offs = [(13.4, 27.8), (50.25, 4.5)]
+    for k, (xo, yo) in enumerate(offs):
+        gs.add_layer("atmos", k, xo, yo)
+    gs.lazy_raytrace = False
+    gs.raytrace(at)
+    n = s.p_geom._n
+    ref = ort.raytrace([dict(screen=screens[k], xoff=o[0], yoff=o[1],
+                             delta=float(np.float32(1.0 - np.float32(alts[k]) / np.float32(hg))))
+                        for k, o in enumerate(offs)], n, n)
+    err = relative_array_error(np.array(gs.d_phase), ref)
+    print("lgs raytrace (hg %.1f): rel err %.2e" % (hg, err), flush=True)
+    assert err < 1e-5
