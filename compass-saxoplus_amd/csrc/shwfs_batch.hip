@@ -34,6 +34,7 @@ struct shwfs_batch {
   // stage groups: the tasks fall into three stage kinds for which a group
   // kernel is compiled (group_kernel >= 0, shwfs_batch_groups.hip)
   int group_kernel = -1;
+  int group_wpe = 4;  // waves per SIMD of the noise-free group kernel (3 or 4, autotuned)
   GroupArgs gargs{};
   size_t lds = 0;  // bytes of LDS per wave
   int waves = 1;   // waves per workgroup of the mixed kernel (kBatchWaves when the slots fit)
@@ -361,8 +362,9 @@ int32_t launch_frame(shwfs_batch* b, uint32_t frame_off, hipStream_t st) {
                             b->lds,
                             b->d_frame, frame_off, b->remap, st);
   if (b->group_kernel >= 0)
-    return launch_batch_groups(b->group_kernel, b->noisy, b->starts, b->args, b->ntasks,
-                               b->total_tiles, b->gargs, b->lds, b->d_frame, frame_off, st);
+    return launch_batch_groups(b->group_kernel, b->noisy, b->group_wpe, b->starts, b->args,
+                               b->ntasks, b->total_tiles, b->gargs, b->lds, b->d_frame, frame_off,
+                               st);
   if (b->noisy)
     return launch_batch_noisy(b->kind_set, b->d_tasks, b->starts, b->ntasks, b->total_tiles, grid,
                               b->waves, b->lds, b->d_frame, frame_off, b->remap, st);
@@ -386,9 +388,33 @@ int resident_blocks(shwfs_batch* b) {
 
 // Time per frame (us) of the current configuration: median of 3 runs of
 // `reps` back-to-back frames.
+int32_t time_launches(shwfs_batch* b, int reps, hipStream_t st, float* us);
+
+// Median time per frame of the current configuration; for a noise-free
+// group kernel, of its faster register budget (kept in b->group_wpe).
 int32_t time_config(shwfs_batch* b, int reps, hipStream_t st, float* us) {
   int32_t rc = build_tasks(b, st);
   if (rc) return rc;
+  if (b->group_kernel < 0 || b->noisy) return time_launches(b, reps, st, us);
+  float best = 0.f;
+  int best_wpe = 4;
+  for (int wpe : {4, 3}) {
+    b->group_wpe = wpe;
+    float t = 0.f;
+    rc = time_launches(b, reps, st, &t);
+    if (rc) return rc;
+    if (wpe == 4 || t < best) {
+      best = t;
+      best_wpe = wpe;
+    }
+  }
+  b->group_wpe = best_wpe;
+  *us = best;
+  return SHWFS_OK;
+}
+
+int32_t time_launches(shwfs_batch* b, int reps, hipStream_t st, float* us) {
+  int32_t rc = SHWFS_OK;
   hipEvent_t e0, e1;
   SHWFS_HIP(hipEventCreate(&e0));
   SHWFS_HIP(hipEventCreate(&e1));
@@ -668,7 +694,10 @@ int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps,
   for (int gi = 0; gi < 3; ++gi)
     for (int rm = 0; rm < 2; ++rm) {
       if (grids[gi] == b->grid && rm == b->remap) continue;
-      if (b->uni_method >= 0 && grids[gi] != 0) continue;  // same-kind kernel: no persistent grid
+      // the same-kind and group kernels take one tile per wave: no persistent
+      // grid, and the group kernels no XCD remapping either
+      if (b->uni_method >= 0 && grids[gi] != 0) continue;
+      if (b->group_kernel >= 0 && (grids[gi] != 0 || rm != 0)) continue;
       const int sg = b->grid, sr = b->remap;
       b->grid = grids[gi];
       b->remap = rm;
@@ -683,10 +712,13 @@ int32_t shwfs_batch_autotune(shwfs_batch* b, uint32_t method_mask, int32_t reps,
     }
   b->grid = best_grid;
   b->remap = best_remap;
-  rc = build_tasks(b, st);
+  // the kept configuration, timed once more: this also settles the register
+  // budget of a group kernel (time_config keeps the faster one)
+  float final_us = 0.f;
+  rc = time_config(b, reps, st, &final_us);
   if (rc) return rc;
   SHWFS_HIP(hipStreamSynchronize(st));
-  if (best_us) *best_us = best;
+  if (best_us) *best_us = std::min(best, final_us);
   return SHWFS_OK;
 }
 

@@ -180,9 +180,10 @@ struct GroupArgs {
 int group_kernel_index(const int codes[3]);
 const int* group_kernel_codes(int index);
 // lds: bytes of one wave's slot; one wave per tile, kDftWaves per workgroup.
-int32_t launch_batch_groups(int index, bool noisy, const TaskStarts& starts, const TaskArgs& args,
-                            int ntasks, int total, const GroupArgs& ga, size_t lds,
-                            const uint32_t* frame_base, uint32_t frame_off, hipStream_t st);
+int32_t launch_batch_groups(int index, bool noisy, int wpe, const TaskStarts& starts,
+                            const TaskArgs& args, int ntasks, int total, const GroupArgs& ga,
+                            size_t lds, const uint32_t* frame_base, uint32_t frame_off,
+                            hipStream_t st);
 
 // Launchers of the instantiations (noise-free / noisy in translation units of
 // their own); `set` is a BatchKindSet.

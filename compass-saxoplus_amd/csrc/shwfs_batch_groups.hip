@@ -53,8 +53,12 @@ __device__ __forceinline__ void run_stage(const ImageParams& P, int s0, int lane
   }
 }
 
-template <class S0, class S1, class S2, bool NOISE>
-__global__ __launch_bounds__(64 * kDftWaves) __attribute__((amdgpu_waves_per_eu(4)))
+// WPE: the register budget as resident waves per SIMD.  At 4 (128 VGPRs)
+// the noise-free kernels spill 180-268 bytes per lane to scratch; at 3 (168)
+// they spill nothing.  Which is faster depends on the stage mix, so both are
+// compiled for the noise-free frames and the autotuner picks.
+template <class S0, class S1, class S2, bool NOISE, int WPE>
+__global__ __launch_bounds__(64 * kDftWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
 void sh_batch_groups_kernel(const TaskStarts starts, const TaskArgs args, int ntasks, int total,
                             GroupArgs ga, int slot, const uint32_t* __restrict__ frame_base,
                             uint32_t frame_off) {
@@ -77,18 +81,24 @@ void sh_batch_groups_kernel(const TaskStarts starts, const TaskArgs args, int nt
 }
 
 template <class S0, class S1, class S2>
-static int32_t launch_groups(bool noisy, const TaskStarts& starts, const TaskArgs& args,
+static int32_t launch_groups(bool noisy, int wpe, const TaskStarts& starts, const TaskArgs& args,
                              int ntasks, int total, const GroupArgs& ga, size_t lds,
                              const uint32_t* frame_base, uint32_t frame_off, hipStream_t st) {
   if (total <= 0) return SHWFS_OK;
   const int slot = (int)(lds / sizeof(float));
   const dim3 grid((total + kDftWaves - 1) / kDftWaves), block(64 * kDftWaves);
-  if (noisy)
-    hipLaunchKernelGGL((sh_batch_groups_kernel<S0, S1, S2, true>), grid, block, lds * kDftWaves,
-                       st, starts, args, ntasks, total, ga, slot, frame_base, frame_off);
+  if (noisy)  // the noisy kernels keep the 4-wave budget
+    hipLaunchKernelGGL((sh_batch_groups_kernel<S0, S1, S2, true, 4>), grid, block,
+                       lds * kDftWaves, st, starts, args, ntasks, total, ga, slot, frame_base,
+                       frame_off);
+  else if (wpe == 3)
+    hipLaunchKernelGGL((sh_batch_groups_kernel<S0, S1, S2, false, 3>), grid, block,
+                       lds * kDftWaves, st, starts, args, ntasks, total, ga, slot, frame_base,
+                       frame_off);
   else
-    hipLaunchKernelGGL((sh_batch_groups_kernel<S0, S1, S2, false>), grid, block, lds * kDftWaves,
-                       st, starts, args, ntasks, total, ga, slot, frame_base, frame_off);
+    hipLaunchKernelGGL((sh_batch_groups_kernel<S0, S1, S2, false, 4>), grid, block,
+                       lds * kDftWaves, st, starts, args, ntasks, total, ga, slot, frame_base,
+                       frame_off);
   SHWFS_LAUNCHED("sh_batch_groups_kernel");
   return SHWFS_OK;
 }
@@ -108,8 +118,8 @@ using Fp16a = GroupStage<kFp, 16, 1, false, 2>;
 using Fp32a = GroupStage<kFp, 32, 1, true, kHalf4>;
 using Fp64b = GroupStage<kFp, 64, 2, true, kHalf4>;
 
-using Launcher = int32_t (*)(bool, const TaskStarts&, const TaskArgs&, int, int, const GroupArgs&,
-                             size_t, const uint32_t*, uint32_t, hipStream_t);
+using Launcher = int32_t (*)(bool, int, const TaskStarts&, const TaskArgs&, int, int,
+                             const GroupArgs&, size_t, const uint32_t*, uint32_t, hipStream_t);
 struct GroupKernel {
   int code[3];
   Launcher launch;
@@ -139,14 +149,16 @@ int group_kernel_index(const int codes[3]) {
 
 const int* group_kernel_codes(int index) { return kGroupKernels[index].code; }
 
-int32_t launch_batch_groups(int index, bool noisy, const TaskStarts& starts, const TaskArgs& args,
-                            int ntasks, int total, const GroupArgs& ga, size_t lds,
-                            const uint32_t* frame_base, uint32_t frame_off, hipStream_t st) {
+int32_t launch_batch_groups(int index, bool noisy, int wpe, const TaskStarts& starts,
+                            const TaskArgs& args, int ntasks, int total, const GroupArgs& ga,
+                            size_t lds, const uint32_t* frame_base, uint32_t frame_off,
+                            hipStream_t st) {
   if (index < 0 || index >= (int)(sizeof(kGroupKernels) / sizeof(kGroupKernels[0])))
     return fail(SHWFS_ERR_ARG, "batch: no group kernel of that index");
+  if (wpe != 3 && wpe != 4) return fail(SHWFS_ERR_ARG, "batch: group kernels run 3 or 4 waves per SIMD");
   if (lds * kDftWaves > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "batch: LDS footprint exceeds the per-workgroup limit");
-  return kGroupKernels[index].launch(noisy, starts, args, ntasks, total, ga, lds, frame_base,
+  return kGroupKernels[index].launch(noisy, wpe, starts, args, ntasks, total, ga, lds, frame_base,
                                      frame_off, st);
 }
 
