@@ -201,6 +201,9 @@ bool setup_groups(shwfs_batch* b, std::vector<BatchTask>& tasks, std::vector<dou
       l = sizeof(float) * (size_t)f;
     }
     t.ntiles = (g.nvalid + t.per_tile - 1) / t.per_tile;
+    // the 64-point MFMA stage shares each sub-aperture among the waves of a
+    // workgroup: one tile per wave, a quarter sub-aperture each
+    if (b->tiles[i].method != SHWFS_METHOD_FFT_FP32 && g.nfft == 64) t.ntiles = g.nvalid * kDftWaves;
     cost[i] = tile_cost(t, g.nfft);
     wlds = std::max(wlds, l);
   }

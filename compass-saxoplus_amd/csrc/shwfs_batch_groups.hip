@@ -29,27 +29,36 @@ struct GroupStage {
 };
 
 // Sub-apertures per tile (one wave): 64 / N for the FFT; for the MFMA stages
-// kDftTaskSubaps (spots one after the other, one shared tail), or one for the
-// 64-point transform, whose sub-apertures are few and long: four in a row
-// would quadruple the latency of the frame's longest tiles.
+// kDftTaskSubaps (spots one after the other, one shared tail).  The 64-point
+// MFMA stage's sub-apertures are few and long (the frame's longest tiles):
+// each takes a whole workgroup, one kx quarter per wave (dft_subap_parts),
+// so its tiles are quarter sub-apertures and the stage must be the first
+// group (workgroups then never straddle two stages).
 template <class S>
 __host__ __device__ constexpr int stage_subaps() {
   return S::PREC < 0 ? 64 / S::N : dft_stage_subaps(S::N);
 }
+template <class S>
+__host__ __device__ constexpr bool stage_split() {
+  return S::PREC >= 0 && S::N == 64;
+}
 
+// tt: the tile within its task
 template <class S, bool NOISE>
-__device__ __forceinline__ void run_stage(const ImageParams& P, int s0, int lane, float* smem,
-                                          const void* wtab, unsigned call) {
+__device__ __forceinline__ void run_stage(const ImageParams& P, int tt, int lane, int wave,
+                                          float* smem, int slot, const void* wtab,
+                                          unsigned call) {
+  float* ws = smem + wave * slot;
   if constexpr (S::PREC < 0) {
-    fft_tile<S::N, S::QR, 64, NOISE>(P, s0, smem, P.binmap, call, lane);
-  } else if constexpr (dft_stage_subaps(S::N) == 1) {
+    fft_tile<S::N, S::QR, 64, NOISE>(P, tt * stage_subaps<S>(), ws, P.binmap, call, lane);
+  } else if constexpr (stage_split<S>()) {
     const AwOperand<S::N, S::QR, S::PREC> aw(static_cast<const Frag<S::PREC>*>(wtab), lane);
-    dft_subap<S::N, S::QR, S::PREC, NOISE, S::FULL, S::NR>(P, min(s0, P.nvalid - 1), lane, smem,
-                                                          P.binmap, S::NR & 7, aw, call);
+    dft_subap_parts<S::N, S::QR, S::PREC, NOISE, S::FULL, S::NR, kDftWaves>(
+        P, min(tt / kDftWaves, P.nvalid - 1), lane, tt % kDftWaves, smem, aw, call);
   } else {
     const AwOperand<S::N, S::QR, S::PREC> aw(static_cast<const Frag<S::PREC>*>(wtab), lane);
     dft_subaps<S::N, S::QR, S::PREC, NOISE, S::FULL, S::NR, kDftTaskSubaps>(
-        P, s0, lane, smem, P.binmap, S::NR & 7, aw, call);
+        P, tt * stage_subaps<S>(), lane, ws, P.binmap, S::NR & 7, aw, call);
   }
 }
 
@@ -69,15 +78,15 @@ void sh_batch_groups_kernel(const TaskStarts starts, const TaskArgs args, int nt
   // spare waves of the last workgroup recompute the last tile (same values)
   const int t = min((int)blockIdx.x * kDftWaves + wave, total - 1);
   const int i = find_task_args(starts, ntasks, t);
+  static_assert(!stage_split<S1>() && !stage_split<S2>(), "the split stage must be the first");
   const ImageParams& P = args.P[i];
   const int tt = t - starts.first[i];  // tile within its task
-  float* ws = smem + wave * slot;
   if (t < ga.first[1])
-    run_stage<S0, NOISE>(P, tt * stage_subaps<S0>(), lane, ws, ga.wtab[0], call);
+    run_stage<S0, NOISE>(P, tt, lane, wave, smem, slot, ga.wtab[0], call);
   else if (t < ga.first[2])
-    run_stage<S1, NOISE>(P, tt * stage_subaps<S1>(), lane, ws, ga.wtab[1], call);
+    run_stage<S1, NOISE>(P, tt, lane, wave, smem, slot, ga.wtab[1], call);
   else
-    run_stage<S2, NOISE>(P, tt * stage_subaps<S2>(), lane, ws, ga.wtab[2], call);
+    run_stage<S2, NOISE>(P, tt, lane, wave, smem, slot, ga.wtab[2], call);
 }
 
 template <class S0, class S1, class S2>

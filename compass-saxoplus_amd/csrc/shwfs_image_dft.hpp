@@ -402,12 +402,18 @@ __device__ __forceinline__ bool unit_window(const ImageParams& P, int s) {
 // compile time by NRC >= 0 (NRC & 7; bit kHalfGrid: the half-grid 4x4 form,
 // see dft_half_grid); FULL: pdiam == P16 (the window fills the operand
 // tiles; no masking).
-template <int N, int Q, int PREC, bool FULL, int NRC = -1>
+// PARTS > 1: the sub-aperture is shared by PARTS waves, wave `part` taking
+// kx blocks part, part + PARTS, ... (N / 16 blocks; register binning only):
+// each wave evaluates the whole field, computes stage A for its blocks' re
+// and im tiles only, and bins its blocks into the shared spot `bin`.
+template <int N, int Q, int PREC, bool FULL, int NRC = -1, int PARTS = 1>
 __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, float* hr,
                                          float* bin, const int* bm, const int nr_rt,
                                          const AwOperand<N, Q, PREC>& aw, const FieldIn<Q>& fin,
                                          const float (&of)[Q][FieldIn<Q>::NP], const unsigned win,
-                                         const bool unit, float& inten_scale) {
+                                         const bool unit, float& inten_scale, const int part = 0) {
+  static_assert(PARTS == 1 || (NRC >= 0 && (NRC & 7) > 0 && (N / 16) % PARTS == 0),
+                "dft_spot: a shared sub-aperture needs register binning and whole kx blocks");
   const int nr = NRC >= 0 ? (NRC & 7) : nr_rt;
   constexpr bool HALF = NRC >= 0 && (NRC & kHalfGrid) != 0;
   using L = DftLayout<N, Q>;
@@ -460,16 +466,33 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
       for (int e = 0; e < 8; ++e) ae[ma][ks].set(e, ev[ma][ks][e]);
 
   // ---- stage A: G (rows y, cols (re/im, kx)) = E A_W^T
+  constexpr int NTW = L::NTB / PARTS;  // kx blocks of this wave
   f32x4 ga[Q][L::MT];
+  if constexpr (PARTS == 1) {
 #pragma unroll
-  for (int ma = 0; ma < Q; ++ma)
+    for (int ma = 0; ma < Q; ++ma)
 #pragma unroll
-    for (int nt = 0; nt < L::MT; ++nt) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int nt = 0; nt < L::MT; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
-      ga[ma][nt] = acc;
-    }
+        for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
+        ga[ma][nt] = acc;
+      }
+  } else {
+    // the re and im tiles of blocks part + PARTS j, stored at j and H + j
+#pragma unroll
+    for (int ma = 0; ma < Q; ++ma)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+          const int nt = h * (L::MT / 2) + part + PARTS * j;
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int ks = 0; ks < L::KS; ++ks) acc = mma(ae[ma][ks], aw.get(nt, ks), acc);
+          ga[ma][h * (L::MT / 2) + j] = acc;
+        }
+  }
   // F carries 2^(e_total + t) (stage B's table scale too): |F|^2 is scaled
   // back by 2^(-2 (e_total + t)) (may be subnormal)
   inten_scale = ldexpf(1.0f, -2 * (e_total + t));
@@ -482,7 +505,8 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
   const int o = P.bin_origin, npix = P.npix;
   wave_lds_sync();
 #pragma unroll
-  for (int nt = 0; nt < L::NTB; ++nt) {
+  for (int nt = 0; nt < NTW; ++nt) {
+    const int ntk = PARTS == 1 ? nt : part + PARTS * nt;  // this tile's kx block
     F bg[L::KS];
 #pragma unroll
     for (int ks = 0; ks < L::KS; ++ks) {
@@ -500,7 +524,7 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(aw.get(mt, ks), bg[ks], acc);
       fb[mt] = acc;
     }
-    const int kx = 16 * nt + (lane & 15);
+    const int kx = 16 * ntk + (lane & 15);
     const int u = (kx - o) & (N - 1);
 #pragma unroll
     for (int mt = 0; mt < L::MT / 2; ++mt) {
@@ -534,9 +558,9 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
           const int a = (lane & 15) >> 2, c = lane >> 4;  // < 4
           int off;
           if constexpr (G >= 8)  // H >= 4: the XOR leaves a and c alone
-            off = (a + c * G) + (((4 * nt) ^ H) + ((4 * mt) ^ H) * G);
+            off = (a + c * G) + (((4 * ntk) ^ H) + ((4 * mt) ^ H) * G);
           else
-            off = ((4 * nt + a) ^ H) + ((4 * mt + c) ^ H) * G;
+            off = ((4 * ntk + a) ^ H) + ((4 * mt + c) ^ H) * G;
           if ((lane & 3) == 0) bin[off] = b;
         } else {
           const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
@@ -596,6 +620,37 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   void* red = nr > 0 ? smem + spot_f : smem;
   finish_spot<64, NOISE>(P, s, true, lane, bin, I, inten_scale, call_add, red, &pre);
   wave_lds_sync();
+}
+
+// Sub-aperture s shared by the PARTS waves of a workgroup (part = wave
+// index): each bins its kx blocks into the spot at the start of `shared`
+// (the first wave's LDS slot), then the first wave runs the tail.  All PARTS
+// waves must call it (workgroup barrier).
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC, int PARTS>
+__device__ __forceinline__ void dft_subap_parts(const ImageParams& P, const int s, const int lane,
+                                                const int part, float* shared,
+                                                const AwOperand<N, Q, PREC>& aw,
+                                                unsigned call_add = 0u) {
+  static_assert(NRC >= 0 && (NRC & 7) > 0, "dft_subap_parts: register binning only");
+  SubapScalars pre;
+  pre.load(P, s);
+  const int npix = P.npix;
+  const int spot_f = (npix * npix + 3) & ~3;
+  const bool unit = unit_window(P, s);
+  float of[Q][FieldIn<Q>::NP];
+  field_offsets<N, Q, FULL>(P, lane, of);
+  const unsigned win = window_corner(P, s);
+  FieldIn<Q> fin;
+  field_load<N, Q, FULL>(P, win, lane, fin);
+  float inten_scale;
+  dft_spot<N, Q, PREC, FULL, NRC, PARTS>(P, lane, shared, shared, nullptr, NRC & 7, aw, fin, of,
+                                         win, unit, inten_scale, part);
+  __syncthreads();  // every kx block binned
+  if (part == 0) {
+    const float I = spot_sum<64>(shared, npix * npix, true, lane);
+    finish_spot<64, NOISE>(P, s, true, lane, shared, I, inten_scale, call_add, shared + spot_f,
+                           &pre);
+  }
 }
 
 // LDS floats per wave of dft_subaps: the |F|^2 image (nr == 0), then SPW
