@@ -735,6 +735,13 @@ constexpr int dft_waves_per_eu() {
 // waves.
 constexpr int kDftWaves = 4;
 
+// The 64-point transform with register binning shares each sub-aperture
+// among the waves of a workgroup (dft_subap_parts): its sub-apertures are few
+// and long, one wave each would leave most of the chip idle.
+__host__ __device__ constexpr bool dft_split_subap(int n, int nr_code) {
+  return n == 64 && (nr_code & 7) > 0;
+}
+
 // One frame: wave w of workgroup g handles sub-aperture kDftWaves g' + w with
 // g' = xcd_remap(g) (workgroups of one XCD hold contiguous sub-apertures).
 // One sub-aperture per wave, with no loop: values derived from the lane index
@@ -759,8 +766,14 @@ void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
     __syncthreads();
   }
   const AwOperand<N, Q, PREC> aw(wtab, lane);
-  const int s = min(xcd_remap(blockIdx.x, gridDim.x) * kDftWaves + wave, P.nvalid - 1);
-  dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, nr, aw);
+  if constexpr (dft_split_subap(N, NR)) {
+    // the 64-point transform: one sub-aperture per workgroup, a kx quarter per wave
+    const int s = min(xcd_remap(blockIdx.x, gridDim.x), P.nvalid - 1);
+    dft_subap_parts<N, Q, PREC, NOISE, FULL, NR, kDftWaves>(P, s, lane, wave, smem, aw);
+  } else {
+    const int s = min(xcd_remap(blockIdx.x, gridDim.x) * kDftWaves + wave, P.nvalid - 1);
+    dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, nr, aw);
+  }
 }
 
 // Sub-apertures per wave of the same-kind batch kernel (its tiles).
@@ -855,8 +868,6 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wt
   const size_t lds = sizeof(float) * ((size_t)kDftWaves * L::wave_floats(p.npix, NR == 0) + nbm);
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "DFT spot kernel: LDS footprint exceeds the per-workgroup limit");
-  // grid: one single-wave workgroup per sub-aperture (the dispatcher balances
-  // them over the CUs as they drain)
   const bool noisy = p.noise > -1.0f;
   const bool full = p.pdiam == L::P16;
   auto kern = noisy ? (full ? sh_image_dft_kernel<N, Q, PREC, true, true, NR>
@@ -868,7 +879,8 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wt
                                    : wfrag_table<N, Q, PREC>(st);
   if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
   if (p.nvalid <= 0) return SHWFS_OK;
-  const int grid = (p.nvalid + kDftWaves - 1) / kDftWaves;  // one wave per sub-aperture
+  // one wave per sub-aperture, or one workgroup per 64-point sub-aperture
+  const int grid = dft_split_subap(N, NR) ? p.nvalid : (p.nvalid + kDftWaves - 1) / kDftWaves;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kDftWaves), lds, st, p, wtab);
   SHWFS_LAUNCHED("sh_image_dft_kernel");
   return SHWFS_OK;

@@ -148,3 +148,32 @@ def test_corr_arguments_rejected_without_launch(lib):
     assert call(sx=7, sy=9) == capi.SHWFS_ERR_ARG
     assert call(npix=2, size=8, sx=5, sy=3) == capi.SHWFS_ERR_ARG
     assert "centroids_corr" in capi.last_error(lib)
+
+
+def _header_struct_members(name):
+    """Member names of `typedef struct name { ... } name;` in include/shwfs.h."""
+    import os
+    import re
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "shwfs.h")).read()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        for part in decl.split(","):  # "float xoff, yoff"
+            names.append(re.findall(r"[A-Za-z_][A-Za-z_0-9]*", part)[-1])
+    return names
+
+
+@pytest.mark.parametrize("cname,pyname", [("shwfs_sh_geom", "ShGeom"),
+                                          ("shwfs_centroid_args", "CentroidArgs"),
+                                          ("shwfs_layer", "Layer"),
+                                          ("shwfs_frame_args", "FrameArgs")])
+def test_ctypes_structs_follow_the_header(cname, pyname):
+    """The ctypes mirrors list the header's members in order (a missing
+    trailing member would leave the library reading past the Python struct)."""
+    want = _header_struct_members(cname)
+    got = [f[0] for f in getattr(capi, pyname)._fields_]
+    assert got == want
