@@ -103,26 +103,36 @@ int32_t launch_image_fft_ray(const ImageParams& p, const RayParams& r, int nfft,
 }
 
 // ---- spots + learned centroider in one launch ------------------------------
-// 16 sub-apertures per workgroup (N lanes each, npix 8).  The tile's tail
-// leaves each spot's spot-cube values in its slot (fft_tile<..., KEEP>); the
-// 16 spots then go through one MLP forward on the matrix cores (weights
-// converted to bf16 as they are read) and each wave turns network outputs
-// into slopes, or takes the CoG of the same pixels where the spot flux is
-// outside the network's range.  Same arithmetic as shwfs_mlp_centroids over
-// the spot cube, so both give the same slopes.
-constexpr int kMlpSubaps = 16;
+// 256 / N sub-apertures per workgroup of four waves (N lanes each, npix 8):
+// 8 at N = 32, so the 1,256 sub-apertures of a 40x40 WFS spread over 157
+// workgroups (16 per workgroup left most CUs idle during the spot stage).
+// The tile's tail leaves each spot's spot-cube values in its slot
+// (fft_tile<..., KEEP>); the spots then go through one MLP forward on the
+// matrix cores in a 16-row tile (rows past the workgroup's spots are zero;
+// weights converted to bf16 as they are read) and each wave turns network
+// outputs into slopes, or takes the CoG of the same pixels where the spot
+// flux is outside the network's range.  Same arithmetic as
+// shwfs_mlp_centroids over the spot cube, so both give the same slopes.
+constexpr int kMlpThreads = 256;  // tile_forward computes with four waves
+constexpr int kMlpTile = 16;      // MFMA rows of the MLP tile
+
+template <int N>
+__host__ __device__ constexpr int mlp_subaps() {
+  return kMlpThreads / N;
+}
 
 template <int N, int R>
 __host__ __device__ constexpr int mlp_lds_floats_min() {
-  // spots (16 x 64 floats) + the MLP tile, laid over the dead slots
-  return kMlpSubaps * 64 + (int)(mlp::Lds<kMlpSubaps>::bytes() / sizeof(float));
+  // spots (S x 64 floats) + the MLP tile, laid over the dead slots
+  return mlp_subaps<N>() * 64 + (int)(mlp::Lds<kMlpTile>::bytes() / sizeof(float));
 }
 
 template <int N, int R, bool NOISE>
-__global__ __launch_bounds__(kMlpSubaps * N) void sh_image_fft_mlp_kernel(ImageParams P,
-                                                                          MlpParams M) {
+__global__ __launch_bounds__(kMlpThreads) void sh_image_fft_mlp_kernel(ImageParams P,
+                                                                       MlpParams M) {
   using L = FftLayout<N, R>;
-  constexpr int S = kMlpSubaps, TPB = kMlpSubaps * N, NW = TPB / 64;
+  constexpr int S = mlp_subaps<N>(), TPB = kMlpThreads, NW = TPB / 64;
+  static_assert(S <= kMlpTile && S * N == TPB, "fused MLP: one tile of spots per workgroup");
   extern __shared__ float smem[];
   const int slot_f = L::slot_floats(P.pdiam, 8);
   int* s_binmap = (int*)(smem + S * slot_f);
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(kMlpSubaps * N) void sh_image_fft_mlp_kernel(ImageP
   fft_tile<N, R, TPB, NOISE, true>(P, s0, smem, s_binmap);  // ends with a barrier
   const int tid = threadIdx.x;
   const int rows = P.nvalid - s0 < S ? P.nvalid - s0 : S;
-  // the 16 spots (pixel c of spot k at slot k's spot area) into registers,
+  // the S spots (pixel c of spot k at slot k's spot area) into registers,
   // then, past a barrier, to their places in the MLP tile
   constexpr int PER = S * 64 / TPB;
   float v[PER];
@@ -144,15 +154,19 @@ __global__ __launch_bounds__(kMlpSubaps * N) void sh_image_fft_mlp_kernel(ImageP
     v[e] = k < rows ? smem[k * slot_f + slot_f - 64 + c] : 0.0f;
   }
   __syncthreads();
-  float* spots = smem;  // [16][64]
-  const mlp::Lds<S> T(reinterpret_cast<char*>(smem + S * 64));
+  float* spots = smem;  // [S][64]
+  const mlp::Lds<kMlpTile> T(reinterpret_cast<char*>(smem + S * 64));
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int i = tid + e * TPB, k = i >> 6, c = i & 63;
     spots[i] = v[e];
     T.x[k * mlp::SX + c] = (__bf16)(v[e] * M.xs);
   }
-  mlp::tile_forward<S>(M.params, mlp::Fp32Weights{M.params}, T, tid);  // starts with a barrier
+  // tile rows without a spot of this workgroup: zeros
+  for (int i = S * 64 + tid; i < kMlpTile * 64; i += TPB)
+    T.x[(i >> 6) * mlp::SX + (i & 63)] = (__bf16)0.0f;
+  // starts with a barrier
+  mlp::tile_forward<kMlpTile, mlp::Fp32Weights, false>(M.params, mlp::Fp32Weights{M.params}, T, tid);
   const int lane = tid & 63, w = tid >> 6;
   for (int k = w; k < rows; k += NW)
     mlp::spot_slopes(spots[k * 64 + lane], lane, s0 + k, P.nvalid, T.y[k * mlp::NOUT],
@@ -163,7 +177,7 @@ __global__ __launch_bounds__(kMlpSubaps * N) void sh_image_fft_mlp_kernel(ImageP
 template <int N, int R>
 static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipStream_t st) {
   using L = FftLayout<N, R>;
-  constexpr int S = kMlpSubaps;
+  constexpr int S = mlp_subaps<N>();
   const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * 64;
   size_t floats = (size_t)S * L::slot_floats(p.pdiam, 8) + nbm;
   if (floats < (size_t)mlp_lds_floats_min<N, R>()) floats = mlp_lds_floats_min<N, R>();
@@ -172,9 +186,11 @@ static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipSt
     return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: LDS footprint exceeds the per-workgroup limit");
   const int grid = (p.nvalid + S - 1) / S;
   if (p.noise > -1.0f)
-    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, true>), dim3(grid), dim3(S * N), lds, st, p, m);
+    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, true>), dim3(grid), dim3(kMlpThreads), lds,
+                       st, p, m);
   else
-    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, false>), dim3(grid), dim3(S * N), lds, st, p, m);
+    hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, false>), dim3(grid), dim3(kMlpThreads), lds,
+                       st, p, m);
   SHWFS_LAUNCHED("sh_image_fft_mlp_kernel");
   return SHWFS_OK;
 }

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(TPB) void mlp_tile_kernel(const float* __restrict__
   const int rows = (int)((n - row0) < TS ? (n - row0) : TS);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   load_rows(x, index, row0, rows, x_scale, L);
-  mlp::tile_forward<TS>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
+  mlp::tile_forward<TS, Bf16Weights, TRAIN>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
   if constexpr (!TRAIN) {
     if (tid < rows * NOUT) y[(row0 + tid / NOUT) * NOUT + tid % NOUT] = L.y[tid];
     return;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(TPB) void mlp_centroid_kernel(
     const int sl = i / NIN, c = i % NIN;
     L.x[sl * SX + c] = (__bf16)(sl < rows ? pixel(row0 + sl, c) * x_scale : 0.0f);
   }
-  mlp::tile_forward<TS>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
+  mlp::tile_forward<TS, Bf16Weights, false>(P, Bf16Weights{W1b, W2b}, L, (int)threadIdx.x);
   // flux and CoG from the raw pixels: one wave per 8 sub-apertures
   for (int k = 0; k < TS / 4; ++k) {
     const int sl = w * (TS / 4) + k;

@@ -103,10 +103,13 @@ struct Fp32Weights {
 // Forward of the TS samples in L.x (bf16, scaled inputs) into LDS (z1, h1,
 // z2, h2, y).  Every thread of the workgroup calls it (it contains
 // barriers); waves 0-3 compute the products (wave w: output tiles 2w and
-// 2w+1 of each layer), the first TS * NOUT threads the last layer.  The
+// 2w+1 of each layer), four lanes per output the last layer.  The
 // value of a sample does not depend on TS or on the other samples of the
 // tile.  tid: the thread index.
-template <int TS, class WS>
+// KEEP_Z: z1 and z2 keep the pre-activations (the backward pass reads them);
+// otherwise z2 receives gelu(z2) in fp32, which the last layer then reads
+// without computing it again (the same values either way).
+template <int TS, class WS, bool KEEP_Z = true>
 __device__ __forceinline__ void tile_forward(const float* __restrict__ P, const WS& ws,
                                              const Lds<TS>& L, int tid) {
   const int lane = tid & 63, w = tid >> 6;
@@ -130,7 +133,7 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ P, const 
         for (int v = 0; v < 4; ++v) {
           const int row = 16 * mi + 4 * (lane >> 4) + v;
           const float z = acc[v] + P[OB1 + col];
-          L.z1[row * SZ + col] = z;
+          if constexpr (KEEP_Z) L.z1[row * SZ + col] = z;
           L.h1[row * SH + col] = (__bf16)gelu(z);
         }
       }
@@ -155,19 +158,34 @@ __device__ __forceinline__ void tile_forward(const float* __restrict__ P, const 
         for (int v = 0; v < 4; ++v) {
           const int row = 16 * mi + 4 * (lane >> 4) + v;
           const float z = acc[v] + P[OB2 + col];
-          L.z2[row * SZ + col] = z;
-          L.h2[row * SH + col] = (__bf16)gelu(z);
+          const float g = gelu(z);
+          L.z2[row * SZ + col] = KEEP_Z ? z : g;
+          L.h2[row * SH + col] = (__bf16)g;
         }
       }
   }
   __syncthreads();
-  // Y = gelu(Z2) W3^T + b3 (fp32): thread (s, o), the 128-term dot products
-  // read Z2 from LDS
-  if (tid < TS * NOUT) {
-    const int s = tid / NOUT, o = tid % NOUT;
-    float acc = P[OB3 + o];
-    for (int k = 0; k < NH; ++k) acc += gelu(L.z2[s * SZ + k]) * P[OW3 + o * NH + k];
-    L.y[s * NOUT + o] = acc;
+  // Y = gelu(Z2) W3^T + b3 (fp32): each 128-term dot product (s, o) is
+  // shared by PARTS consecutive lanes (terms k = part + PARTS j), whose
+  // partial sums meet in a fixed butterfly order; the bias is added last.
+  // PARTS does not depend on TS: a sample's value stays the same in every
+  // tile size.
+  constexpr int PARTS = 4;
+  static_assert(TS * NOUT * PARTS <= 256 && NH % PARTS == 0,
+                "tile_forward: four-wave workgroups, whole dot products per lane group");
+  if (tid < TS * NOUT * PARTS) {
+    const int pair = tid / PARTS, part = tid % PARTS;
+    const int s = pair / NOUT, o = pair % NOUT;
+    float acc = 0.0f;
+#pragma unroll 4
+    for (int j = 0; j < NH / PARTS; ++j) {
+      const int k = part + PARTS * j;
+      const float z = L.z2[s * SZ + k];
+      acc += (KEEP_Z ? gelu(z) : z) * P[OW3 + o * NH + k];
+    }
+#pragma unroll
+    for (int m = 1; m < PARTS; m <<= 1) acc += __shfl_xor(acc, m, PARTS);
+    if (part == 0) L.y[s * NOUT + o] = acc + P[OB3 + o];
   }
   __syncthreads();
 }
