@@ -270,6 +270,27 @@ def _spot_slopes_np(sp, kind, thr=0.0, weights=None):
     return np.stack([sx, sy], 1)
 
 
+def measure_noisy(graph_frames=20, reps=10):
+    """The noisy 40x40 workload (1,256 sub-apertures, pdiam 16, Nfft 32, 4x4
+    binning to 8x8 px, 1000 photons per sub-aperture with Poisson noise and
+    0.5 e- readout noise, detector background, WCoG with Gaussian weights of
+    FWHM 4 px): spots, noise, image and slopes in one launch per frame (plus
+    the background launch), graph-replayed, per method; every frame draws new
+    noise (the Philox frame counter advances)."""
+    import sutra
+    from shwfs import configs, system
+    out = {}
+    for m in (0, 1, 2):
+        s = system.ShSystem(configs.get("noisy"), method=m)
+        s.set_phase(system.make_screens(s.cfg, 1)[0])
+        slopes = s.rtc.d_control[0].d_centroids
+        ms = sutra.run_frames_graph(s.wfs, s.centro, slopes, graph_frames, reps, True)
+        us = 1e3 * ms / (graph_frames * reps)
+        out[METHOD_NAMES[m]] = dict(us_per_frame=round(us, 3), frames_per_s=round(1e6 / us, 1),
+                                    dtype=DTYPES[m])
+    return out
+
+
 def measure_learned(graph_frames=20, reps=10):
     """The learned centroider at 50 photons per sub-aperture + 3 e- readout
     (40x40 SH, 8x8-pixel spots): training on 2.0 M spot / true-gradient pairs
@@ -391,6 +412,7 @@ def main():
                          "form (0 = skip)")
     ap.add_argument("--no-uneven", action="store_true", help="skip the eight-WFS batch")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned centroider")
+    ap.add_argument("--no-noisy", action="store_true", help="skip the noisy 40x40 frame")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-centroider leg (for kernel profiles)")
     ap.add_argument("--uneven-only", action="store_true",
@@ -516,6 +538,12 @@ def main():
             print("[bench] uneven 8-WFS batch (%s): %.0f frames/s (%.2f us/frame), methods %s"
                   % (key, r["frames_per_s"], r["us_per_frame"], r["methods"]),
                   file=sys.stderr, flush=True)
+    noisy = None
+    if not a.no_noisy:
+        noisy = measure_noisy()
+        print("[bench] noisy 40x40 WCoG frame: %s" %
+              ", ".join("%s %.2f us" % (k, v["us_per_frame"]) for k, v in noisy.items()),
+              file=sys.stderr, flush=True)
     learned_cen = None
     if not a.no_learned:
         learned_cen = measure_learned()
@@ -580,6 +608,11 @@ def main():
                                    "one launch per frame, autotuned tiles, 1000 hipGraph frames; "
                                    "fp32_accurate: methods FFT/BF16x3, any_method: FP16 too",
                           **uneven) if uneven else None,
+        noisy_40x40=dict(workload="40x40 SH (1256 sub-apertures), pdiam 16, Nfft 32, 4x4 binning "
+                                  "to 8x8 px, 1000 ph/subap Poisson + 0.5 e- readout + detector "
+                                  "background, WCoG (Gaussian FWHM 4 px); one launch per frame "
+                                  "(plus the background), hipGraph replay, new noise every frame",
+                         methods=noisy) if noisy else None,
         learned_centroider=learned_cen,
     )
     print(json.dumps(line), flush=True)

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void prng_kernel(float* d, long n, char gtype,
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
     const unsigned e = (unsigned)i;
     if (gtype == 'P') {
-      d[i] = (float)rng_poisson((double)d[i], e, call, domain, seed);
+      d[i] = rng_poisson_f(d[i], e, call, domain, seed);
     } else if (gtype == 'N') {
       d[i] = beta * d[i] + alpha * (float)rng_normal(e, call, domain, seed);
     } else {
@@ -414,6 +414,8 @@ int32_t check_geom(const shwfs_sh_geom* g, int method) {
   if ((long long)g->nvalid * g->npix * g->npix >= (1LL << 30) || nsub * nsub >= (1LL << 30) ||
       (long long)g->npup * g->npup >= (1LL << 30))
     return fail(SHWFS_ERR_UNSUPPORTED, "geometry too large (spot cube, image or screen >= 2^30 elements)");
+  if (g->d_bgpix && (g->nbgpix < 0 || g->nbgpix > nsub * nsub))
+    return fail(SHWFS_ERR_ARG, "background pixel list: need 0 <= nbgpix <= (npix * nxsub)^2");
   if (!g->d_istart || !g->d_jstart || !g->d_validsubsx || !g->d_validsubsy || !g->d_offsets ||
       !g->d_fluxPerSub || !g->d_binmap)
     return fail(SHWFS_ERR_STATE, "geometry tables not loaded (load_arrays not called?)");
@@ -607,37 +609,57 @@ int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t
   return 0;
 }
 
-int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, void* stream) {
+// One frame of one WFS on stream st: the readout background of binimg, then
+// the spot kernel of the frame's method.  A noisy frame of the FFT or DFT
+// kernel whose geometry lists the pixels outside the spot windows draws the
+// background in the spot launch (extra workgroups on those pixels); any
+// other frame with a background launches the background kernel over the whole
+// image first, which the spots then overwrite.  wtab: the DFT operand table,
+// or NULL for the library's own.
+static int32_t comp_image_on(const shwfs_sh_geom* g, const shwfs_frame_args* f, hipStream_t st,
+                             const void* wtab) {
   ImageParams p;
   int32_t rc = image_params(g, f, &p);
   if (rc) return rc;
-  hipStream_t st = (hipStream_t)stream;
+  const bool ext = ext_needed(g);
+  const bool mlp = p.ckind == SHWFS_CENTROID_MLP;
+  BgParams bg;
   if (f->d_binimg && f->noise_background > 0.0f) {
-    const long n = (long)p.nsub * p.nsub;
-    rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
-    if (rc) return rc;
+    if (g->d_bgpix && p.noise > -1.0f && !ext && f->nlayers <= 0 && !mlp) {
+      bg.pix = g->d_bgpix;
+      bg.n = g->nbgpix;
+      bg.sigma = f->noise_background;
+    } else {
+      rc = launch_background(f->d_binimg, (long)p.nsub * p.nsub, f->noise_background, f->seed,
+                             f->call, st);
+      if (rc) return rc;
+    }
   }
-  if (ext_needed(g)) return launch_image_ext(p, ext_params(g), g->nfft, g->ntot, st);
+  if (ext) return launch_image_ext(p, ext_params(g), g->nfft, g->ntot, st);
   if (f->nlayers > 0) {
     RayParams r;
     rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
     if (rc) return rc;
     return launch_image_fft_ray(p, r, g->nfft, st);
   }
-  if (p.ckind == SHWFS_CENTROID_MLP) {
+  if (mlp) {
     const shwfs_centroid_args* c = f->centroid;
     const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
                       c->mlp_flux_lo, c->mlp_flux_hi};
     return launch_image_fft_mlp(p, m, g->nfft, st);
   }
-  if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
-  return launch_image_dft(p, g->nfft, f->method, st);
+  if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st, bg);
+  return launch_image_dft(p, g->nfft, f->method, st, wtab, bg);
+}
+
+int32_t shwfs_sh_comp_image(const shwfs_sh_geom* g, const shwfs_frame_args* f, void* stream) {
+  return comp_image_on(g, f, (hipStream_t)stream, nullptr);
 }
 
 int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* g, const shwfs_frame_args* f,
                                     void* stream) {
   ImageParams p;
-  int32_t rc = image_params(g, f, &p);
+  int32_t rc = image_params(g, f, &p);  // arguments checked before the table is written
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   const void* wtab = nullptr;
@@ -652,26 +674,7 @@ int32_t shwfs_sh_comp_image_noalloc(const shwfs_sh_geom* g, const shwfs_frame_ar
     if (rc) return rc;
     wtab = buf;
   }
-  if (f->d_binimg && f->noise_background > 0.0f) {
-    const long n = (long)p.nsub * p.nsub;
-    rc = launch_background(f->d_binimg, n, f->noise_background, f->seed, f->call, st);
-    if (rc) return rc;
-  }
-  if (ext_needed(g)) return launch_image_ext(p, ext_params(g), g->nfft, g->ntot, st);
-  if (f->nlayers > 0) {
-    RayParams r;
-    rc = ray_params(f->layers, f->nlayers, g->npup, g->npup, &r);
-    if (rc) return rc;
-    return launch_image_fft_ray(p, r, g->nfft, st);
-  }
-  if (p.ckind == SHWFS_CENTROID_MLP) {
-    const shwfs_centroid_args* c = f->centroid;
-    const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
-                      c->mlp_flux_lo, c->mlp_flux_hi};
-    return launch_image_fft_mlp(p, m, g->nfft, st);
-  }
-  if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st);
-  return launch_image_dft(p, g->nfft, f->method, st, wtab);
+  return comp_image_on(g, f, st, wtab);
 }
 
 int32_t shwfs_centroids(const float* d_img_in, float* d_img_out, const shwfs_centroid_args* a,

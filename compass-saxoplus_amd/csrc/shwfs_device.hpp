@@ -298,6 +298,138 @@ __device__ __forceinline__ double rng_poisson(double lam, uint32_t elem, uint32_
   return floor(lam);
 }
 
+// log(k!) for the integers 0 <= k < 10, as a select chain (no indexed table)
+__device__ __forceinline__ float log_factorial_small(float k) {
+  float r = 12.8018274800814696f;               // 9
+  r = (k < 8.5f) ? 10.6046029027452502f : r;  // 8
+  r = (k < 7.5f) ? 8.52516136106541430f : r;
+  r = (k < 6.5f) ? 6.57925121201010100f : r;
+  r = (k < 5.5f) ? 4.78749174278204599f : r;
+  r = (k < 4.5f) ? 3.17805383034794562f : r;
+  r = (k < 3.5f) ? 1.79175946922805500f : r;
+  r = (k < 2.5f) ? 0.693147180559945309f : r;
+  r = (k < 1.5f) ? 0.0f : r;
+  return r;
+}
+
+// Poisson(lam) of a float lam, always equal to rng_poisson((double)lam, ...):
+// the same counters, draws and decisions, taken in float32 and integers.
+//   * The PTRS uniforms are formed from the Philox words: with m = min(wu,
+//     2^32 - 1 - wu), rng_poisson's us = 0.5 - |U| is (m + 1/2) 2^-32
+//     exactly, so its tests us >= 0.07, us < 0.013 and V > us are the integer
+//     tests m >= 300647711, m <= 55834574 and wv > m.
+//   * Every other decision (u > F of the inversion, the floor of the PTRS
+//     candidate, V <= vr, the acceptance test) is checked against a bound on
+//     the float value's error: 2^-20 (k + 1) for the inversion's running sum,
+//     2^-18 or 2^-19 times the sum of the magnitudes of the terms elsewhere,
+//     8 to 16 times the rounding of every operation plus 1-ulp logf /
+//     log1pf / expf / sqrtf / reciprocals.  A lane whose float value falls
+//     within that bound of its threshold draws in float64 (rng_poisson), so
+//     both give one value for every counter.
+//   * The candidate floor(lam + y) (y = (2a/us + b) U + 0.43) is taken as
+//     floor(lam) + floor(frac(lam) + y), and the acceptance's right-hand side
+//     k log(lam) - lam - log(k!) as (k - lam) - k log1p((k - lam) / lam) -
+//     log(2 pi k) / 2 - 1/(12 k) + 1/(360 k^3) for k >= 10 (Stirling's
+//     series, truncation below 1e-8) and from log(k!) itself below: terms of
+//     size sqrt(lam), not lam log(lam), so the bounds stay narrow at high flux.
+// Both branches read block 0 of the element's stream, drawn once.
+__device__ __forceinline__ float rng_poisson_f(float lam, uint32_t elem, uint32_t call,
+                                               uint32_t domain, uint64_t seed) {
+  if (!(lam > 0.0f)) return 0.0f;
+  constexpr float k2m32 = 0x1p-32f;
+  u32x4 w = rng_block(elem, 0, call, domain, seed);
+  bool exact = false;
+  float k = 0.0f;
+  if (lam < 10.0f) {
+    const float u = ((float)w.x + 0.5f) * k2m32;
+    float p = expf(-lam);
+    float F = p;
+    int kk = 0;
+    while (kk < 200) {
+      if (fabsf(u - F) <= 0x1p-20f * (float)(kk + 1)) {
+        exact = true;
+        break;
+      }
+      if (!(u > F)) break;
+      kk += 1;
+      p = (p * lam) * __builtin_amdgcn_rcpf((float)kk);
+      F = F + p;
+    }
+    k = (float)kk;
+  } else if (lam < 1.0e6f) {
+    const float slam = sqrtf(lam);
+    const float b = 0.931f + 2.53f * slam;
+    const float a = -0.059f + 0.02483f * b;
+    const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
+    const float vr = 0.9277f - 3.6224f / (b - 2.0f);
+    const float loglam = logf(lam), linv = logf(invalpha);
+    const float rlam = __builtin_amdgcn_rcpf(lam);
+    const float L = floorf(lam), c0 = 0.43f + (lam - L);
+    bool done = false;
+    for (int t = 0; t < 4096; ++t) {
+      if (t > 0 && (t & 1) == 0) w = rng_block(elem, (uint32_t)(t >> 1), call, domain, seed);
+      const uint32_t wu = (t & 1) ? w.z : w.x;
+      const uint32_t wv = (t & 1) ? w.w : w.y;
+      const uint32_t m = min(wu, ~wu);
+      const float U = ((float)(int32_t)(wu ^ 0x80000000u) + 0.5f) * k2m32;
+      const float V = ((float)wv + 0.5f) * k2m32;
+      const float us = ((float)m + 0.5f) * k2m32;
+      const float rus = __builtin_amdgcn_rcpf(us);
+      const float qU = (2.0f * a * rus) * U, bU = b * U;
+      const float y = (qU + bU) + c0;
+      const float fy = floorf(y);
+      const float tk = 0x1p-18f * (fabsf(qU) + fabsf(bU) + 2.0f);
+      if (y - fy <= tk || fy + 1.0f - y <= tk) {
+        exact = true;
+        break;
+      }
+      const float kf = L + fy;
+      if (m >= 300647711u) {  // us >= 0.07
+        if (fabsf(V - vr) <= 0x1p-19f) {
+          exact = true;
+          break;
+        }
+        if (V <= vr) {
+          k = kf;
+          done = true;
+          break;
+        }
+      }
+      if (kf < 0.0f || (m <= 55834574u && wv > m)) continue;  // us < 0.013 && V > us
+      const float lV = logf(V), lY = logf(a * (rus * rus) + b);
+      const float lhs = lV + linv - lY;
+      float rhs, mag;
+      if (kf < 10.0f) {
+        const float lf = log_factorial_small(kf), kl = kf * loglam;
+        rhs = (kl - lam) - lf;
+        mag = fabsf(kl) + lam + lf;
+      } else {
+        const float d = kf - lam;
+        const float m1 = kf * log1pf(d * rlam);
+        const float h = 0.5f * logf(6.28318530717958648f * kf);
+        const float ik = __builtin_amdgcn_rcpf(kf);
+        rhs = (d - m1) - (h + ik * (1.0f / 12.0f - ik * ik * (1.0f / 360.0f)));
+        mag = fabsf(d) + fabsf(m1) + h + 1.0f;
+      }
+      const float ts = 0x1p-19f * (1.0f + fabsf(lV) + fabsf(linv) + fabsf(lY) + mag);
+      if (fabsf(lhs - rhs) <= ts) {
+        exact = true;
+        break;
+      }
+      if (lhs <= rhs) {
+        k = kf;
+        done = true;
+        break;
+      }
+    }
+    if (!exact && !done) k = L;
+  } else {
+    exact = true;
+  }
+  if (exact) return (float)rng_poisson((double)lam, elem, call, domain, seed);
+  return k;
+}
+
 // ---------------------------------------------------------------- reductions
 // Sum over groups of G consecutive lanes (G a power of two <= 64).
 template <int G, typename T>

@@ -3,7 +3,7 @@
 
 namespace shwfs {
 template int32_t launch_dft_pq<SHWFS_METHOD_DFT_BF16X3, 1>(const ImageParams&, int, hipStream_t,
-                                                   const void*);
+                                                   const void*, const BgParams&);
 template int32_t build_dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, void*, size_t, hipStream_t,
                                                         size_t*);
 template const void* dft_table_pq<SHWFS_METHOD_DFT_BF16X3, 1>(int, hipStream_t);

@@ -5,9 +5,10 @@
 namespace shwfs {
 
 template <int PREC>
-static int32_t launch_dft_p(const ImageParams& p, int nfft, hipStream_t st, const void* wtab) {
-  if (p.pdiam <= 16) return launch_dft_pq<PREC, 1>(p, nfft, st, wtab);
-  if (p.pdiam <= 32) return launch_dft_pq<PREC, 2>(p, nfft, st, wtab);
+static int32_t launch_dft_p(const ImageParams& p, int nfft, hipStream_t st, const void* wtab,
+                            const BgParams& bg) {
+  if (p.pdiam <= 16) return launch_dft_pq<PREC, 1>(p, nfft, st, wtab, bg);
+  if (p.pdiam <= 32) return launch_dft_pq<PREC, 2>(p, nfft, st, wtab, bg);
   return fail(SHWFS_ERR_UNSUPPORTED, "MFMA DFT path: needs nfft in {16, 32, 64} and pdiam <= min(32, nfft)");
 }
 
@@ -48,10 +49,13 @@ int32_t launch_dft_tasks(int method, int nfft, int pdiam, bool full, int nr, boo
 }
 
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t st,
-                         const void* wtab) {
-  if (method == SHWFS_METHOD_DFT_FP16) return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st, wtab);
+                         const void* wtab, const BgParams& bg) {
+  if (bg.n > 0 && !(p.noise > -1.0f))
+    return fail(SHWFS_ERR_ARG, "DFT spot kernel: the fused background needs a noisy frame");
+  if (method == SHWFS_METHOD_DFT_FP16)
+    return launch_dft_p<SHWFS_METHOD_DFT_FP16>(p, nfft, st, wtab, bg);
   if (method == SHWFS_METHOD_DFT_BF16X3)
-    return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st, wtab);
+    return launch_dft_p<SHWFS_METHOD_DFT_BF16X3>(p, nfft, st, wtab, bg);
   return fail(SHWFS_ERR_ARG, "unknown DFT method");
 }
 

@@ -721,11 +721,12 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
 // Register budgets (waves per SIMD), the largest at which the code still
 // fits without spilling: 7 (72 VGPRs) for the noise-free FP16 full-window
 // 4x4-binning variant, 5 (96 VGPRs) for the other noise-free variants, so a
-// frame of ~5000 sub-apertures is resident at once; 4 for the noise samplers
-// and the N = 64 tiles.
+// frame of ~5000 sub-apertures is resident at once; 4 for the N = 64 tiles;
+// 2 (240 VGPRs) with the noise samplers, which spill at 3 or more.
 template <int PREC, bool NOISE, int N, bool FULL, int NR>
 constexpr int dft_waves_per_eu() {
-  if (NOISE || N == 64) return 4;
+  if (NOISE) return 2;
+  if (N == 64) return 4;
   return (PREC == SHWFS_METHOD_DFT_FP16 && FULL && (NR & 7) == 4) ? 7 : 5;
 }
 
@@ -752,9 +753,16 @@ __host__ __device__ constexpr bool dft_split_subap(int n, int nr_code) {
 template <int N, int Q, int PREC, bool NOISE, bool FULL, int NR>
 __global__ __launch_bounds__(64 * kDftWaves)
 __attribute__((amdgpu_waves_per_eu(dft_waves_per_eu<PREC, NOISE, N, FULL, NR>())))
-void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
+void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab, BgParams bg) {
   constexpr int nr = NR & 7;
   using L = DftLayout<N, Q>;
+  // workgroups past the spot part draw the fused background (noisy frames)
+  if constexpr (NOISE) {
+    if ((int)blockIdx.x >= bg.spot_blocks) {
+      draw_background<64 * kDftWaves>(P, bg, (int)blockIdx.x - bg.spot_blocks);
+      return;
+    }
+  }
   extern __shared__ float smem[];
   const int lane = (int)threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -768,10 +776,10 @@ void sh_image_dft_kernel(ImageParams P, const Frag<PREC>* wtab) {
   const AwOperand<N, Q, PREC> aw(wtab, lane);
   if constexpr (dft_split_subap(N, NR)) {
     // the 64-point transform: one sub-aperture per workgroup, a kx quarter per wave
-    const int s = min(xcd_remap(blockIdx.x, gridDim.x), P.nvalid - 1);
+    const int s = min(xcd_remap(blockIdx.x, bg.spot_blocks), P.nvalid - 1);
     dft_subap_parts<N, Q, PREC, NOISE, FULL, NR, kDftWaves>(P, s, lane, wave, smem, aw);
   } else {
-    const int s = min(xcd_remap(blockIdx.x, gridDim.x) * kDftWaves + wave, P.nvalid - 1);
+    const int s = min(xcd_remap(blockIdx.x, bg.spot_blocks) * kDftWaves + wave, P.nvalid - 1);
     dft_subap<N, Q, PREC, NOISE, FULL, NR>(P, s, lane, smem + wave * WF, s_binmap, nr, aw);
   }
 }
@@ -862,7 +870,8 @@ inline bool dft_half_grid(const ImageParams& p, int nfft) {
 }
 
 template <int N, int Q, int PREC, int NR>
-static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wtab_in) {
+static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wtab_in,
+                            BgParams bg) {
   using L = DftLayout<N, Q>;
   const int nbm = (NR == 0 && !p.bin_block) ? p.nrebin * p.nrebin * p.npix * p.npix : 0;
   const size_t lds = sizeof(float) * ((size_t)kDftWaves * L::wave_floats(p.npix, NR == 0) + nbm);
@@ -879,22 +888,25 @@ static int32_t launch_dft_t(const ImageParams& p, hipStream_t st, const void* wt
                                    : wfrag_table<N, Q, PREC>(st);
   if (!wtab) return fail(SHWFS_ERR_HIP, "DFT spot kernel: building the DFT operand table failed");
   if (p.nvalid <= 0) return SHWFS_OK;
-  // one wave per sub-aperture, or one workgroup per 64-point sub-aperture
-  const int grid = dft_split_subap(N, NR) ? p.nvalid : (p.nvalid + kDftWaves - 1) / kDftWaves;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kDftWaves), lds, st, p, wtab);
+  // one wave per sub-aperture, or one workgroup per 64-point sub-aperture;
+  // then the workgroups of the fused background
+  bg.spot_blocks = dft_split_subap(N, NR) ? p.nvalid : (p.nvalid + kDftWaves - 1) / kDftWaves;
+  const int grid = bg.spot_blocks + (noisy ? (bg.n + 64 * kDftWaves - 1) / (64 * kDftWaves) : 0);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kDftWaves), lds, st, p, wtab, bg);
   SHWFS_LAUNCHED("sh_image_dft_kernel");
   return SHWFS_OK;
 }
 
 template <int N, int Q, int PREC>
-static int32_t launch_dft_n(const ImageParams& p, hipStream_t st, const void* wtab) {
+static int32_t launch_dft_n(const ImageParams& p, hipStream_t st, const void* wtab,
+                            const BgParams& bg) {
   switch (dft_register_rebin(p)) {
     case 4:
-      return dft_half_grid(p, N) ? launch_dft_t<N, Q, PREC, 4 | kHalfGrid>(p, st, wtab)
-                                 : launch_dft_t<N, Q, PREC, 4>(p, st, wtab);
-    case 2: return launch_dft_t<N, Q, PREC, 2>(p, st, wtab);
-    case 1: return launch_dft_t<N, Q, PREC, 1>(p, st, wtab);
-    default: return launch_dft_t<N, Q, PREC, 0>(p, st, wtab);
+      return dft_half_grid(p, N) ? launch_dft_t<N, Q, PREC, 4 | kHalfGrid>(p, st, wtab, bg)
+                                 : launch_dft_t<N, Q, PREC, 4>(p, st, wtab, bg);
+    case 2: return launch_dft_t<N, Q, PREC, 2>(p, st, wtab, bg);
+    case 1: return launch_dft_t<N, Q, PREC, 1>(p, st, wtab, bg);
+    default: return launch_dft_t<N, Q, PREC, 0>(p, st, wtab, bg);
   }
 }
 
@@ -903,18 +915,19 @@ static int32_t launch_dft_n(const ImageParams& p, hipStream_t st, const void* wt
 // NULL (the library's table of the shape, built on first use) or a table
 // the caller has built with build_dft_table_pq.
 template <int PREC, int Q>
-int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st, const void* wtab) {
+int32_t launch_dft_pq(const ImageParams& p, int nfft, hipStream_t st, const void* wtab,
+                      const BgParams& bg) {
   if constexpr (Q == 1) {
     switch (nfft) {
-      case 16: return launch_dft_n<16, 1, PREC>(p, st, wtab);
-      case 32: return launch_dft_n<32, 1, PREC>(p, st, wtab);
-      case 64: return launch_dft_n<64, 1, PREC>(p, st, wtab);
+      case 16: return launch_dft_n<16, 1, PREC>(p, st, wtab, bg);
+      case 32: return launch_dft_n<32, 1, PREC>(p, st, wtab, bg);
+      case 64: return launch_dft_n<64, 1, PREC>(p, st, wtab, bg);
       default: break;
     }
   } else {
     switch (nfft) {
-      case 32: return launch_dft_n<32, 2, PREC>(p, st, wtab);
-      case 64: return launch_dft_n<64, 2, PREC>(p, st, wtab);
+      case 32: return launch_dft_n<32, 2, PREC>(p, st, wtab, bg);
+      case 64: return launch_dft_n<64, 2, PREC>(p, st, wtab, bg);
       default: break;
     }
   }
@@ -1001,7 +1014,7 @@ int32_t build_dft_table_pq(int nfft, void* dst, size_t cap, hipStream_t st, size
 
 #define SHWFS_DFT_EXTERN(PREC, Q)                                                         \
   extern template int32_t launch_dft_pq<PREC, Q>(const ImageParams&, int, hipStream_t,  \
-                                                 const void*);                          \
+                                                 const void*, const BgParams&);         \
   extern template const void* dft_table_pq<PREC, Q>(int, hipStream_t);                \
   extern template int32_t build_dft_table_pq<PREC, Q>(int, void*, size_t, hipStream_t,  \
                                                       size_t*);                         \

@@ -5,52 +5,64 @@
 
 namespace shwfs {
 
+// Workgroups [0, bg.spot_blocks) compute spots; with noise, those past them
+// draw the fused background (bg).
 template <int N, int R, int TPB, bool NOISE>
-__global__ __launch_bounds__(TPB) void sh_image_fft_kernel(ImageParams P) {
+__global__ __launch_bounds__(TPB) void sh_image_fft_kernel(ImageParams P, BgParams bg) {
   using L = FftLayout<N, R>;
   constexpr int S = TPB / N;
+  if constexpr (NOISE) {
+    if ((int)blockIdx.x >= bg.spot_blocks) {
+      draw_background<TPB>(P, bg, (int)blockIdx.x - bg.spot_blocks);
+      return;
+    }
+  }
   extern __shared__ float smem[];
   int* s_binmap = (int*)(smem + S * L::slot_floats(P.pdiam, P.npix));
   if (!P.bin_block) {
     const int nbm = P.nrebin * P.nrebin * P.npix * P.npix;
     for (int i = threadIdx.x; i < nbm; i += TPB) s_binmap[i] = P.binmap[i];
   }
-  fft_tile<N, R, TPB, NOISE>(P, xcd_remap(blockIdx.x, gridDim.x) * S, smem, s_binmap);
+  fft_tile<N, R, TPB, NOISE>(P, xcd_remap(blockIdx.x, bg.spot_blocks) * S, smem, s_binmap);
 }
 
 template <int N, int R, int TPB>
-static int32_t launch_fft_nrt(const ImageParams& p, hipStream_t st) {
+static int32_t launch_fft_nrt(const ImageParams& p, hipStream_t st, BgParams bg) {
   using L = FftLayout<N, R>;
   constexpr int S = TPB / N;
   const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * p.npix * p.npix;
   const size_t lds = sizeof(float) * ((size_t)S * L::slot_floats(p.pdiam, p.npix) + nbm);
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "spot kernel: LDS footprint exceeds the per-workgroup limit");
-  const int grid = (p.nvalid + S - 1) / S;
-  if (p.noise > -1.0f)
-    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, true>), dim3(grid), dim3(TPB), lds, st, p);
+  bg.spot_blocks = (p.nvalid + S - 1) / S;
+  const bool noisy = p.noise > -1.0f;
+  const int grid = bg.spot_blocks + (noisy ? (bg.n + TPB - 1) / TPB : 0);
+  if (noisy)
+    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, true>), dim3(grid), dim3(TPB), lds, st, p, bg);
   else
-    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, false>), dim3(grid), dim3(TPB), lds, st, p);
+    hipLaunchKernelGGL((sh_image_fft_kernel<N, R, TPB, false>), dim3(grid), dim3(TPB), lds, st, p, bg);
   SHWFS_LAUNCHED("sh_image_fft_kernel");
   return SHWFS_OK;
 }
 
 template <int N, int R>
-static int32_t launch_fft_nr(const ImageParams& p, hipStream_t st) {
+static int32_t launch_fft_nr(const ImageParams& p, hipStream_t st, const BgParams& bg) {
   // workgroup size: 64 lanes by default (more, smaller workgroups spread the
   // sub-apertures evenly over the 256 CUs); 256 on request (tuning)
-  if (p.tpb == 256) return launch_fft_nrt<N, R, 256>(p, st);
-  if (p.tpb == 128 && N <= 128) return launch_fft_nrt<N, R, 128>(p, st);
-  return launch_fft_nrt<N, R, 64>(p, st);
+  if (p.tpb == 256) return launch_fft_nrt<N, R, 256>(p, st, bg);
+  if (p.tpb == 128 && N <= 128) return launch_fft_nrt<N, R, 128>(p, st, bg);
+  return launch_fft_nrt<N, R, 64>(p, st, bg);
 }
 
-int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t st) {
+int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t st, const BgParams& bg) {
+  if (bg.n > 0 && !(p.noise > -1.0f))
+    return fail(SHWFS_ERR_ARG, "spot kernel: the fused background needs a noisy frame");
   const bool half = 2 * p.pdiam <= nfft;
   switch (nfft) {
-    case 8: return half ? launch_fft_nr<8, 2>(p, st) : launch_fft_nr<8, 1>(p, st);
-    case 16: return half ? launch_fft_nr<16, 2>(p, st) : launch_fft_nr<16, 1>(p, st);
-    case 32: return half ? launch_fft_nr<32, 2>(p, st) : launch_fft_nr<32, 1>(p, st);
-    case 64: return half ? launch_fft_nr<64, 2>(p, st) : launch_fft_nr<64, 1>(p, st);
+    case 8: return half ? launch_fft_nr<8, 2>(p, st, bg) : launch_fft_nr<8, 1>(p, st, bg);
+    case 16: return half ? launch_fft_nr<16, 2>(p, st, bg) : launch_fft_nr<16, 1>(p, st, bg);
+    case 32: return half ? launch_fft_nr<32, 2>(p, st, bg) : launch_fft_nr<32, 1>(p, st, bg);
+    case 64: return half ? launch_fft_nr<64, 2>(p, st, bg) : launch_fft_nr<64, 1>(p, st, bg);
     default: return fail(SHWFS_ERR_UNSUPPORTED, "nfft must be 8, 16, 32 or 64");
   }
 }

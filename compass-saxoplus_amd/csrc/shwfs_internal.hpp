@@ -80,6 +80,19 @@ struct ImageParams {
   float* cintens;
 };
 
+// Readout background of the detector image drawn by extra workgroups of a
+// noisy FFT or DFT spot kernel, after its spot workgroups: binimg[pix[i]] =
+// sigma N(0, 1) (element = image index, Philox domain 2, the frame's call),
+// i < n.  The pixels lie outside every spot window, so the two parts of the
+// launch write disjoint pixels.  A kernel argument of its own (ImageParams is
+// also the per-task record of the batches).
+struct BgParams {
+  const int* pix = nullptr;
+  int n = 0;
+  float sigma = 0.0f;
+  int spot_blocks = 0;  // workgroups of the spot part (set by the launcher)
+};
+
 // Learned centroider run after binning by the FFT spot kernel (a kernel
 // argument of its own, not part of ImageParams, which batches pass by value).
 struct MlpParams {
@@ -160,7 +173,8 @@ __device__ __forceinline__ int find_task_args(const TaskStarts& ts, int ntasks, 
 int32_t check_geom(const shwfs_sh_geom* g, int method);
 int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImageParams* p);
 
-int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s);
+// bg: the fused background (noisy frames; bg.n == 0 for none).
+int32_t launch_image_fft(const ImageParams& p, int nfft, hipStream_t s, const BgParams& bg = {});
 // The enlarged HR grid (ntot != nfft) and the kernel convolution: one
 // workgroup per sub-aperture (shwfs_image_ext.hip).
 struct ExtParams {
@@ -189,7 +203,7 @@ int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft,
 // wtab: NULL (the library's operand table of the shape, built and cached on
 // first use) or a table built by build_dft_operand_table.
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s,
-                         const void* wtab = nullptr);
+                         const void* wtab = nullptr, const BgParams& bg = {});
 int32_t launch_mfma_probe(const float* A, const float* B, float* D, int method, hipStream_t s);
 // img = sigma * N(0,1) (Philox domain `domain`, element = image index); the
 // call is `call` plus *call_base when call_base is not NULL (device frame

@@ -207,6 +207,19 @@ __device__ __forceinline__ void store_at(float* a, unsigned i, float v) {
   *reinterpret_cast<float*>(reinterpret_cast<char*>(a) + (i << 2)) = v;
 }
 
+// Workgroup b (of TPB threads) of the fused background part of a spot
+// launch: one listed pixel per thread, the draw of the background kernel
+// (image index as the element, domain 2, the frame's call).
+template <int TPB>
+__device__ __forceinline__ void draw_background(const ImageParams& P, const BgParams& bg, int b) {
+  const int i = b * TPB + (int)threadIdx.x;
+  if (i < bg.n) {
+    const unsigned e = (unsigned)bg.pix[i];
+    if (e < (unsigned)P.nsub * (unsigned)P.nsub)
+      P.binimg[e] = bg.sigma * (float)rng_normal(e, P.call, 2u, P.seed);
+  }
+}
+
 // Row of pixel p in a spot of npix columns: p / npix through the float
 // reciprocal of npix (exact for p < 2^20: (p + 1/2) / npix stays at least
 // 1/(2 npix) away from an integer, far beyond the few-ulp product error).
@@ -234,7 +247,7 @@ __device__ __forceinline__ float tail_pixel(const ImageParams& P, unsigned e, fl
 #pragma clang fp contract(off)
   if constexpr (NOISE) {
     const unsigned call = P.call + call_add;
-    if (P.noise > -1.0f) b = (float)rng_poisson((double)b, e, call, 0u, P.seed);
+    if (P.noise > -1.0f) b = rng_poisson_f(b, e, call, 0u, P.seed);
     if (P.noise > 0.0f) b = 1.0f * b + P.noise * (float)rng_normal(e, call, 1u, P.seed);
   }
   store_at(P.bincube, e, b);

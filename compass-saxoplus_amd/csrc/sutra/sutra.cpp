@@ -557,6 +557,8 @@ SutraWfsSH::SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel,
       bin_block(0),
       bin_origin(0),
       d_pupil_unit(nullptr),
+      d_bgpix(nullptr),
+      nbgpix(0),
       noise_this_call(true) {}
 
 // reference: libsutra/src.cpp/sutra_wfs_sh.cpp:48-75 (vestigial MPI split;
@@ -627,6 +629,7 @@ int32_t SutraWfsSH::allocate_buffers(PlanMap& campli_plans, PlanMap& fttotim_pla
 SutraWfsSH::~SutraWfsSH() {
   current_context->set_active_device(device, 1);
   del(d_pupil_unit);
+  del(d_bgpix);
   del(d_ftkernel);
   del(d_bincube);
   del(d_binimg);
@@ -700,6 +703,24 @@ int32_t SutraWfsSH::load_arrays(int32_t* phasemap, int32_t* hrmap, int32_t* binm
   d_validpuppixx->host2device(istart);
   d_validpuppixy->host2device(jstart);
   d_ftkernel->host2device(kernel);
+  // the image pixels no spot window covers (reference: fill_binimage draws
+  // the background on every pixel, then fillbinimg overwrites the windows)
+  {
+    std::vector<uint8_t> covered((size_t)(nsub * nsub), 0);
+    for (int64_t s = 0; s < nvalid; ++s) {
+      const int64_t vx = validsubsx[offset + s], vy = validsubsy[offset + s];
+      for (int64_t y = vy; y < vy + npix; ++y)
+        for (int64_t x = vx; x < vx + npix; ++x) covered[(size_t)(x + y * nsub)] = 1;
+    }
+    std::vector<int32_t> bg;
+    for (int64_t i = 0; i < nsub * nsub; ++i)
+      if (!covered[(size_t)i]) bg.push_back((int32_t)i);
+    nbgpix = (int64_t)bg.size();
+    if (bg.empty()) bg.push_back(0);  // a valid pointer for an empty list
+    del(d_bgpix);
+    int64_t d[2] = {1, (int64_t)bg.size()};
+    d_bgpix = new CarmaObj<int32_t>(current_context, d, bg.data());
+  }
   int32_t origin = 0, rc = 0;
   if (ntot == nfft) {  // the block form is a property of the nfft grid
     rc = shwfs_binmap_block_origin(binmap, (int32_t)nfft, (int32_t)npix, (int32_t)nrebin, &origin);
@@ -752,6 +773,8 @@ shwfs_sh_geom SutraWfsSH::geom() const {
     SutraLGS* l = (lgs && d_gs) ? d_gs->d_lgs : nullptr;
     g.d_ftlgskern = (l && l->ready) ? reinterpret_cast<const float*>(l->d_ftlgskern->get_data())
                                     : nullptr;
+    g.d_bgpix = d_bgpix ? d_bgpix->get_data() : nullptr;
+    g.nbgpix = (int32_t)nbgpix;
   }
   return g;
 }

@@ -279,6 +279,10 @@ class SutraWfsSH : public SutraWfs {
   bool own_plan;  // campli_plan was cloned by set_method and belongs to this WFS
   int32_t bin_block, bin_origin;  // binmap analysis done at load_arrays
   CarmaObj<int32_t>* d_pupil_unit;  // [nvalid] 1 where the pupil window is all ones
+  // detector-image pixels outside every spot window (built by load_arrays):
+  // a noisy frame draws its readout background on them in the spot launch
+  CarmaObj<int32_t>* d_bgpix;
+  int64_t nbgpix;
 
   SutraWfsSH(CarmaContext* context, SutraTelescope* d_tel, CarmaObj<hipFloatComplex>* d_camplipup,
              CarmaObj<hipFloatComplex>* d_camplifoc, CarmaObj<hipFloatComplex>* d_fttotim,

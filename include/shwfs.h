@@ -84,6 +84,15 @@ typedef struct shwfs_sh_geom {
                                   The HR image is convolved with it before binning; with
                                   ntot != nfft and kernconv, then with ftkernel as well (with
                                   ntot == nfft, kernconv is not applied, as the reference) */
+  /* Detector-image pixels outside every spot window (index x + y * npix * nxsub),
+   * or NULL.  The reference draws the readout background on the whole binimg and
+   * then overwrites the spot windows (SutraWfsSH::fill_binimage, sutra_wfs_sh.cpp:
+   * 599-621); with this list a noisy frame of the FFT or DFT spot kernel draws it
+   * on these pixels only, by extra workgroups of the spot launch (the same draws,
+   * element = image index, Philox domain 2).  NULL: a launch of its own draws it
+   * on every pixel ahead of the spots. */
+  const int32_t* d_bgpix;      /* [nbgpix], each index < (npix * nxsub)^2 */
+  int32_t nbgpix;
 } shwfs_sh_geom;
 
 /* A centroider applied to a detector image (or fused after the spot stage).

@@ -25,7 +25,8 @@ class ShGeom(C.Structure):
                 ("d_istart", vp), ("d_jstart", vp), ("d_validsubsx", vp), ("d_validsubsy", vp),
                 ("d_offsets", vp), ("d_fluxPerSub", vp), ("d_binmap", vp),
                 ("binmap_block", i32), ("bin_origin", i32), ("d_hrmap", vp),
-                ("d_ftkernel", vp), ("kernconv", i32), ("d_ftlgskern", vp)]
+                ("d_ftkernel", vp), ("kernconv", i32), ("d_ftlgskern", vp),
+                ("d_bgpix", vp), ("nbgpix", i32)]
 
 
 class CentroidArgs(C.Structure):

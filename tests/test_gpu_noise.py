@@ -34,15 +34,43 @@ def test_prng_streams_match_oracle():
     np.testing.assert_allclose(z.cpu().numpy(), zr, rtol=1e-5, atol=1e-6)
 
 
-def _noisy_system(nscreen=1):
+def test_poisson_float_decisions_equal_the_float64_sampler():
+    """The device draws Poisson variates with float32 decisions checked
+    against their rounding bounds (a float64 redraw where a decision is too
+    close to call), so every draw must equal the float64 sampler's: around
+    the inversion / PTRS switch at 10, at low and high flux, and above 1e6
+    (always the float64 draw)."""
+    import torch
+    lib = capi.load()
+    rng = np.random.default_rng(11)
+    lam = np.concatenate([
+        rng.uniform(0.0, 10.0, 4000), 10.0 + rng.uniform(-0.05, 0.05, 2000),
+        rng.uniform(10.0, 300.0, 4000), rng.uniform(300.0, 5000.0, 2000),
+        rng.uniform(5e4, 2e6, 400), [0.0, -1.0, 1e-30, 9.999999, 10.0, 1e6, 3e6],
+    ]).astype(np.float32)
+    d = torch.from_numpy(lam.copy()).cuda()
+    assert lib.shwfs_prng(capi.ptr(d), lam.size, b"P", 0.0, 0.0, 2024, 3, 0, None) == 0
+    got = d.cpu().numpy()
+    ref = op.poisson(lam.astype(np.float64), 0, 3, 0, 2024)
+    bad = np.nonzero(got != ref)[0]
+    print("Poisson draws differing from the float64 sampler: %d of %d" % (bad.size, lam.size),
+          flush=True)
+    assert bad.size == 0, list(zip(lam[bad[:5]], got[bad[:5]], ref[bad[:5]]))
+
+
+def _noisy_system(method=0):
     c = configs.get("noisy")
-    s = system.ShSystem(c)
+    s = system.ShSystem(c, method=method)
     s.set_phase(system.make_screens(c, 1)[0])
     return c, s
 
 
-def test_noisy_frame_matches_oracle_noise_on_device_spots():
-    c, s = _noisy_system()
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_noisy_frame_matches_oracle_noise_on_device_spots(method):
+    """Per method (the readout background drawn by extra workgroups of the
+    spot launch on the pixels outside the spot windows): the noisy spots, the
+    background draws, and the spot windows of the image equal to the spots."""
+    c, s = _noisy_system(method)
     w = s.wfs
     seed = 4321
     w.set_noise(0.5, seed)           # call counter 0
@@ -70,6 +98,9 @@ def test_noisy_frame_matches_oracle_noise_on_device_spots():
     bg = 0.5 * op.normal(idx, 0, 2, seed)
     np.testing.assert_allclose(img.ravel(order="F")[idx], bg, rtol=1e-5, atol=1e-6)
     assert npix2 * w.nvalid == got.size
+    cube = noisy.reshape(p.npix, p.npix, -1, order="F")
+    for k, (x, y) in enumerate(zip(p._validsubsx, p._validsubsy)):
+        np.testing.assert_array_equal(img[x:x + p.npix, y:y + p.npix], cube[:, :, k])
 
 
 def test_wcog_of_the_noisy_image_matches_oracle():
