@@ -415,6 +415,8 @@ def main():
     ap.add_argument("--no-noisy", action="store_true", help="skip the noisy 40x40 frame")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned-centroider leg (for kernel profiles)")
+    ap.add_argument("--noisy-only", action="store_true",
+                    help="run only the noisy 40x40 frame leg (for kernel profiles)")
     ap.add_argument("--uneven-only", action="store_true",
                     help="run only a short eight-WFS batch (for kernel profiles)")
     ap.add_argument("--uneven-fixed", type=str, default="",
@@ -449,6 +451,9 @@ def main():
 
     if a.learned_only:
         print(json.dumps(dict(learned_centroider=measure_learned())), flush=True)
+        return
+    if a.noisy_only:
+        print(json.dumps(dict(noisy_40x40=measure_noisy())), flush=True)
         return
     if a.uneven_only and a.uneven_fixed:
         out = [measure_uneven_fixed(int(m)) for m in a.uneven_fixed.split(",") if m.strip()]
