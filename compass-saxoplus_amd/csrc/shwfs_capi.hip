@@ -610,11 +610,11 @@ int32_t shwfs_binmap_block_origin(const int32_t* h_binmap, int32_t nfft, int32_t
 }
 
 // One frame of one WFS on stream st: the readout background of binimg, then
-// the spot kernel of the frame's method.  A noisy frame of the FFT or DFT
-// kernel whose geometry lists the pixels outside the spot windows draws the
-// background in the spot launch (extra workgroups on those pixels); any
-// other frame with a background launches the background kernel over the whole
-// image first, which the spots then overwrite.  wtab: the DFT operand table,
+// the spot kernel of the frame's method.  A noisy frame of the FFT, DFT or
+// fused learned-centroider kernel whose geometry lists the pixels outside the
+// spot windows draws the background in the spot launch (extra workgroups on
+// those pixels); any other frame with a background launches the background
+// kernel over the whole image first, which the spots then overwrite.  wtab: the DFT operand table,
 // or NULL for the library's own.
 static int32_t comp_image_on(const shwfs_sh_geom* g, const shwfs_frame_args* f, hipStream_t st,
                              const void* wtab) {
@@ -625,7 +625,7 @@ static int32_t comp_image_on(const shwfs_sh_geom* g, const shwfs_frame_args* f, 
   const bool mlp = p.ckind == SHWFS_CENTROID_MLP;
   BgParams bg;
   if (f->d_binimg && f->noise_background > 0.0f) {
-    if (g->d_bgpix && p.noise > -1.0f && !ext && f->nlayers <= 0 && !mlp) {
+    if (g->d_bgpix && p.noise > -1.0f && !ext && f->nlayers <= 0) {
       bg.pix = g->d_bgpix;
       bg.n = g->nbgpix;
       bg.sigma = f->noise_background;
@@ -646,7 +646,7 @@ static int32_t comp_image_on(const shwfs_sh_geom* g, const shwfs_frame_args* f, 
     const shwfs_centroid_args* c = f->centroid;
     const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
                       c->mlp_flux_lo, c->mlp_flux_hi};
-    return launch_image_fft_mlp(p, m, g->nfft, st);
+    return launch_image_fft_mlp(p, m, g->nfft, st, bg);
   }
   if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st, bg);
   return launch_image_dft(p, g->nfft, f->method, st, wtab, bg);

@@ -139,12 +139,19 @@ __host__ __device__ constexpr int mlp_lds_floats_min() {
   return mlp_subaps<N>() * 64 + (int)(mlp::Lds<kMlpTile>::bytes() / sizeof(float));
 }
 
+// Workgroups past bg.spot_blocks draw the fused background (noisy frames).
 template <int N, int R, bool NOISE>
 __global__ __launch_bounds__(kMlpThreads) void sh_image_fft_mlp_kernel(ImageParams P,
-                                                                       MlpParams M) {
+                                                                       MlpParams M, BgParams bg) {
   using L = FftLayout<N, R>;
   constexpr int S = mlp_subaps<N>(), TPB = kMlpThreads, NW = TPB / 64;
   static_assert(S <= kMlpTile && S * N == TPB, "fused MLP: one tile of spots per workgroup");
+  if constexpr (NOISE) {
+    if ((int)blockIdx.x >= bg.spot_blocks) {
+      draw_background<TPB>(P, bg, (int)blockIdx.x - bg.spot_blocks);
+      return;
+    }
+  }
   extern __shared__ float smem[];
   const int slot_f = L::slot_floats(P.pdiam, 8);
   int* s_binmap = (int*)(smem + S * slot_f);
@@ -152,7 +159,7 @@ __global__ __launch_bounds__(kMlpThreads) void sh_image_fft_mlp_kernel(ImagePara
     const int nbm = P.nrebin * P.nrebin * 64;
     for (int i = threadIdx.x; i < nbm; i += TPB) s_binmap[i] = P.binmap[i];
   }
-  const int s0 = xcd_remap(blockIdx.x, gridDim.x) * S;
+  const int s0 = xcd_remap(blockIdx.x, bg.spot_blocks) * S;
   fft_tile<N, R, TPB, NOISE, true>(P, s0, smem, s_binmap);  // ends with a barrier
   const int tid = threadIdx.x;
   const int rows = P.nvalid - s0 < S ? P.nvalid - s0 : S;
@@ -187,7 +194,8 @@ __global__ __launch_bounds__(kMlpThreads) void sh_image_fft_mlp_kernel(ImagePara
 }
 
 template <int N, int R>
-static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipStream_t st) {
+static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipStream_t st,
+                                 BgParams bg) {
   using L = FftLayout<N, R>;
   constexpr int S = mlp_subaps<N>();
   const int nbm = p.bin_block ? 0 : p.nrebin * p.nrebin * 64;
@@ -196,24 +204,29 @@ static int32_t launch_fft_mlp_nr(const ImageParams& p, const MlpParams& m, hipSt
   const size_t lds = sizeof(float) * floats;
   if (lds > max_lds_per_block())
     return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: LDS footprint exceeds the per-workgroup limit");
-  const int grid = (p.nvalid + S - 1) / S;
-  if (p.noise > -1.0f)
+  bg.spot_blocks = (p.nvalid + S - 1) / S;
+  const bool noisy = p.noise > -1.0f;
+  const int grid = bg.spot_blocks + (noisy ? (bg.n + kMlpThreads - 1) / kMlpThreads : 0);
+  if (noisy)
     hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, true>), dim3(grid), dim3(kMlpThreads), lds,
-                       st, p, m);
+                       st, p, m, bg);
   else
     hipLaunchKernelGGL((sh_image_fft_mlp_kernel<N, R, false>), dim3(grid), dim3(kMlpThreads), lds,
-                       st, p, m);
+                       st, p, m, bg);
   SHWFS_LAUNCHED("sh_image_fft_mlp_kernel");
   return SHWFS_OK;
 }
 
-int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t st) {
+int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t st,
+                             const BgParams& bg) {
   if (p.npix != 8) return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: needs 8x8-pixel spots");
+  if (bg.n > 0 && !(p.noise > -1.0f))
+    return fail(SHWFS_ERR_ARG, "spot + MLP kernel: the fused background needs a noisy frame");
   if (p.nvalid <= 0) return SHWFS_OK;
   const bool half = 2 * p.pdiam <= nfft;
   switch (nfft) {
-    case 16: return half ? launch_fft_mlp_nr<16, 2>(p, m, st) : launch_fft_mlp_nr<16, 1>(p, m, st);
-    case 32: return half ? launch_fft_mlp_nr<32, 2>(p, m, st) : launch_fft_mlp_nr<32, 1>(p, m, st);
+    case 16: return half ? launch_fft_mlp_nr<16, 2>(p, m, st, bg) : launch_fft_mlp_nr<16, 1>(p, m, st, bg);
+    case 32: return half ? launch_fft_mlp_nr<32, 2>(p, m, st, bg) : launch_fft_mlp_nr<32, 1>(p, m, st, bg);
     default: return fail(SHWFS_ERR_UNSUPPORTED, "spot + MLP kernel: nfft must be 16 or 32");
   }
 }

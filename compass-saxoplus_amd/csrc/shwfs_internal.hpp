@@ -199,7 +199,8 @@ struct RayParams;
 int32_t launch_image_fft_ray(const ImageParams& p, const RayParams& r, int nfft, hipStream_t s);
 // The FFT spot kernel with the learned centroider after binning (p.ckind ==
 // SHWFS_CENTROID_MLP): 16 sub-apertures per workgroup, npix 8, nfft 16 or 32.
-int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t s);
+int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t s,
+                             const BgParams& bg = {});
 // wtab: NULL (the library's operand table of the shape, built and cached on
 // first use) or a table built by build_dft_operand_table.
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s,

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import capi
+from oracle import philox as op
 from shwfs import configs, learned, system
 
 pytestmark = pytest.mark.gpu
@@ -60,8 +61,23 @@ def test_fused_frame_equals_spots_then_network(lc):
     lo, hi = 20.0, 80.0  # some spots outside: their slopes are the CoG of the same pixels
     m.set_flux_range(lo, hi)
     slopes = _obj(lc, np.zeros(2 * nv))
+    seed = 777
+    s.wfs.set_noise(3.0, seed)                    # call counter 0
     lc.sutra.comp_frame(s.wfs, m, slopes, True)   # spots + network in one launch
     got = np.array(slopes).ravel()
+    # the readout background drawn by the same launch: 3 N(0, 1) (domain 2) on
+    # the pixels outside the spot windows, the spots inside them
+    p = s.p_wfs
+    img = s.binimg()
+    mask = np.zeros(img.shape, bool)
+    for x, y in zip(p._validsubsx, p._validsubsy):
+        mask[x:x + p.npix, y:y + p.npix] = True
+    idx = np.nonzero(~mask.ravel(order="F"))[0]
+    np.testing.assert_allclose(img.ravel(order="F")[idx], 3.0 * op.normal(idx, 0, 2, seed),
+                               rtol=1e-5, atol=1e-6)
+    cube = s.bincube().reshape(p.npix, p.npix, -1, order="F")
+    for k, (x, y) in enumerate(zip(p._validsubsx, p._validsubsy)):
+        np.testing.assert_array_equal(img[x:x + p.npix, y:y + p.npix], cube[:, :, k])
     got_i = np.array(m.d_intensities).ravel()
     got_f = np.array(m.d_fallback).ravel()
     print("fused frame: %d of %d sub-apertures fell back to CoG" % (got_f.sum(), nv), flush=True)
