@@ -394,6 +394,76 @@ __device__ __forceinline__ bool unit_window(const ImageParams& P, int s) {
 }
 
 
+// |F|^2 of one kx block (stage B's accumulator tiles fb of block ntk) into
+// LDS: binned in registers (nr > 0) into the spot bin, or the whole image hr
+// for the general binmap gather (nr == 0).  o: the binning origin.
+template <int N, int Q, int NRC>
+__device__ __forceinline__ void spot_bin_block(const f32x4 (&fb)[DftLayout<N, Q>::MT],
+                                               const int ntk, const int lane, const int nr,
+                                               const int o, const int npix, float* hr,
+                                               float* bin) {
+  using L = DftLayout<N, Q>;
+  constexpr bool HALF = NRC >= 0 && (NRC & kHalfGrid) != 0;
+  const int kx = 16 * ntk + (lane & 15);
+  const int u = (kx - o) & (N - 1);
+#pragma unroll
+  for (int mt = 0; mt < L::MT / 2; ++mt) {
+    float iv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
+      iv[v] = fmaf(fi, fi, fr * fr);  // one rounding order in every instantiation
+    }
+    const int ky0 = 16 * mt + 4 * (lane >> 4);  // this lane's 4 consecutive ky
+    if (nr == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) hr[kx + (ky0 + v) * N] = iv[v];
+    } else if (nr == 4) {
+      // the 4 ky of the lane x lanes kx .. kx+3 form one binned pixel
+      float b = fb[mt][0] * fb[mt][0];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (v > 0) b = fmaf(fb[mt][v], fb[mt][v], b);
+        b = fmaf(fb[mt + L::MT / 2][v], fb[mt + L::MT / 2][v], b);
+      }
+      b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
+      b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
+      // both sums with every lane active (two DPP adds): kept from being moved
+      // into the store's lane mask, where the second needs a separate DPP move
+      asm volatile("" : "+v"(b));
+      if constexpr (HALF) {
+        // the binned cell (cx, cy) = (kx / 4, ky / 4) is pixel (cx ^ H, cy ^ H):
+        // a per-lane base plus a constant offset per tile pair
+        constexpr int G = N / 4, H = G / 2;
+        const int a = (lane & 15) >> 2, c = lane >> 4;  // < 4
+        int off;
+        if constexpr (G >= 8)  // H >= 4: the XOR leaves a and c alone
+          off = (a + c * G) + (((4 * ntk) ^ H) + ((4 * mt) ^ H) * G);
+        else
+          off = ((4 * ntk + a) ^ H) + ((4 * mt + c) ^ H) * G;
+        if ((lane & 3) == 0) bin[off] = b;
+      } else {
+        const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
+        if ((lane & 3) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
+      }
+    } else if (nr == 2) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        float b = iv[2 * g] + iv[2 * g + 1];
+        b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
+        const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
+        if ((lane & 1) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
+      }
+    } else {  // nr == 1
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int px = u, py = (ky0 + v - o) & (N - 1);
+        if (px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = iv[v];
+      }
+    }
+  }
+}
+
 // Binned spot of sub-aperture s (< nvalid, uniform over the wave), before
 // the flux normalisation, into bin (npix^2 floats of LDS, x fastest), in units
 // of 1/inten_scale (returned through inten_scale).  hr: N^2 floats of LDS for
@@ -415,7 +485,6 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
   static_assert(PARTS == 1 || (NRC >= 0 && (NRC & 7) > 0 && (N / 16) % PARTS == 0),
                 "dft_spot: a shared sub-aperture needs register binning and whole kx blocks");
   const int nr = NRC >= 0 ? (NRC & 7) : nr_rt;
-  constexpr bool HALF = NRC >= 0 && (NRC & kHalfGrid) != 0;
   using L = DftLayout<N, Q>;
   using F = Frag<PREC>;
   const int pdiam = FULL ? L::P16 : P.pdiam;
@@ -524,68 +593,151 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
       for (int ks = 0; ks < L::KS; ++ks) acc = mma(aw.get(mt, ks), bg[ks], acc);
       fb[mt] = acc;
     }
-    const int kx = 16 * ntk + (lane & 15);
-    const int u = (kx - o) & (N - 1);
-#pragma unroll
-    for (int mt = 0; mt < L::MT / 2; ++mt) {
-      float iv[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float fr = fb[mt][v], fi = fb[mt + L::MT / 2][v];
-        iv[v] = fmaf(fi, fi, fr * fr);  // one rounding order in every instantiation
-      }
-      const int ky0 = 16 * mt + 4 * (lane >> 4);  // this lane's 4 consecutive ky
-      if (nr == 0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) hr[kx + (ky0 + v) * N] = iv[v];
-      } else if (nr == 4) {
-        // the 4 ky of the lane x lanes kx .. kx+3 form one binned pixel
-        float b = fb[mt][0] * fb[mt][0];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          if (v > 0) b = fmaf(fb[mt][v], fb[mt][v], b);
-          b = fmaf(fb[mt + L::MT / 2][v], fb[mt + L::MT / 2][v], b);
-        }
-        b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
-        b += __int_as_float(dpp_row_perm(__float_as_int(b), 1));  // lane ^ 2
-        // both sums with every lane active (two DPP adds): kept from being moved
-        // into the store's lane mask, where the second needs a separate DPP move
-        asm volatile("" : "+v"(b));
-        if constexpr (HALF) {
-          // the binned cell (cx, cy) = (kx / 4, ky / 4) is pixel (cx ^ H, cy ^ H):
-          // a per-lane base plus a constant offset per tile pair
-          constexpr int G = N / 4, H = G / 2;
-          const int a = (lane & 15) >> 2, c = lane >> 4;  // < 4
-          int off;
-          if constexpr (G >= 8)  // H >= 4: the XOR leaves a and c alone
-            off = (a + c * G) + (((4 * ntk) ^ H) + ((4 * mt) ^ H) * G);
-          else
-            off = ((4 * ntk + a) ^ H) + ((4 * mt + c) ^ H) * G;
-          if ((lane & 3) == 0) bin[off] = b;
-        } else {
-          const int px = u >> 2, py = ((ky0 - o) & (N - 1)) >> 2;
-          if ((lane & 3) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
-        }
-      } else if (nr == 2) {
-#pragma unroll
-        for (int g = 0; g < 2; ++g) {
-          float b = iv[2 * g] + iv[2 * g + 1];
-          b += __int_as_float(dpp_row_perm(__float_as_int(b), 0));  // lane ^ 1
-          const int px = u >> 1, py = ((ky0 + 2 * g - o) & (N - 1)) >> 1;
-          if ((lane & 1) == 0 && px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = b;
-        }
-      } else {  // nr == 1
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int px = u, py = (ky0 + v - o) & (N - 1);
-          if (px < npix && py < npix) bin[px + (int)__umul24(py, npix)] = iv[v];
-        }
-      }
-    }
+    spot_bin_block<N, Q, NRC>(fb, ntk, lane, nr, o, npix, hr, bin);
   }
   if (nr == 0) {
     wave_lds_sync();
     (void)bin_gather<64, N>(P, true, lane, hr, bin, bm);
+  }
+}
+
+// ---- two sub-apertures of an 8-point window per stage A (N = 16) ----------
+// Sub-apertures a and b with pdiam <= 8 on the 16-point split-BF16x3 stage
+// with register binning share one pass of stage A: the lanes of operand rows
+// y = lane & 15 below 8 carry a's window rows, rows 8-15 b's (rows y >= pdiam
+// are zero in the one-sub-aperture layout, as points x >= pdiam are).  Stage
+// A's accumulator tiles then hold a's rows in lanes 0-31 and b's in lanes
+// 32-63 (lane j + 16 g holds rows 4 g .. 4 g + 3), and one v_permlane32_swap
+// per register of stage B's operand gives a's operand (lanes 32-63 zero) and
+// b's (moved to lanes 0-31, lanes 32-63 zero): each that of dft_spot on the
+// sub-aperture alone, so both spots are the same bit for bit, for half the
+// field evaluations, stage A products and operand splits.
+template <int N, int Q, int PREC, bool FULL, int NRC>
+__host__ __device__ constexpr bool dft_pair_stage() {
+  return N == 16 && Q == 1 && PREC == SHWFS_METHOD_DFT_BF16X3 && !FULL && NRC >= 0 &&
+         (NRC & 7) > 0;
+}
+
+// Offsets (in turns) at the lane's points of its pair row (row = lane & 7).
+__device__ __forceinline__ void field_offsets_pair(const ImageParams& P, int lane,
+                                                   float (&of)[1][FieldIn<1>::NP]) {
+  const int pdiam = P.pdiam, row = lane & 7;
+#pragma unroll
+  for (int j = 0; j < FieldIn<1>::NP; ++j) {
+    const int x = field_px<1>(lane, j);
+    const bool in = x < pdiam && row < pdiam;
+    of[0][j] = P.offsets[(unsigned)(in ? x + row * pdiam : 0)];
+  }
+#pragma unroll
+  for (int j = 0; j < FieldIn<1>::NP; ++j) of[0][j] = -of[0][j] * 0.15915494309189533577f;
+}
+
+// Screen values at the lane's points: window a for rows y < 8, b above.
+__device__ __forceinline__ void field_load_pair(const ImageParams& P, unsigned win_a,
+                                                unsigned win_b, int lane, FieldIn<1>& f) {
+  const int pdiam = P.pdiam, row = lane & 7;
+  const unsigned npup = (unsigned)P.npup;
+  const unsigned win = (lane & 8) ? win_b : win_a;
+#pragma unroll
+  for (int j = 0; j < FieldIn<1>::NP; ++j) {
+    const int x = field_px<1>(lane, j);
+    const bool in = x < pdiam && row < pdiam;
+    const unsigned g = win + (in ? (unsigned)x + __umul24((unsigned)row, npup) : 0u);  // clamped
+    f.ph[0][j] =
+        *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.phase) + (g << 2));
+  }
+}
+
+// Amplitude at the lane's points of its pair row (field_amplitude's rule).
+__device__ __forceinline__ void field_amplitude_pair(const ImageParams& P, unsigned win_a,
+                                                     unsigned win_b, bool unit_a, bool unit_b,
+                                                     int lane, float (&pu)[1][FieldIn<1>::NP]) {
+  const int pdiam = P.pdiam, row = lane & 7;
+  const unsigned npup = (unsigned)P.npup;
+  const bool second = (lane & 8) != 0;
+  const unsigned win = second ? win_b : win_a;
+  const bool unit = second ? unit_b : unit_a;
+#pragma unroll
+  for (int j = 0; j < FieldIn<1>::NP; ++j) {
+    const int x = field_px<1>(lane, j);
+    const bool in = x < pdiam && row < pdiam;
+    const unsigned g = win + (in ? (unsigned)x + __umul24((unsigned)row, npup) : 0u);
+    float a = 1.0f;
+    if (!unit)
+      a = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(P.pupil) + (g << 2));
+    pu[0][j] = in ? a : 0.0f;
+  }
+}
+
+// Lanes 0-31 of f (lanes 32-63 zero) and lanes 32-63 of f moved to lanes
+// 0-31 (lanes 32-63 zero).  Every lane active.
+__device__ __forceinline__ void split_lane_halves(const Frag<SHWFS_METHOD_DFT_BF16X3>& f,
+                                                  Frag<SHWFS_METHOD_DFT_BF16X3>& lower,
+                                                  Frag<SHWFS_METHOD_DFT_BF16X3>& upper) {
+  typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
+  const u32x4v h = __builtin_bit_cast(u32x4v, f.hi), l = __builtin_bit_cast(u32x4v, f.lo);
+  u32x4v h0, h1, l0, l1;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    // swaps the old value's lanes 32-63 with the source's lanes 0-31
+    const auto rh = __builtin_amdgcn_permlane32_swap(h[d], 0u, false, false);
+    const auto rl = __builtin_amdgcn_permlane32_swap(l[d], 0u, false, false);
+    h0[d] = rh[0];
+    h1[d] = rh[1];
+    l0[d] = rl[0];
+    l1[d] = rl[1];
+  }
+  lower.hi = __builtin_bit_cast(bf16x8, h0);
+  lower.lo = __builtin_bit_cast(bf16x8, l0);
+  upper.hi = __builtin_bit_cast(bf16x8, h1);
+  upper.lo = __builtin_bit_cast(bf16x8, l1);
+}
+
+// The binned spots of sub-apertures a and b (see above) into bin_a, bin_b.
+template <int N, int NRC>
+__device__ __forceinline__ void dft_spot_pair(const ImageParams& P, const int lane, float* bin_a,
+                                              float* bin_b,
+                                              const AwOperand<N, 1, SHWFS_METHOD_DFT_BF16X3>& aw,
+                                              const FieldIn<1>& fin,
+                                              const float (&of)[1][FieldIn<1>::NP],
+                                              const unsigned win_a, const unsigned win_b,
+                                              const bool unit_a, const bool unit_b,
+                                              float& inten_scale) {
+  constexpr int PREC = SHWFS_METHOD_DFT_BF16X3;
+  using L = DftLayout<N, 1>;
+  using F = Frag<PREC>;
+  static_assert(L::MT == 2 && L::KS == 1 && L::NTB == 1, "dft_spot_pair: the 16-point stage");
+  float ev[1][1][8];
+  {
+    float pu[1][FieldIn<1>::NP];
+    field_amplitude_pair(P, win_a, win_b, unit_a, unit_b, lane, pu);
+    field_eval<N, 1, false>(P, lane, fin, of, pu, ev);
+  }
+  F ae;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) ae.set(e, ev[0][0][e]);
+  f32x4 ga[L::MT];
+#pragma unroll
+  for (int nt = 0; nt < L::MT; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    ga[nt] = mma(ae, aw.get(nt, 0), acc);
+  }
+  inten_scale = ldexpf(1.0f, -2 * (2 * table_exp<1, PREC>()));
+  const int o = P.bin_origin, npix = P.npix;
+  const int nr = NRC & 7;
+  wave_lds_sync();
+  F both = acc_to_operand<PREC>(ga[0], ga[1]);
+  F op[2];
+  split_lane_halves(both, op[0], op[1]);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    f32x4 fb[L::MT];
+#pragma unroll
+    for (int mt = 0; mt < L::MT; ++mt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      fb[mt] = mma(aw.get(mt, 0), op[k], acc);
+    }
+    spot_bin_block<N, 1, NRC>(fb, 0, lane, nr, o, npix, nullptr, k == 0 ? bin_a : bin_b);
   }
 }
 
@@ -698,17 +850,36 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
     unit[k] = unit_window(P, sk);
   }
   float of[Q][FieldIn<Q>::NP];
-  field_offsets<N, Q, FULL>(P, lane, of);
   FieldIn<Q> fin[2];
-  field_load<N, Q, FULL>(P, win[0], lane, fin[0]);
+  if (dft_pair_stage<N, Q, PREC, FULL, NRC>() && pdiam <= 8) {
+    // two sub-apertures per stage A (dft_spot_pair); each pair's screen values
+    // are loaded while the previous pair is computed
+    if constexpr (dft_pair_stage<N, Q, PREC, FULL, NRC>()) {
+      field_offsets_pair(P, lane, of);
+      field_load_pair(P, win[0], win[1], lane, fin[0]);
 #pragma unroll
-  for (int k = 0; k < SPW; ++k) {
-    if (s0 + k > last) break;  // uniform
-    if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], lane, fin[(k + 1) & 1]);
-    float inten_scale;
-    dft_spot<N, Q, PREC, FULL, NRC>(P, lane, hr, spots + k * spot_f, bm, nr_rt, aw, fin[k & 1],
-                                    of, win[k], unit[k], inten_scale);
-    if (lane == 0) scales[k] = inten_scale;
+      for (int k = 0; k < SPW; k += 2) {
+        if (s0 + k > last) break;  // uniform
+        if (k + 2 < SPW) field_load_pair(P, win[k + 2], win[k + 3], lane, fin[((k >> 1) + 1) & 1]);
+        float inten_scale;
+        dft_spot_pair<N, NRC>(P, lane, spots + k * spot_f, spots + (k + 1) * spot_f, aw,
+                              fin[(k >> 1) & 1], of, win[k], win[k + 1], unit[k], unit[k + 1],
+                              inten_scale);
+        if (lane == 0) scales[k] = scales[k + 1] = inten_scale;
+      }
+    }
+  } else {
+    field_offsets<N, Q, FULL>(P, lane, of);
+    field_load<N, Q, FULL>(P, win[0], lane, fin[0]);
+#pragma unroll
+    for (int k = 0; k < SPW; ++k) {
+      if (s0 + k > last) break;  // uniform
+      if (k + 1 < SPW) field_load<N, Q, FULL>(P, win[k + 1], lane, fin[(k + 1) & 1]);
+      float inten_scale;
+      dft_spot<N, Q, PREC, FULL, NRC>(P, lane, hr, spots + k * spot_f, bm, nr_rt, aw, fin[k & 1],
+                                      of, win[k], unit[k], inten_scale);
+      if (lane == 0) scales[k] = inten_scale;
+    }
   }
   wave_lds_sync();
   float* sp = spots + q * spot_f;
