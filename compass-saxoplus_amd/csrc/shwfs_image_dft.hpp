@@ -611,11 +611,12 @@ __device__ __forceinline__ void dft_spot(const ImageParams& P, const int lane, f
 // per register of stage B's operand gives a's operand (lanes 32-63 zero) and
 // b's (moved to lanes 0-31, lanes 32-63 zero): each that of dft_spot on the
 // sub-aperture alone, so both spots are the same bit for bit, for half the
-// field evaluations, stage A products and operand splits.
+// field evaluations, stage A products and operand splits.  FP16: each half
+// takes its window's field scale from the maximum over its own 32 lanes,
+// which is dft_spot's wave maximum over that window.
 template <int N, int Q, int PREC, bool FULL, int NRC>
 __host__ __device__ constexpr bool dft_pair_stage() {
-  return N == 16 && Q == 1 && PREC == SHWFS_METHOD_DFT_BF16X3 && !FULL && NRC >= 0 &&
-         (NRC & 7) > 0;
+  return N == 16 && Q == 1 && !FULL && NRC >= 0 && (NRC & 7) > 0;
 }
 
 // Offsets (in turns) at the lane's points of its pair row (row = lane & 7).
@@ -693,25 +694,68 @@ __device__ __forceinline__ void split_lane_halves(const Frag<SHWFS_METHOD_DFT_BF
   upper.lo = __builtin_bit_cast(bf16x8, l1);
 }
 
-// The binned spots of sub-apertures a and b (see above) into bin_a, bin_b.
-template <int N, int NRC>
+__device__ __forceinline__ void split_lane_halves(const Frag<SHWFS_METHOD_DFT_FP16>& f,
+                                                  Frag<SHWFS_METHOD_DFT_FP16>& lower,
+                                                  Frag<SHWFS_METHOD_DFT_FP16>& upper) {
+  typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
+  const u32x4v v = __builtin_bit_cast(u32x4v, f.v);
+  u32x4v v0, v1;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v[d], 0u, false, false);
+    v0[d] = r[0];
+    v1[d] = r[1];
+  }
+  lower.v = __builtin_bit_cast(f16x8, v0);
+  upper.v = __builtin_bit_cast(f16x8, v1);
+}
+
+// Maximum over the 32 lanes with the same lane bit 3 (one pair half).
+__device__ __forceinline__ float pair_half_max(float m) {
+  m = fmaxf(m, __shfl_xor(m, 1));
+  m = fmaxf(m, __shfl_xor(m, 2));
+  m = fmaxf(m, __shfl_xor(m, 4));
+  m = fmaxf(m, __shfl_xor(m, 16));
+  return fmaxf(m, __shfl_xor(m, 32));
+}
+
+// The binned spots of sub-apertures a and b (see above) into bin_a, bin_b,
+// in units of 1 / scale_a and 1 / scale_b.
+template <int N, int PREC, int NRC>
 __device__ __forceinline__ void dft_spot_pair(const ImageParams& P, const int lane, float* bin_a,
-                                              float* bin_b,
-                                              const AwOperand<N, 1, SHWFS_METHOD_DFT_BF16X3>& aw,
+                                              float* bin_b, const AwOperand<N, 1, PREC>& aw,
                                               const FieldIn<1>& fin,
                                               const float (&of)[1][FieldIn<1>::NP],
                                               const unsigned win_a, const unsigned win_b,
                                               const bool unit_a, const bool unit_b,
-                                              float& inten_scale) {
-  constexpr int PREC = SHWFS_METHOD_DFT_BF16X3;
+                                              float& scale_a, float& scale_b) {
   using L = DftLayout<N, 1>;
   using F = Frag<PREC>;
   static_assert(L::MT == 2 && L::KS == 1 && L::NTB == 1, "dft_spot_pair: the 16-point stage");
+  constexpr int t = table_exp<1, PREC>();
   float ev[1][1][8];
   {
     float pu[1][FieldIn<1>::NP];
     field_amplitude_pair(P, win_a, win_b, unit_a, unit_b, lane, pu);
     field_eval<N, 1, false>(P, lane, fin, of, pu, ev);
+  }
+  // FP16 field scale of each half (dft_spot's rule on its window alone)
+  int e_a = t, e_b = t;
+  if constexpr (PREC == SHWFS_METHOD_DFT_FP16) {
+    if (!unit_a || !unit_b) {  // uniform
+      float m = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(ev[0][0][e]));
+      const float amax = pair_half_max(m) * (float)P.pdiam;
+      const int ex = ((__float_as_int(amax) >> 23) & 0xff) - 127;  // ilogb for normal amax
+      const bool unit = (lane & 8) ? unit_b : unit_a;
+      const int e_lane = unit ? t : (amax >= 1.17549435e-38f ? 14 - ex : t);
+      const float fscale = __int_as_float((127 + e_lane - t) << 23);  // |e_lane - t| < 127
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ev[0][0][e] *= fscale;
+      e_a = __builtin_amdgcn_readlane(e_lane, 0);  // lane 0: half a
+      e_b = __builtin_amdgcn_readlane(e_lane, 8);  // lane 8: half b
+    }
   }
   F ae;
 #pragma unroll
@@ -722,7 +766,8 @@ __device__ __forceinline__ void dft_spot_pair(const ImageParams& P, const int la
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     ga[nt] = mma(ae, aw.get(nt, 0), acc);
   }
-  inten_scale = ldexpf(1.0f, -2 * (2 * table_exp<1, PREC>()));
+  scale_a = ldexpf(1.0f, -2 * (e_a + t));
+  scale_b = ldexpf(1.0f, -2 * (e_b + t));
   const int o = P.bin_origin, npix = P.npix;
   const int nr = NRC & 7;
   wave_lds_sync();
@@ -861,11 +906,14 @@ __device__ __forceinline__ void dft_subaps(const ImageParams& P, const int s0, c
       for (int k = 0; k < SPW; k += 2) {
         if (s0 + k > last) break;  // uniform
         if (k + 2 < SPW) field_load_pair(P, win[k + 2], win[k + 3], lane, fin[((k >> 1) + 1) & 1]);
-        float inten_scale;
-        dft_spot_pair<N, NRC>(P, lane, spots + k * spot_f, spots + (k + 1) * spot_f, aw,
-                              fin[(k >> 1) & 1], of, win[k], win[k + 1], unit[k], unit[k + 1],
-                              inten_scale);
-        if (lane == 0) scales[k] = scales[k + 1] = inten_scale;
+        float scale_a, scale_b;
+        dft_spot_pair<N, PREC, NRC>(P, lane, spots + k * spot_f, spots + (k + 1) * spot_f, aw,
+                                    fin[(k >> 1) & 1], of, win[k], win[k + 1], unit[k],
+                                    unit[k + 1], scale_a, scale_b);
+        if (lane == 0) {
+          scales[k] = scale_a;
+          scales[k + 1] = scale_b;
+        }
       }
     }
   } else {
