@@ -53,3 +53,50 @@ def test_streams_are_separated_by_counter_fields():
     for other in (op.block(np.arange(64), 1, 0, 0, 1), op.block(np.arange(64), 0, 1, 0, 1),
                   op.block(np.arange(64), 0, 0, 1, 1), op.block(np.arange(64), 0, 0, 0, 2)):
         assert not np.any(np.all(a == other, axis=-1))
+
+
+def _device_ptrs_thresholds():
+    """The integer thresholds the device sampler (rng_poisson_f in
+    shwfs_device.hpp) uses in place of the oracle's float64 tests
+    us >= 0.07 and us < 0.013, read from the source."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "compass-saxoplus_amd", "csrc",
+                            "shwfs_device.hpp")).read()
+    hi = re.search(r"if \(m >= (\d+)u\) \{\s*// us >= 0\.07", src)
+    lo = re.search(r"\(m <= (\d+)u && wv > m\)\) continue;\s*// us < 0\.013 && V > us", src)
+    assert hi and lo, "threshold lines not found in shwfs_device.hpp"
+    return int(hi.group(1)), int(lo.group(1))
+
+
+def _oracle_us(wu):
+    # the oracle's arithmetic (philox._poisson_one): U = uniform - 0.5, us = 0.5 - |U|
+    U = op.uniform(np.asarray(wu, dtype=np.uint64)) - 0.5
+    return 0.5 - np.abs(U)
+
+
+def test_ptrs_integer_tests_equal_the_oracle_float_tests():
+    """us = (min(wu, ~wu) + 1/2) 2^-32 exactly, so the oracle's tests
+    us >= 0.07, us < 0.013 and V > us are integer tests on the Philox words:
+    checked for random words and for every word within 4 of each threshold
+    (both halves of the word range), against the oracle's float64 values."""
+    hi, lo = _device_ptrs_thresholds()
+    rng = np.random.default_rng(5)
+    words = rng.integers(0, 2 ** 32, 200000, dtype=np.uint64)
+    near = []
+    for t in (hi, lo):
+        for d in range(-4, 5):
+            m = t + d
+            near += [m, 2 ** 32 - 1 - m]  # wu below and above 2^31 with min(wu, ~wu) = m
+    words = np.concatenate([words, np.array(near, dtype=np.uint64)])
+    us = _oracle_us(words)
+    m = np.minimum(words, np.uint64(2 ** 32 - 1) - words)
+    np.testing.assert_array_equal(us, (m.astype(np.float64) + 0.5) * 2.0 ** -32)
+    np.testing.assert_array_equal(us >= 0.07, m >= hi)
+    np.testing.assert_array_equal(us < 0.013, m <= lo)
+    wv = rng.integers(0, 2 ** 32, words.size, dtype=np.uint64)
+    wv[-len(near):] = m[-len(near):]  # V == us exactly at the threshold words
+    wv[:1000] = m[:1000]              # and at some random ones
+    wv[1000:2000] = m[1000:2000] + 1  # one step above
+    V = op.uniform(wv)
+    np.testing.assert_array_equal(V > us, wv > m)
