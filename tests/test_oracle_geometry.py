@@ -14,9 +14,17 @@ def test_small_has_52_valid_subaps():
     assert c["p_wfs"]._nvalid == 52
 
 
-@pytest.mark.parametrize("name", ["small", "noisy"])
+def _config(name):
+    # named configs, or "batch<i>": WFS i of the uneven eight-WFS batch
+    if name.startswith("batch"):
+        return configs.batch()[int(name[5:])]
+    return configs.get(name)
+
+
+# the 80x80 WFS and the three geometries of the uneven batch (74x74, 40x40, 8x8)
+@pytest.mark.parametrize("name", ["small", "noisy", "large", "batch0", "batch6", "batch7"])
 def test_product_geometry_matches_oracle(name):
-    c = system.geometry(configs.get(name))
+    c = system.geometry(_config(name))
     w, g = c["p_wfs"], c["p_geom"]
     o = og.sh_geometry(g.pupdiam, w.nxsub, w._pdiam, w.npix, w._nrebin, w._Nfft, w._Ntot,
                        w.fracsub, c["p_tel"].cobs)
