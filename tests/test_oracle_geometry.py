@@ -84,6 +84,27 @@ def test_fft_goodsize_is_strictly_next_power_of_two():
     assert [geom.fft_goodsize(s) for s in (15, 16, 17, 31, 32)] == [16, 32, 32, 32, 64]
 
 
+def test_sizing_rule_known_answers():
+    """The sizing rule worked by hand (lambda 0.5 um, 0.5 m sub-apertures,
+    1 rad = 206264.8 arcsec):
+    * pdiam 16, 0.3 arcsec pixels: first rebin int(0.3 / 0.10313) + 1 = 3,
+      Nfft = goodsize(int(32 * 3 / 0.3 * 0.10313)) = goodsize(33) = 64,
+      quantum pixel (16 * 0.5e-6 / 0.5 rad) / 64 = 3.30024 / 64 = 0.0515662,
+      ratio 5.82 -> 6 (fraction above 1/2), pixel 0.309397; 8 pixels
+      (2.48 arcsec) fit in 64 quantum pixels: Ntot = Nfft;
+    * pdiam 8, 1 arcsec pixels: first rebin 10, Nfft = goodsize(16) = 32,
+      quantum pixel 1.65012 / 32 = 0.0515662 again, ratio 19.39 -> 19,
+      pixel 0.979758; 8 pixels span 152 quantum pixels: Ntot =
+      goodsize(153) = 256."""
+    a = og.wfs_size(8.0, 16, 16, 8, 0.3, 0.5)
+    assert (a["Nfft"], a["Ntot"], a["nrebin"]) == (64, 64, 6)
+    assert a["pixsize"] == pytest.approx(0.309397, abs=2e-6)
+    b = og.wfs_size(8.0, 16, 8, 8, 1.0, 0.5)
+    assert (b["Nfft"], b["Ntot"], b["nrebin"]) == (32, 256, 19)
+    assert b["pixsize"] == pytest.approx(0.979758, abs=2e-6)
+    assert a["qpixsize"] == pytest.approx(0.0515662, abs=2e-7) == b["qpixsize"]
+
+
 def test_sizing_rule_matches_oracle():
     class W:
         pass
