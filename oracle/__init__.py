@@ -33,7 +33,7 @@ independent check.
 """
 
 PINNING = {
-    "geometry.wfs_size": "pinned: tests/test_oracle_geometry.py (stated Nfft/nrebin/Ntot of the configs)",
+    "geometry.wfs_size": "pinned: tests/test_oracle_geometry.py (two hand-worked cases, Ntot = Nfft and Ntot > Nfft; stated Nfft/nrebin/Ntot of the configs)",
     "geometry.sh_geometry": "pinned: c1 known answer (52 valid), closed-form halfxy, "
                             "binmap injection/fftshift invariants, brute-force recount",
     "wfs.comp_image": "pinned: brute-force DFT, Parseval, flux normalisation, Fourier-shift "
@@ -47,7 +47,7 @@ PINNING = {
     "centroid.cog/wcog/tcog": "pinned: scipy.ndimage.center_of_mass (as the reference tests do), "
                               "threshold/epsilon special cases (tests/test_oracle_centroid.py)",
     "philox.philox4x32": "pinned: Random123 known-answer vectors (tests/test_oracle_philox.py)",
-    "philox.poisson/normal": "pinned: mean/variance/KS tests against scipy.stats",
+    "philox.poisson/normal": "pinned: mean/variance/KS tests against scipy.stats; the integer forms of the PTRS uniform tests checked at every threshold (test_oracle_philox.py)",
     "turbulence.structure_function": "pinned: Kolmogorov limit 6.88 (r/r0)^(5/3) and "
                                      "saturation 2*variance for r >> L0",
     "mlp": "pinned: central finite differences of the float64 loss",
