@@ -87,10 +87,11 @@ typedef struct shwfs_sh_geom {
   /* Detector-image pixels outside every spot window (index x + y * npix * nxsub),
    * or NULL.  The reference draws the readout background on the whole binimg and
    * then overwrites the spot windows (SutraWfsSH::fill_binimage, sutra_wfs_sh.cpp:
-   * 599-621); with this list a noisy frame of the FFT or DFT spot kernel draws it
-   * on these pixels only, by extra workgroups of the spot launch (the same draws,
-   * element = image index, Philox domain 2).  NULL: a launch of its own draws it
-   * on every pixel ahead of the spots. */
+   * 599-621); with this list a noisy frame of the FFT, DFT or fused learned-
+   * centroider spot kernel draws it on these pixels only, by extra workgroups of
+   * the spot launch (the same draws, element = image index, Philox domain 2).
+   * NULL, and the extended-grid and traced spot kernels: a launch of its own draws
+   * it on every pixel ahead of the spots. */
   const int32_t* d_bgpix;      /* [nbgpix], each index < (npix * nxsub)^2 */
   int32_t nbgpix;
 } shwfs_sh_geom;
