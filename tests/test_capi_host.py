@@ -36,6 +36,16 @@ def test_version_and_error_reporting(lib):
     g.ntot = 128  # beyond the enlarged-grid kernel
     assert lib.shwfs_sh_check_geom(C.byref(g), 0) == capi.SHWFS_ERR_UNSUPPORTED
     assert lib.shwfs_sh_check_geom(None, 0) == capi.SHWFS_ERR_ARG
+    # the background pixel list: its length within [0, (npix * nxsub)^2]
+    # (the pointer is not dereferenced by the check)
+    g.ntot = 16
+    g.d_bgpix = 256
+    for n, rc in ((-1, capi.SHWFS_ERR_ARG), ((6 * 8) ** 2 + 1, capi.SHWFS_ERR_ARG),
+                  (0, capi.SHWFS_ERR_STATE), ((6 * 8) ** 2, capi.SHWFS_ERR_STATE)):
+        g.nbgpix = n
+        assert lib.shwfs_sh_check_geom(C.byref(g), 0) == rc, n
+        if rc == capi.SHWFS_ERR_ARG:
+            assert "background" in capi.last_error(lib)
 
 
 def test_null_and_size_arguments_rejected_without_launch(lib):
