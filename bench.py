@@ -331,8 +331,16 @@ def measure_learned(graph_frames=20, reps=10):
         errs[key] = {k: round(float(np.sqrt(np.mean((v - t)[keep] ** 2))), 4) for k, v in est.items()}
     # fused spots + network, noisy frames
     slopes = s.rtc.d_control[0].d_centroids  # a 2 * nvalid slope vector
-    ms = sutra.run_frames_graph(s.wfs, lc.mlp, slopes, graph_frames, reps, True)
+    # fused frame per spot method (FFT, split-BF16x3, FP16 spots + network in
+    # one launch); the headline figure is split-BF16x3's, FP32-accurate spots
     frames = graph_frames * reps
+    by_method = {}
+    for m in (0, 1, 2):
+        s.wfs.set_method(m)
+        ms_m = sutra.run_frames_graph(s.wfs, lc.mlp, slopes, graph_frames, reps, True)
+        by_method[METHOD_NAMES[m]] = round(1e3 * ms_m / frames, 3)
+    s.wfs.set_method(0)
+    ms = by_method["dft_bf16x3"] * frames * 1e-3
     return dict(workload="40x40 SH (1256 sub-apertures), pdiam 16, Nfft 32, 8x8 px, 50 ph/subap "
                          "Poisson + 3 e- readout; MLP 64-128-128-2 GELU, bf16 MFMA",
                 pairs=int(n), generate_s=round(gen_s, 2), train_s=round(train_s, 2),
@@ -340,7 +348,8 @@ def measure_learned(graph_frames=20, reps=10):
                 heldout_pairs=int(ne), slope_rms_err_arcsec=errs,
                 tcog_threshold_e=9.0, wcog_weights="Gaussian FWHM 4 px",
                 fused_frame_us=round(1e3 * ms / frames, 3),
-                fused_frames_per_s=round(frames / (ms * 1e-3), 1))
+                fused_frames_per_s=round(frames / (ms * 1e-3), 1),
+                fused_frame_spot_method="dft_bf16x3", fused_frame_us_by_method=by_method)
 
 
 def oracle_frame(s, phase, thr):

@@ -26,7 +26,7 @@ HIP_SOURCES = ["shwfs_capi.hip", "shwfs_image_fft.hip", "shwfs_image_dft.hip",
                "shwfs_dft_fp16_q2.hip", "shwfs_batch.hip", "shwfs_batch_quiet.hip",
                "shwfs_batch_noisy.hip", "shwfs_mlp.hip", "shwfs_raytrace.hip",
                "shwfs_image_ext.hip", "shwfs_batch_groups.hip", "shwfs_corr.hip",
-               "shwfs_field_stop.hip", "shwfs_lgs.hip"]
+               "shwfs_field_stop.hip", "shwfs_lgs.hip", "shwfs_dft_mlp.hip"]
 CPP_SOURCES = ["carma/carma.cpp", "sutra/sutra.cpp", "sutra/sutra_centroider.cpp",
                "sutra/sutra_batch.cpp", "sutra/sutra_centroider_mlp.cpp"]
 

@@ -485,9 +485,12 @@ int32_t image_params(const shwfs_sh_geom* g, const shwfs_frame_args* f, ImagePar
     if (c->kind < SHWFS_CENTROID_COG || c->kind > SHWFS_CENTROID_MLP)
       return fail(SHWFS_ERR_ARG, "unknown centroider kind");
     if (c->kind == SHWFS_CENTROID_MLP) {
-      if (f->method != SHWFS_METHOD_FFT_FP32)
+      if (f->method != SHWFS_METHOD_FFT_FP32 &&
+          !(g->nfft == 32 && g->nphase == 16 && g->npix == 8 && g->nrebin == 4 &&
+            g->binmap_block && g->bin_origin == 16))
         return fail(SHWFS_ERR_UNSUPPORTED,
-                    "fused learned centroider: runs in the FFT spot kernel only (method 0)");
+                    "fused learned centroider: an MFMA method needs nfft 32, 16-point windows "
+                    "and 8x8 pixels binned 4x4 about the grid centre (else the FFT method)");
       if (g->npix != 8 || (g->nfft != 16 && g->nfft != 32))
         return fail(SHWFS_ERR_UNSUPPORTED,
                     "fused learned centroider: needs 8x8-pixel spots and nfft 16 or 32");
@@ -646,6 +649,8 @@ static int32_t comp_image_on(const shwfs_sh_geom* g, const shwfs_frame_args* f, 
     const shwfs_centroid_args* c = f->centroid;
     const MlpParams m{c->d_mlp_params, c->d_mlp_fallback, c->mlp_x_scale, c->mlp_y_scale,
                       c->mlp_flux_lo, c->mlp_flux_hi};
+    if (f->method != SHWFS_METHOD_FFT_FP32)
+      return launch_image_dft_mlp(p, m, g->nfft, f->method, st, wtab, bg);
     return launch_image_fft_mlp(p, m, g->nfft, st, bg);
   }
   if (f->method == SHWFS_METHOD_FFT_FP32) return launch_image_fft(p, g->nfft, st, bg);

@@ -789,8 +789,9 @@ __device__ __forceinline__ void dft_spot_pair(const ImageParams& P, const int la
 // Spot, detector pixels and centroid of sub-aperture s by one wave.  smem: the
 // wave's LDS slot (wave_floats): the spot sits at its start (nr > 0) or end
 // (nr == 0), the moment totals use the 192 doubles after the spot (nr > 0)
-// or at the start, over the dead |F|^2 image (nr == 0).
-template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1>
+// or at the start, over the dead |F|^2 image (nr == 0).  KEEP: no centroider;
+// the spot-cube values stay in the spot.
+template <int N, int Q, int PREC, bool NOISE, bool FULL, int NRC = -1, bool KEEP = false>
 __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, const int lane,
                                           float* smem, const int* bm, const int nr_rt,
                                           const AwOperand<N, Q, PREC>& aw,
@@ -815,7 +816,7 @@ __device__ __forceinline__ void dft_subap(const ImageParams& P, const int s, con
   wave_lds_sync();
   const float I = spot_sum<64>(bin, npix * npix, true, lane);
   void* red = nr > 0 ? smem + spot_f : smem;
-  finish_spot<64, NOISE>(P, s, true, lane, bin, I, inten_scale, call_add, red, &pre);
+  finish_spot<64, NOISE, KEEP>(P, s, true, lane, bin, I, inten_scale, call_add, red, &pre);
   wave_lds_sync();
 }
 

@@ -201,6 +201,11 @@ int32_t launch_image_fft_ray(const ImageParams& p, const RayParams& r, int nfft,
 // SHWFS_CENTROID_MLP): 16 sub-apertures per workgroup, npix 8, nfft 16 or 32.
 int32_t launch_image_fft_mlp(const ImageParams& p, const MlpParams& m, int nfft, hipStream_t s,
                              const BgParams& bg = {});
+// The same after MFMA DFT spots (shwfs_dft_mlp.hip) for the shapes
+// dft_mlp_fits accepts (nfft 32, pdiam 16, 8x8 px, half-grid 4x4 binning).
+bool dft_mlp_fits(const ImageParams& p, int nfft);
+int32_t launch_image_dft_mlp(const ImageParams& p, const MlpParams& m, int nfft, int method,
+                             hipStream_t s, const void* wtab, const BgParams& bg);
 // wtab: NULL (the library's operand table of the shape, built and cached on
 // first use) or a table built by build_dft_operand_table.
 int32_t launch_image_dft(const ImageParams& p, int nfft, int method, hipStream_t s,

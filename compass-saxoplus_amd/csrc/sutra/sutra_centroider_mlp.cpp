@@ -176,9 +176,11 @@ int32_t SutraCentroiderMlp<Tin, T>::get_cog() {
   return get_cog(this->wfs->d_intensities->get_data(), this->wfs->d_slopes->get_data(), true);
 }
 
-// Fused after binning (kind SHWFS_CENTROID_MLP) when the attached WFS
-// computes its spots with the FFT method, 8x8-pixel spots and nfft 16 or 32;
-// otherwise NONE (comp_frame runs the spots, then get_cog).
+// Fused after binning (kind SHWFS_CENTROID_MLP) when the attached WFS has
+// 8x8-pixel spots and either computes them with the FFT method at nfft 16 or
+// 32, or with an MFMA method at nfft 32 from 16-point windows binned 4x4 in
+// the half-grid form (block binmap, origin 16); otherwise NONE (comp_frame
+// runs the spots, then get_cog).
 template <class Tin, class T>
 shwfs_centroid_args SutraCentroiderMlp<Tin, T>::centroid_args(float* intensities, T* slopes,
                                                               int32_t nvalid, int32_t npix,
@@ -186,9 +188,11 @@ shwfs_centroid_args SutraCentroiderMlp<Tin, T>::centroid_args(float* intensities
   shwfs_centroid_args a{};
   a.kind = SHWFS_CENTROID_NONE;
   auto* sh = dynamic_cast<SutraWfsSH*>(this->wfs);
-  if (!sh || sh->get_method() != SHWFS_METHOD_FFT_FP32 || npix != 8 ||
-      (sh->nfft != 16 && sh->nfft != 32) || d_fallback->get_nb_elements() < nvalid)
-    return a;
+  if (!sh || npix != 8 || d_fallback->get_nb_elements() < nvalid) return a;
+  const bool fft = sh->get_method() == SHWFS_METHOD_FFT_FP32 && (sh->nfft == 16 || sh->nfft == 32);
+  const bool mfma = sh->get_method() != SHWFS_METHOD_FFT_FP32 && sh->nfft == 32 &&
+                    sh->nphase == 16 && sh->nrebin == 4 && sh->bin_block && sh->bin_origin == 16;
+  if (!fft && !mfma) return a;
   a.kind = SHWFS_CENTROID_MLP;
   a.nvalid = nvalid;
   a.npix = npix;

@@ -114,9 +114,11 @@ typedef struct shwfs_centroid_args {
   float* d_intensities;    /* [nvalid] centroider flux */
   int32_t slope_order;     /* 0 untied (x..., y...), 1 interlaced (x, y, x, y...) */
   int32_t fp64_moments;    /* 1: flux and first moments summed in float64 */
-  /* SHWFS_CENTROID_MLP only (fused into the spot kernel of the FFT method;
-   * 8x8-pixel spots, nfft 16 or 32): as shwfs_mlp_centroids over the spot
-   * cube, in the same launch as the spots.  Other kinds ignore these. */
+  /* SHWFS_CENTROID_MLP only (fused into the spot kernel: the FFT method with
+   * 8x8-pixel spots at nfft 16 or 32, or an MFMA method at nfft 32 with
+   * 16-point windows and 8x8 pixels binned 4x4 about the grid centre): as
+   * shwfs_mlp_centroids over the spot cube, in the same launch as the spots.
+   * Other kinds ignore these. */
   const float* d_mlp_params; /* [SHWFS_MLP_NPARAM] */
   float mlp_x_scale;         /* input scale of the spot pixels */
   float mlp_y_scale;         /* network output (pixels) -> slope units */

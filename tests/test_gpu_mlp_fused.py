@@ -9,7 +9,7 @@ stand-alone forms, through the C++ layer and the C ABI:
   entry on the same spots times the pixel scale, bit for bit.
 
 Workload: 40x40 SH (pdiam 16, Nfft 32, 8x8-pixel spots), 50 photons per
-sub-aperture, Poisson + 3 e- readout noise, FFT spot method."""
+sub-aperture, Poisson + 3 e- readout noise; FFT, split-BF16x3 and FP16 spots."""
 import ctypes as C
 
 import numpy as np
@@ -55,8 +55,20 @@ def _standalone_on_cube(lc, cube_ptr, nvalid, lo, hi):
     return np.array(slopes).ravel(), np.array(inten).ravel(), np.array(fb).ravel()
 
 
-def test_fused_frame_equals_spots_then_network(lc):
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_fused_frame_equals_spots_then_network(lc, method):
+    """FFT spots (method 0) or MFMA DFT spots (1: split-BF16x3, 2: FP16; the
+    eight-sub-aperture kernel of shwfs_dft_mlp.hip) with the network after
+    them, in one launch."""
     s, m = lc.sys, lc.mlp
+    s.wfs.set_method(method)
+    try:
+        _fused_frame_checks(lc, s, m)
+    finally:
+        s.wfs.set_method(0)
+
+
+def _fused_frame_checks(lc, s, m):
     nv = s.wfs.nvalid
     lo, hi = 20.0, 80.0  # some spots outside: their slopes are the CoG of the same pixels
     m.set_flux_range(lo, hi)
